@@ -1,0 +1,476 @@
+// C ABI of libbskhip: plans, forward transform, shell fields, triple sums.
+// Every data buffer is caller-owned device memory; plans own only their small
+// constant tables (twiddles, shell thresholds, window compensation).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/bskhip.h"
+#include "launch.h"
+
+static thread_local char g_err[512];
+
+void bsk_set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+struct bsk_plan {
+  int N;
+  double box;
+  int nbin;
+  std::vector<int> n2lo, n2hi;
+  int m;
+  Geom g;
+  int variant[6];
+  int* d_lo = nullptr;
+  int* d_hi = nullptr;
+  cplx<float>* tw_f = nullptr;
+  cplx<double>* tw_d = nullptr;
+  float* comp_f = nullptr;
+  double* comp_d = nullptr;
+};
+
+namespace {
+
+const double kTieRtol = 1e-9;
+
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+template <typename T>
+int forward_t(bsk_plan* p, const T* field, cplx<T>* dk, void* work, hipStream_t s,
+              const cplx<T>* tw, const T* comp) {
+  const int N = p->N;
+  const Geom g = p->g;
+  cplx<T>* e1 = reinterpret_cast<cplx<T>*>(work);
+  cplx<T>* e2 = e1 + (size_t)N * N * g.MZ;
+  int rc = bsk_fwd_z<T>(N, p->variant[0], field, e1, tw, (long long)N * N, g.MZ, s);
+  if (rc) return rc;
+  rc = bsk_fwd_y<T>(N, p->variant[1], e1, e2, tw, g, s);
+  if (rc) return rc;
+  const T scale = (T)(1.0 / ((double)N * N * N));
+  return bsk_fwd_x<T>(N, p->variant[2], e2, dk, tw, g, scale, comp, s);
+}
+
+template <typename T>
+int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int x0, int xs,
+             int stages, int fmt, void* out, void* out_lo, long long stride, long long cell0,
+             void* work, hipStream_t s, const cplx<T>* tw) {
+  const int N = p->N;
+  const Geom g = p->g;
+  cplx<T>* d1 = reinterpret_cast<cplx<T>*>(work);
+  cplx<T>* d2 = d1 + (size_t)nsh * g.MY * N * g.MZ;
+  const T kf = (T)(2.0 * M_PI / p->box);
+  int rc = BSK_OK;
+  if (stages & BSK_STAGE_X) {
+    rc = bsk_inv_x<T>(N, p->variant[3], dk, d1, tw, g, p->d_lo, p->d_hi, shell0, nsh, kind, kf, s);
+    if (rc) return rc;
+  }
+  if (!(stages & BSK_STAGE_YZ)) return BSK_OK;
+  rc = bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, nsh, s);
+  if (rc) return rc;
+  // shells [shell0, shell0+nsh) of the output arrays, cells of the slab
+  const size_t esz = (fmt == BSK_FMT_F64) ? 8 : (fmt == BSK_FMT_F32 ? 4 : 2);
+  char* o = reinterpret_cast<char*>(out) + (size_t)shell0 * stride * esz;
+  char* ol = out_lo ? reinterpret_cast<char*>(out_lo) + (size_t)shell0 * stride * esz : nullptr;
+  return bsk_inv_z<T>(N, p->variant[5], d2, o, ol, tw, g.MZ, (long long)xs * N, cell0, stride, fmt,
+                      nsh, s);
+}
+
+int check_plan(const bsk_plan* p) {
+  if (!p) {
+    bsk_set_error("null plan");
+    return BSK_EINVAL;
+  }
+  return BSK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bsk_last_error(void) { return g_err; }
+
+int bsk_version(void) { return BSK_ABI_VERSION; }
+
+int bsk_shell_thresholds(int nmesh, double box_size, int nbin, const double* k_low,
+                         const double* k_high, int* n2lo, int* n2hi) {
+  if (nmesh <= 0 || box_size <= 0 || nbin < 0 || (nbin && (!k_low || !k_high || !n2lo || !n2hi))) {
+    bsk_set_error("bsk_shell_thresholds: invalid argument");
+    return BSK_EINVAL;
+  }
+  const double kf = 2.0 * M_PI / box_size;
+  const double n2max = 3.0 * (nmesh / 2.0) * (nmesh / 2.0);
+  for (int b = 0; b < nbin; ++b) {
+    const double a = k_low[b] > 0 ? (k_low[b] / kf) * (k_low[b] / kf) : 0.0;
+    const double c = k_high[b] >= 0 ? (k_high[b] / kf) * (k_high[b] / kf) : -1.0;
+    double lo = a > 0 ? ceil(a * (1.0 - kTieRtol)) : 0.0;
+    double hi = c >= 0 ? floor(c * (1.0 + kTieRtol)) : -1.0;
+    if (lo > n2max + 1) lo = n2max + 1;
+    if (hi > n2max) hi = n2max;
+    n2lo[b] = (int)lo;
+    n2hi[b] = (int)hi;
+  }
+  return BSK_OK;
+}
+
+int bsk_plan_create(bsk_plan** out, int nmesh, double box_size, int nbin, const double* k_low,
+                    const double* k_high) {
+  if (!out || nbin <= 0 || !k_low || !k_high || box_size <= 0) {
+    bsk_set_error("bsk_plan_create: invalid argument");
+    return BSK_EINVAL;
+  }
+  *out = nullptr;
+  if (!bsk_fft_size_supported(nmesh)) {
+    bsk_set_error("bsk_plan_create: Nmesh=%d is not a supported FFT size "
+                  "(16,32,48,64,96,128,192,256,384,512,768)", nmesh);
+    return BSK_EUNSUPPORTED;
+  }
+  bsk_plan* p = new bsk_plan();
+  p->N = nmesh;
+  p->box = box_size;
+  p->nbin = nbin;
+  p->n2lo.resize(nbin);
+  p->n2hi.resize(nbin);
+  int rc = bsk_shell_thresholds(nmesh, box_size, nbin, k_low, k_high, p->n2lo.data(), p->n2hi.data());
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  int n2max = 0;
+  for (int b = 0; b < nbin; ++b) n2max = std::max(n2max, p->n2hi[b]);
+  int m = (int)floor(sqrt((double)n2max));
+  while ((long long)(m + 1) * (m + 1) <= n2max) ++m;
+  while ((long long)m * m > n2max) --m;
+  p->m = m;
+  const int N = nmesh;
+  Geom g;
+  g.N = N;
+  if (2 * m + 1 >= N) {
+    g.MX = g.MY = N;
+    g.nlo = -N / 2;
+  } else {
+    g.MX = g.MY = 2 * m + 1;
+    g.nlo = -m;
+  }
+  g.MZ = std::min(m + 1, N / 2 + 1);
+  p->g = g;
+  for (int i = 0; i < 6; ++i) p->variant[i] = 0;
+
+  std::vector<cplx<float>> twf(N);
+  std::vector<cplx<double>> twd(N);
+  for (int t = 0; t < N; ++t) {
+    const double a = -2.0 * M_PI * (double)t / (double)N;
+    twd[t] = {cos(a), sin(a)};
+    twf[t] = {(float)twd[t].x, (float)twd[t].y};
+  }
+  hipError_t e = hipSuccess;
+  e = hipMalloc(&p->d_lo, sizeof(int) * nbin);
+  if (e == hipSuccess) e = hipMalloc(&p->d_hi, sizeof(int) * nbin);
+  if (e == hipSuccess) e = hipMalloc(&p->tw_f, sizeof(cplx<float>) * N);
+  if (e == hipSuccess) e = hipMalloc(&p->tw_d, sizeof(cplx<double>) * N);
+  if (e == hipSuccess) e = hipMemcpy(p->d_lo, p->n2lo.data(), sizeof(int) * nbin, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->d_hi, p->n2hi.data(), sizeof(int) * nbin, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->tw_f, twf.data(), sizeof(cplx<float>) * N, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->tw_d, twd.data(), sizeof(cplx<double>) * N, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    bsk_set_error("bsk_plan_create: %s", hipGetErrorString(e));
+    bsk_plan_destroy(p);
+    return e == hipErrorOutOfMemory ? BSK_ENOMEM : BSK_EHIP;
+  }
+  *out = p;
+  return BSK_OK;
+}
+
+int bsk_plan_destroy(bsk_plan* p) {
+  if (!p) return BSK_OK;
+  (void)hipFree(p->d_lo);
+  (void)hipFree(p->d_hi);
+  (void)hipFree(p->tw_f);
+  (void)hipFree(p->tw_d);
+  (void)hipFree(p->comp_f);
+  (void)hipFree(p->comp_d);
+  delete p;
+  return BSK_OK;
+}
+
+int bsk_plan_geometry(const bsk_plan* p, int* m, int* mx, int* my, int* mz, int* nlo) {
+  if (int rc = check_plan(p)) return rc;
+  if (m) *m = p->m;
+  if (mx) *mx = p->g.MX;
+  if (my) *my = p->g.MY;
+  if (mz) *mz = p->g.MZ;
+  if (nlo) *nlo = p->g.nlo;
+  return BSK_OK;
+}
+
+int bsk_plan_thresholds(const bsk_plan* p, int* n2lo, int* n2hi) {
+  if (int rc = check_plan(p)) return rc;
+  for (int b = 0; b < p->nbin; ++b) {
+    if (n2lo) n2lo[b] = p->n2lo[b];
+    if (n2hi) n2hi[b] = p->n2hi[b];
+  }
+  return BSK_OK;
+}
+
+int bsk_plan_set_variant(bsk_plan* p, int pass, int variant) {
+  if (int rc = check_plan(p)) return rc;
+  if (pass < 0 || pass >= 6 || variant < 0 || variant > 1) {
+    bsk_set_error("bsk_plan_set_variant: pass in 0..5, variant in 0..1");
+    return BSK_EINVAL;
+  }
+  p->variant[pass] = variant;
+  return BSK_OK;
+}
+
+int bsk_plan_set_compensation(bsk_plan* p, int nmesh_window) {
+  if (int rc = check_plan(p)) return rc;
+  (void)hipFree(p->comp_f);
+  (void)hipFree(p->comp_d);
+  p->comp_f = nullptr;
+  p->comp_d = nullptr;
+  if (nmesh_window <= 0) return BSK_OK;
+  const int N = p->N;
+  std::vector<float> cf(N + 1);
+  std::vector<double> cd(N + 1);
+  for (int n = -N / 2; n <= N / 2; ++n) {
+    const double sn = sin(M_PI * (double)n / (double)nmesh_window);
+    cd[n + N / 2] = 1.0 / sqrt(1.0 - 2.0 / 3.0 * sn * sn);
+    cf[n + N / 2] = (float)cd[n + N / 2];
+  }
+  BSK_HIP_CHECK(hipMalloc(&p->comp_f, sizeof(float) * (N + 1)));
+  BSK_HIP_CHECK(hipMalloc(&p->comp_d, sizeof(double) * (N + 1)));
+  BSK_HIP_CHECK(hipMemcpy(p->comp_f, cf.data(), sizeof(float) * (N + 1), hipMemcpyHostToDevice));
+  BSK_HIP_CHECK(hipMemcpy(p->comp_d, cd.data(), sizeof(double) * (N + 1), hipMemcpyHostToDevice));
+  return BSK_OK;
+}
+
+int bsk_forward_workspace(const bsk_plan* p, int dtype, size_t* bytes) {
+  if (int rc = check_plan(p)) return rc;
+  const size_t c = dtype == BSK_DT_F64 ? 16 : 8;
+  const size_t N = p->N;
+  *bytes = c * (N * N * p->g.MZ + N * p->g.MY * p->g.MZ);
+  return BSK_OK;
+}
+
+int bsk_forward(bsk_plan* p, int dtype, const void* field, void* dk, void* work, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!field || !dk || !work) {
+    bsk_set_error("bsk_forward: null buffer");
+    return BSK_EINVAL;
+  }
+  if (dtype == BSK_DT_F32)
+    return forward_t<float>(p, (const float*)field, (cplx<float>*)dk, work, as_stream(stream),
+                            p->tw_f, p->comp_f);
+  if (dtype == BSK_DT_F64)
+    return forward_t<double>(p, (const double*)field, (cplx<double>*)dk, work, as_stream(stream),
+                             p->tw_d, p->comp_d);
+  bsk_set_error("bsk_forward: dtype must be BSK_DT_F32 or BSK_DT_F64");
+  return BSK_EINVAL;
+}
+
+int bsk_shells_workspace(const bsk_plan* p, int dtype, int nshell, int xslab, size_t* bytes) {
+  if (int rc = check_plan(p)) return rc;
+  const size_t c = dtype == BSK_DT_F64 ? 16 : 8;
+  const size_t N = p->N;
+  *bytes = c * (size_t)nshell * p->g.MZ * ((size_t)p->g.MY * N + (size_t)xslab * N);
+  return BSK_OK;
+}
+
+int bsk_shells(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int nshell, int x0,
+               int xslab, int stages, int fmt, void* out, void* out_lo, long long shell_stride,
+               long long out_cell0, void* work, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  const long long slab_cells = (long long)xslab * p->N * p->N;
+  const bool yz = (stages & BSK_STAGE_YZ) != 0;
+  if (nshell <= 0 || shell0 < 0 || shell0 + nshell > p->nbin || x0 < 0 || xslab <= 0 ||
+      x0 + xslab > p->N || !work || stages < 1 || stages > 3 ||
+      ((stages & BSK_STAGE_X) && kind == BSK_KIND_DELTA && !dk) ||
+      (kind < BSK_KIND_DELTA || kind > BSK_KIND_KMAG) ||
+      (yz && (!out || out_cell0 < 0 || shell_stride < out_cell0 + slab_cells))) {
+    bsk_set_error("bsk_shells: invalid argument");
+    return BSK_EINVAL;
+  }
+  if (nshell > 65535) {
+    bsk_set_error("bsk_shells: at most 65535 shells per call");
+    return BSK_EINVAL;
+  }
+  if (dtype == BSK_DT_F32) {
+    if (fmt != BSK_FMT_F32 && fmt != BSK_FMT_BF16X2) {
+      bsk_set_error("bsk_shells: FP32 shells are stored as F32 or BF16X2");
+      return BSK_EINVAL;
+    }
+    if (yz && fmt == BSK_FMT_BF16X2 && !out_lo) {
+      bsk_set_error("bsk_shells: BF16X2 needs the lo plane");
+      return BSK_EINVAL;
+    }
+    return shells_t<float>(p, kind, (const cplx<float>*)dk, shell0, nshell, x0, xslab, stages, fmt,
+                           out, out_lo, shell_stride, out_cell0, work, as_stream(stream), p->tw_f);
+  }
+  if (dtype == BSK_DT_F64) {
+    if (fmt != BSK_FMT_F64) {
+      bsk_set_error("bsk_shells: FP64 shells are stored as F64");
+      return BSK_EINVAL;
+    }
+    return shells_t<double>(p, kind, (const cplx<double>*)dk, shell0, nshell, x0, xslab, stages,
+                            fmt, out, nullptr, shell_stride, out_cell0, work, as_stream(stream),
+                            p->tw_d);
+  }
+  bsk_set_error("bsk_shells: dtype must be BSK_DT_F32 or BSK_DT_F64");
+  return BSK_EINVAL;
+}
+
+// ---------------------------------------------------------------- triples
+int bsk_triples_plain_workspace(int ntri, int nchunk, size_t* bytes) {
+  *bytes = sizeof(double) * (size_t)std::max(ntri, 0) * std::max(nchunk, 1);
+  return BSK_OK;
+}
+
+int bsk_triples_plain(int dtype, const void* fa, const void* fb, const void* fc,
+                      long long shell_stride, long long ncell, int ntri, const int* tri_dev,
+                      int nchunk, double scale, double* out_dev, void* work, void* stream) {
+  if (!fa || !fb || !fc || !tri_dev || !out_dev || !work || nchunk <= 0 || ncell <= 0) {
+    bsk_set_error("bsk_triples_plain: invalid argument");
+    return BSK_EINVAL;
+  }
+  hipStream_t s = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
+  int rc;
+  if (dtype == BSK_DT_F32)
+    rc = bsk_triple_plain_launch<float>((const float*)fa, (const float*)fb, (const float*)fc,
+                                        shell_stride, shell_stride, shell_stride, ncell, ntri,
+                                        tri_dev, part, nchunk, s);
+  else if (dtype == BSK_DT_F64)
+    rc = bsk_triple_plain_launch<double>((const double*)fa, (const double*)fb, (const double*)fc,
+                                         shell_stride, shell_stride, shell_stride, ncell, ntri,
+                                         tri_dev, part, nchunk, s);
+  else {
+    bsk_set_error("bsk_triples_plain: bad dtype");
+    return BSK_EINVAL;
+  }
+  if (rc) return rc;
+  return bsk_reduce_plain_launch(part, ntri, nchunk, scale, out_dev, s);
+}
+
+int bsk_triples_tiled_workspace(int ntri, int nsum, int nchunk, size_t* bytes) {
+  *bytes = sizeof(double) * (size_t)std::max(ntri, 0) * nsum * std::max(nchunk, 1);
+  return BSK_OK;
+}
+
+int bsk_triples_tiled(int dtype, const void* fields, long long shell_stride, int nfield,
+                      long long ncell, int ntri, const int* tri_dev, int nsum, int nchunk,
+                      double scale, double* out_dev, void* work, void* stream) {
+  if (!fields || !tri_dev || !out_dev || !work || nchunk <= 0 || nfield <= 0 ||
+      (nsum != 1 && nsum != 4) || (nsum == 4 && (nfield & 1))) {
+    bsk_set_error("bsk_triples_tiled: invalid argument");
+    return BSK_EINVAL;
+  }
+  hipStream_t s = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
+  int rc;
+  if (dtype == BSK_DT_F32)
+    rc = bsk_triple_tiled_launch<float>((const float*)fields, shell_stride, nfield, ncell, ntri,
+                                        tri_dev, nsum, part, nchunk, s);
+  else if (dtype == BSK_DT_F64)
+    rc = bsk_triple_tiled_launch<double>((const double*)fields, shell_stride, nfield, ncell, ntri,
+                                         tri_dev, nsum, part, nchunk, s);
+  else {
+    bsk_set_error("bsk_triples_tiled: bad dtype");
+    return BSK_EINVAL;
+  }
+  if (rc) return rc;
+  return bsk_reduce_tiled_launch(part, ntri, nsum, nchunk, scale, out_dev, s);
+}
+
+int bsk_mfma_tiles_per_wg(int nshell, int mode, int* tiles) {
+  const int nblk = (nshell + 15) / 16;
+  if (nblk < 1 || nblk > 3 || mode < 0 || mode > 2) {
+    bsk_set_error("mfma: 1..48 shells and mode 0..2 supported (got %d shells, mode %d)", nshell,
+                  mode);
+    return BSK_EUNSUPPORTED;
+  }
+  *tiles = bsk_mfma_tiles(nblk, mode);
+  return BSK_OK;
+}
+
+int bsk_mfma_slots(int nshell, int mode, int ntri, const int* tri, int* slot) {
+  const int nblk = (nshell + 15) / 16;
+  if (nblk < 1 || nblk > 3 || mode < 0 || mode > 2 || ntri < 0 || (ntri && (!tri || !slot))) {
+    bsk_set_error("bsk_mfma_slots: invalid argument");
+    return BSK_EINVAL;
+  }
+  for (int t = 0; t < ntri; ++t) {
+    int i = tri[3 * t], j = tri[3 * t + 1], k = tri[3 * t + 2];
+    if (i < 0 || j < 0 || k < 0 || i >= nshell || j >= nshell || k >= nshell) {
+      bsk_set_error("bsk_mfma_slots: triangle %d has a shell index out of range", t);
+      return BSK_EINVAL;
+    }
+    if (mode == 0) {  // symmetric: order i >= j >= k
+      int v[3] = {i, j, k};
+      std::sort(v, v + 3);
+      i = v[2]; j = v[1]; k = v[0];
+    } else if (mode == 1) {  // symmetric in the first two sides
+      if (j > i) std::swap(i, j);
+    }
+    const int ib = i / 16, ii = i % 16, jb = j / 16, kb = k / 16;
+    int before = 0, nti = 0, tin = 0;
+    for (int b = 0; b < ib; ++b)
+      before += mode == 0 ? (b + 1) * (b + 2) / 2 : (mode == 1 ? (b + 1) * nblk : nblk * nblk);
+    nti = mode == 0 ? (ib + 1) * (ib + 2) / 2 : (mode == 1 ? (ib + 1) * nblk : nblk * nblk);
+    tin = mode == 0 ? jb * (jb + 1) / 2 + kb : jb * nblk + kb;
+    const int gtile = 16 * before + ii * nti + tin;
+    slot[t] = gtile * 256 + (j % 16) * 16 + (k % 16);
+  }
+  return BSK_OK;
+}
+
+int bsk_triples_mfma_workspace(int nshell, int mode, int nwg, size_t* bytes) {
+  int tiles = 0;
+  if (int rc = bsk_mfma_tiles_per_wg(nshell, mode, &tiles)) return rc;
+  *bytes = sizeof(float) * 256 * (size_t)tiles * std::max(nwg, 1);
+  return BSK_OK;
+}
+
+int bsk_triples_mfma(int mode, int nshell, const uint16_t* hi_i, const uint16_t* lo_i,
+                     const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
+                     const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
+                     int ntri, const int* slot_dev, double scale, double* out_dev, void* work,
+                     void* stream) {
+  int tiles = 0;
+  if (int rc = bsk_mfma_tiles_per_wg(nshell, mode, &tiles)) return rc;
+  if (!hi_j || !lo_j || !slot_dev || !out_dev || !work || nwg <= 0 ||
+      (mode >= 1 && (!hi_k || !lo_k)) || (mode == 2 && (!hi_i || !lo_i)) ||
+      shell_stride < ncell) {
+    bsk_set_error("bsk_triples_mfma: invalid argument");
+    return BSK_EINVAL;
+  }
+  MfmaArgs a;
+  a.hi[1] = hi_j;
+  a.lo[1] = lo_j;
+  a.hi[0] = mode == 2 ? hi_i : hi_j;
+  a.lo[0] = mode == 2 ? lo_i : lo_j;
+  a.hi[2] = mode >= 1 ? hi_k : hi_j;
+  a.lo[2] = mode >= 1 ? lo_k : lo_j;
+  for (int s = 0; s < 3; ++s) {
+    a.stride[s] = shell_stride;
+    a.nsh[s] = nshell;
+  }
+  a.ncell = ncell;
+  a.mode = mode;
+  a.nblk = (nshell + 15) / 16;
+  a.nwg = nwg;
+  a.partial = reinterpret_cast<float*>(work);
+  hipStream_t s = as_stream(stream);
+  int rc = bsk_triple_mfma_launch(a, s);
+  if (rc) return rc;
+  return bsk_reduce_mfma_launch(a.partial, nwg, tiles, ntri, slot_dev, scale, out_dev, s);
+}
+
+}  // extern "C"

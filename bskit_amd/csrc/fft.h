@@ -1,0 +1,161 @@
+// Batched 1-D complex FFTs on pencils held in LDS (Stockham autosort).
+//
+// A workgroup of W threads transforms P pencils of length L that sit in LDS
+// at stride LD complex elements.  Each stage of radix R performs P*L/R
+// butterflies; a thread loads all of its butterflies' inputs into registers,
+// the workgroup synchronises, and the thread writes the outputs back in
+// place at the Stockham (self-sorting) positions, so no second LDS buffer is
+// needed and the result is in natural order.  Twiddles come from a table
+// tw[t] = exp(-2 pi i t / TWN) in LDS; a length-L sub-transform indexes it
+// with stride TWN/L (TWN is a multiple of L).
+#pragma once
+#include "common.h"
+
+template <int... Rs>
+struct Radices {};
+
+// ---------------------------------------------------------------- butterflies
+template <typename T, int R, bool INV>
+struct Dft;
+
+template <typename T, bool INV>
+struct Dft<T, 2, INV> {
+  static BSK_DEV void run(cplx<T>* v) {
+    cplx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <typename T, bool INV>
+struct Dft<T, 3, INV> {
+  static BSK_DEV void run(cplx<T>* v) {
+    const T s = (T)0.86602540378443864676;  // sin(2 pi / 3)
+    cplx<T> a = v[0], b = v[1], c = v[2];
+    cplx<T> t1 = cadd(b, c);
+    cplx<T> t2 = {a.x - (T)0.5 * t1.x, a.y - (T)0.5 * t1.y};
+    cplx<T> u = cscale(csub(b, c), s);
+    cplx<T> iu = mul_mi<T, INV>(u);  // -i*u forward, +i*u inverse
+    v[0] = cadd(a, t1);
+    v[1] = cadd(t2, iu);
+    v[2] = csub(t2, iu);
+  }
+};
+
+template <typename T, bool INV>
+struct Dft<T, 4, INV> {
+  static BSK_DEV void run(cplx<T>* v) {
+    cplx<T> s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    cplx<T> s13 = cadd(v[1], v[3]), d13 = mul_mi<T, INV>(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+  }
+};
+
+template <typename T, bool INV>
+struct Dft<T, 8, INV> {
+  static BSK_DEV void run(cplx<T>* v) {
+    const T h = (T)0.70710678118654752440;
+    cplx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cplx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<T, 4, INV>::run(e);
+    Dft<T, 4, INV>::run(o);
+    // o[k] *= w8^k, w8 = exp(-+ 2 pi i / 8)
+    cplx<T> o1 = o[1], o2 = o[2], o3 = o[3];
+    if (!INV) {
+      o[1] = {h * (o1.x + o1.y), h * (o1.y - o1.x)};
+      o[2] = {o2.y, -o2.x};
+      o[3] = {h * (o3.y - o3.x), -h * (o3.x + o3.y)};
+    } else {
+      o[1] = {h * (o1.x - o1.y), h * (o1.x + o1.y)};
+      o[2] = {-o2.y, o2.x};
+      o[3] = {-h * (o3.x + o3.y), h * (o3.x - o3.y)};
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = cadd(e[k], o[k]);
+      v[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
+
+// ---------------------------------------------------------------- one stage
+template <typename T, int L, int R, int NS, bool INV, int P, int W, int LD, int TWN>
+BSK_DEV void stockham_stage(cplx<T>* buf, const cplx<T>* tw) {
+  constexpr int NB = L / R;
+  constexpr int TOT = P * NB;
+  constexpr int BPT = (TOT + W - 1) / W;
+  const int tid = threadIdx.x;
+  cplx<T> v[BPT][R];
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int b = tid + q * W;
+    if ((TOT % W) == 0 || b < TOT) {
+      const int p = b / NB, j = b - p * NB;
+      const cplx<T>* src = buf + p * LD + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = src[r * NB];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int b = tid + q * W;
+    if ((TOT % W) == 0 || b < TOT) {
+      const int p = b / NB, j = b - p * NB;
+      const int k = j % NS;
+      if (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cplx<T> w = tw[k * r * (TWN / (NS * R))];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      Dft<T, R, INV>::run(v[q]);
+      cplx<T>* dst = buf + p * LD + (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int L, bool INV, int P, int W, int LD, int TWN, int NS, int R, int... Rest>
+BSK_DEV void stockham_run(cplx<T>* buf, const cplx<T>* tw) {
+  stockham_stage<T, L, R, NS, INV, P, W, LD, TWN>(buf, tw);
+  if constexpr (sizeof...(Rest) > 0)
+    stockham_run<T, L, INV, P, W, LD, TWN, NS * R, Rest...>(buf, tw);
+}
+
+template <typename T, int L, bool INV, int P, int W, int LD, int TWN, int... Rs>
+BSK_DEV void fft_lds(cplx<T>* buf, const cplx<T>* tw, Radices<Rs...>) {
+  stockham_run<T, L, INV, P, W, LD, TWN, 1, Rs...>(buf, tw);
+}
+
+// ---------------------------------------------------------------- radix plans
+// Two variants per length: 0 = radix-8 heavy, 1 = radix-4 heavy.
+template <int L, int V>
+struct Plan;
+#define BSK_PLAN(L, V, ...) \
+  template <>               \
+  struct Plan<L, V> {       \
+    using seq = Radices<__VA_ARGS__>; \
+  };
+BSK_PLAN(8, 0, 8) BSK_PLAN(8, 1, 4, 2)
+BSK_PLAN(12, 0, 4, 3) BSK_PLAN(12, 1, 2, 2, 3)
+BSK_PLAN(16, 0, 4, 4) BSK_PLAN(16, 1, 8, 2)
+BSK_PLAN(24, 0, 8, 3) BSK_PLAN(24, 1, 4, 2, 3)
+BSK_PLAN(32, 0, 8, 4) BSK_PLAN(32, 1, 4, 4, 2)
+BSK_PLAN(48, 0, 8, 2, 3) BSK_PLAN(48, 1, 4, 4, 3)
+BSK_PLAN(64, 0, 8, 8) BSK_PLAN(64, 1, 4, 4, 4)
+BSK_PLAN(96, 0, 8, 4, 3) BSK_PLAN(96, 1, 4, 4, 2, 3)
+BSK_PLAN(128, 0, 8, 4, 4) BSK_PLAN(128, 1, 4, 4, 4, 2)
+BSK_PLAN(192, 0, 8, 8, 3) BSK_PLAN(192, 1, 4, 4, 4, 3)
+BSK_PLAN(256, 0, 8, 8, 4) BSK_PLAN(256, 1, 4, 4, 4, 4)
+BSK_PLAN(384, 0, 8, 8, 2, 3) BSK_PLAN(384, 1, 8, 4, 4, 3)
+BSK_PLAN(512, 0, 8, 8, 8) BSK_PLAN(512, 1, 4, 4, 4, 8)
+BSK_PLAN(768, 0, 8, 8, 4, 3) BSK_PLAN(768, 1, 4, 4, 4, 4, 3)
+#undef BSK_PLAN

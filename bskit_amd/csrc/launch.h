@@ -1,0 +1,61 @@
+// Internal host-side launch API shared by the translation units.
+#pragma once
+#include "common.h"
+#include "fft_passes.h"
+
+bool bsk_fft_size_supported(int N);
+
+template <typename T>
+int bsk_fwd_z(int N, int V, const T* in, cplx<T>* out, const cplx<T>* tw, long long nrows, int MZ,
+              hipStream_t s);
+template <typename T>
+int bsk_fwd_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g,
+              hipStream_t s);
+template <typename T>
+int bsk_fwd_x(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, T scale,
+              const T* comp, hipStream_t s);
+template <typename T>
+int bsk_inv_x(int N, int V, const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g,
+              const int* lo, const int* hi, int shell0, int nsh, int kind, T kf, hipStream_t s);
+template <typename T>
+int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0,
+              int xs, int nsh, hipStream_t s);
+template <typename T>
+int bsk_inv_z(int N, int V, const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
+              long long nrows, long long cell0, long long stride, int fmt, int nsh, hipStream_t s);
+
+// Triple products ------------------------------------------------------------
+// Reference-structured FP32/FP64 path: one sweep over all cells per triangle.
+template <typename T>
+int bsk_triple_plain_launch(const T* a, const T* b, const T* c, long long stride_a,
+                            long long stride_b, long long stride_c, long long ncell, int ntri,
+                            const int* tri_dev, double* partial, int nchunk, hipStream_t s);
+int bsk_reduce_plain_launch(const double* partial, int ntri, int nchunk, double scale, double* out,
+                            hipStream_t s);
+
+// Tiled VALU path: every triangle of a batch per pass over a cell chunk.
+// nsum = 1 (B) or 4 (N_tri and the three k-mean numerators; fields are the
+// unit shells followed by the |k| shells).
+template <typename T>
+int bsk_triple_tiled_launch(const T* f, long long stride, int nfield, long long ncell, int ntri,
+                            const int* tri_dev, int nsum, double* partial, int nchunk,
+                            hipStream_t s);
+int bsk_reduce_tiled_launch(const double* partial, int ntri, int nsum, int nchunk, double scale,
+                            double* out, hipStream_t s);
+
+// Split-BF16 MFMA path.
+struct MfmaArgs {
+  const uint16_t* hi[3];   // i-side, j-side, k-side shell sets (hi planes)
+  const uint16_t* lo[3];
+  long long stride[3];     // elements between consecutive shells of a set
+  int nsh[3];              // real shells per set
+  long long ncell;         // cells per shell (multiple of 128)
+  int mode;                // 0 auto (i>=j>=k), 1 <AAB> (j<=i, all k), 2 <ABC> (all)
+  int nblk;                // 16-shell blocks (1..3)
+  int nwg;                 // workgroups (cell ranges)
+  float* partial;          // [nwg][tiles][256]
+};
+int bsk_mfma_tiles(int nblk, int mode);  // tile slots per workgroup
+int bsk_triple_mfma_launch(const MfmaArgs& a, hipStream_t s);
+int bsk_reduce_mfma_launch(const float* partial, int nwg, int tiles_per_wg, int ntri,
+                           const int* slot_dev, double scale, double* out, hipStream_t s);

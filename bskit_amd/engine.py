@@ -1,0 +1,378 @@
+"""Device engine of the FFT bispectrum path on one MI355X (gfx950).
+
+``GridEngine`` holds one libbskhip plan for an (Nmesh, BoxSize, bin set) key
+and drives the kernels on torch-allocated HBM buffers:
+
+    field --forward--> delta(k) compact cube --shells--> nb shell fields
+          --triples--> per-triangle FP64 sums
+
+torch is used only for device memory and the current stream; every
+arithmetic step is a libbskhip kernel.  There is no CPU fallback: without a
+GPU or without the built library these calls raise.
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+# Triple-product paths
+PATH_PLAIN = 'plain'      # one sweep per triangle in the shell precision
+PATH_MFMA = 'mfma'        # split-BF16 MFMA, FP32 accumulate, FP64 reduce
+PATH_TILED = 'tiled'      # LDS tiled VALU in the shell precision
+PATHS = (PATH_PLAIN, PATH_MFMA, PATH_TILED)
+
+_MODE = {1: L.MODE_AUTO, 2: L.MODE_AAB, 3: L.MODE_ABC}
+
+
+def require_device(device=None):
+    if not torch.cuda.is_available():
+        raise RuntimeError('bskit_amd needs a ROCm GPU (torch.cuda.is_available() is False); '
+                           'there is no CPU fallback')
+    L.lib()
+    return torch.device(device if device is not None else 'cuda')
+
+
+def stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _dt(dtype):
+    if dtype == torch.float32:
+        return L.DT_F32
+    if dtype == torch.float64:
+        return L.DT_F64
+    raise TypeError('fields must be float32 or float64, got %s' % dtype)
+
+
+def shell_limits(nmesh, box_size, k_low, k_high):
+    """Integer |n|^2 limits of the closed shells (host call into the ABI)."""
+    k_low = np.atleast_1d(np.asarray(k_low, dtype=np.float64))
+    k_high = np.atleast_1d(np.asarray(k_high, dtype=np.float64))
+    n = len(k_low)
+    lo = (ctypes.c_int * n)()
+    hi = (ctypes.c_int * n)()
+    L.call('bsk_shell_thresholds', nmesh, float(box_size), n, L.dbl_array(k_low),
+           L.dbl_array(k_high), lo, hi)
+    return np.array(lo[:], dtype=np.int64), np.array(hi[:], dtype=np.int64)
+
+
+class Workspace:
+    """Grow-only scratch buffers (device bytes) keyed by purpose."""
+
+    def __init__(self, device):
+        self.device = device
+        self._bufs = {}
+
+    def get(self, key, nbytes):
+        nbytes = max(int(nbytes), 256)
+        b = self._bufs.get(key)
+        if b is None or b.numel() < nbytes:
+            b = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            self._bufs[key] = b
+        return b
+
+    def release(self):
+        self._bufs.clear()
+
+
+class ShellSet:
+    """nb shell fields of one input field, [nb, N^3] in one storage format."""
+
+    def __init__(self, fmt, data, lo=None):
+        self.fmt = fmt
+        self.data = data      # F32/F64: [nb, ncell] float; BF16X2: hi plane int16
+        self.lo = lo          # BF16X2 lo plane
+        self.nb = data.shape[0]
+        self.ncell = data.shape[1]
+
+    def as_float(self):
+        """FP32/FP64 view of the values (hi + lo for BF16X2), for inspection."""
+        if self.fmt != L.FMT_BF16X2:
+            return self.data
+        def f(p):
+            return (p.to(torch.int32) << 16).view(torch.float32)
+        return f(self.data) + f(self.lo)
+
+
+class GridEngine:
+    """Plan + kernels for one (Nmesh, BoxSize, bins) key on one device."""
+
+    def __init__(self, nmesh, box_size, k_low, k_high, device=None, cic_window=0):
+        self.device = require_device(device)
+        self.N = int(nmesh)
+        self.box = float(box_size)
+        self.k_low = np.atleast_1d(np.asarray(k_low, dtype=np.float64))
+        self.k_high = np.atleast_1d(np.asarray(k_high, dtype=np.float64))
+        self.nb = len(self.k_low)
+        if self.nb == 0:
+            raise ValueError('at least one k bin is needed')
+        self.ncell = self.N ** 3
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            L.call('bsk_plan_create', ctypes.byref(h), self.N, self.box, self.nb,
+                   L.dbl_array(self.k_low), L.dbl_array(self.k_high))
+        self._plan = h
+        g = [ctypes.c_int() for _ in range(5)]
+        L.call('bsk_plan_geometry', self._plan, *[ctypes.byref(x) for x in g])
+        self.m, self.MX, self.MY, self.MZ, self.nlo = [x.value for x in g]
+        lo = (ctypes.c_int * self.nb)()
+        hi = (ctypes.c_int * self.nb)()
+        L.call('bsk_plan_thresholds', self._plan, lo, hi)
+        self.n2lo = np.array(lo[:], dtype=np.int64)
+        self.n2hi = np.array(hi[:], dtype=np.int64)
+        self.work = Workspace(self.device)
+        self._tri_cache = {}
+        self.cic_window = 0
+        if cic_window:
+            self.set_compensation(cic_window)
+
+    # ------------------------------------------------------------------ plan
+    def close(self):
+        if self._plan:
+            L.lib().bsk_plan_destroy(self._plan)
+            self._plan = None
+        self.work.release()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_compensation(self, nmesh_window):
+        L.call('bsk_plan_set_compensation', self._plan, int(nmesh_window))
+        self.cic_window = int(nmesh_window)
+
+    def set_variant(self, fft_pass, variant):
+        L.call('bsk_plan_set_variant', self._plan, int(fft_pass), int(variant))
+
+    @property
+    def kf(self):
+        return 2.0 * math.pi / self.box
+
+    # --------------------------------------------------------------- forward
+    def forward(self, field, stream=None):
+        """delta(k) on the compact cube [MY, MX, MZ] (complex, 1/N^3 scaled)."""
+        if field.device.type != 'cuda':
+            raise ValueError('field must be a device tensor')
+        if tuple(field.shape) != (self.N,) * 3:
+            raise ValueError('field shape %s does not match Nmesh=%d' % (tuple(field.shape), self.N))
+        field = field.contiguous()
+        dt = _dt(field.dtype)
+        ctype = torch.complex64 if dt == L.DT_F32 else torch.complex128
+        dk = torch.empty((self.MY, self.MX, self.MZ), dtype=ctype, device=self.device)
+        nbytes = L.size_out()
+        L.call('bsk_forward_workspace', self._plan, dt, ctypes.byref(nbytes))
+        w = self.work.get('fwd', nbytes.value)
+        L.call('bsk_forward', self._plan, dt, L.ptr(field), L.ptr(dk), L.ptr(w),
+               stream_handle(stream))
+        return dk
+
+    # ---------------------------------------------------------------- shells
+    def shells(self, dk=None, kind=L.KIND_DELTA, fmt=L.FMT_F32, dtype=torch.float32,
+               shell0=0, nshell=None, out=None, stream=None):
+        """All (or a range of) shell fields of one spectrum, full grid."""
+        nshell = self.nb - shell0 if nshell is None else nshell
+        dt = _dt(dtype)
+        if kind == L.KIND_DELTA:
+            if dk is None:
+                raise ValueError('delta shells need the spectrum')
+            dt = L.DT_F32 if dk.dtype == torch.complex64 else L.DT_F64
+        if out is None:
+            out = self.alloc_shells(fmt, self.nb)
+        nbytes = L.size_out()
+        L.call('bsk_shells_workspace', self._plan, dt, nshell, self.N, ctypes.byref(nbytes))
+        w = self.work.get('shell', nbytes.value)
+        L.call('bsk_shells', self._plan, dt, kind, L.ptr(dk), shell0, nshell, 0, self.N,
+               L.STAGE_X | L.STAGE_YZ, fmt, L.ptr(out.data), L.ptr(out.lo), self.ncell, 0,
+               L.ptr(w), stream_handle(stream))
+        return out
+
+    def alloc_shells(self, fmt, nb):
+        if fmt == L.FMT_BF16X2:
+            hi = torch.empty((nb, self.ncell), dtype=torch.int16, device=self.device)
+            lo = torch.empty((nb, self.ncell), dtype=torch.int16, device=self.device)
+            return ShellSet(fmt, hi, lo)
+        dtype = torch.float32 if fmt == L.FMT_F32 else torch.float64
+        return ShellSet(fmt, torch.empty((nb, self.ncell), dtype=dtype, device=self.device))
+
+    # --------------------------------------------------------------- triples
+    def triangles_device(self, tri):
+        tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
+        key = (tri.tobytes(), 'tri')
+        t = self._tri_cache.get(key)
+        if t is None:
+            t = torch.from_numpy(tri.copy()).to(self.device)
+            self._tri_cache[key] = t
+        return t
+
+    def mfma_slots(self, tri, nshell, mode):
+        tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
+        key = (tri.tobytes(), 'slot', nshell, mode)
+        t = self._tri_cache.get(key)
+        if t is None:
+            slots = np.zeros(len(tri), dtype=np.int32)
+            L.call('bsk_mfma_slots', nshell, mode, len(tri),
+                   tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                   slots.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+            t = torch.from_numpy(slots).to(self.device)
+            self._tri_cache[key] = t
+        return t
+
+    def mfma_nwg(self, ncell=None):
+        niter = (self.ncell if ncell is None else ncell) // 64
+        return int(max(1, min(2048, niter // 256)))
+
+    def plain_nchunk(self, ncell=None):
+        ncell = self.ncell if ncell is None else ncell
+        return int(max(1, min(4096, ncell // 65536)))
+
+    def triples(self, sets, tri, path=PATH_MFMA, scale=1.0, out=None, stream=None):
+        """scale * sum_x A_i B_j C_k for every (i, j, k) row of tri.
+
+        sets: [A] (auto), [A, B] (<AAB>: sides A, A, B) or [A, B, C].
+        Returns a float64 device tensor of length T.
+        """
+        tri = np.asarray(tri, dtype=np.int32).reshape(-1, 3)
+        T = len(tri)
+        if T and (tri.min() < 0 or tri.max() >= min(x.nb for x in sets)):
+            raise IndexError('triangle shell index out of range')
+        if out is None:
+            out = torch.empty(T, dtype=torch.float64, device=self.device)
+        if T == 0:
+            return out
+        nf = len(sets)
+        if nf == 1:
+            sa = sb = sc = sets[0]
+        elif nf == 2:
+            sa, sb, sc = sets[0], sets[0], sets[1]
+        elif nf == 3:
+            sa, sb, sc = sets
+        else:
+            raise ValueError('1, 2 or 3 shell sets')
+        s = stream_handle(stream)
+        ncell = sa.ncell
+        if path == PATH_MFMA:
+            for x in sets:
+                if x.fmt != L.FMT_BF16X2:
+                    raise ValueError('the MFMA path needs BF16X2 shells')
+            mode = _MODE[nf]
+            nwg = self.mfma_nwg(ncell)
+            slots = self.mfma_slots(tri, sa.nb, mode)
+            nbytes = L.size_out()
+            L.call('bsk_triples_mfma_workspace', sa.nb, mode, nwg, ctypes.byref(nbytes))
+            w = self.work.get('mfma', nbytes.value)
+            hi_i, lo_i = sa.data, sa.lo
+            hi_j, lo_j = sb.data, sb.lo
+            hi_k, lo_k = sc.data, sc.lo
+            L.call('bsk_triples_mfma', mode, sa.nb, L.ptr(hi_i), L.ptr(lo_i), L.ptr(hi_j),
+                   L.ptr(lo_j), L.ptr(hi_k), L.ptr(lo_k), ncell, ncell, nwg, T, L.ptr(slots),
+                   float(scale), L.ptr(out), L.ptr(w), s)
+            return out
+        if path == PATH_PLAIN:
+            dt = {L.FMT_F32: L.DT_F32, L.FMT_F64: L.DT_F64}.get(sa.fmt)
+            if dt is None:
+                raise ValueError('the plain path needs F32 or F64 shells')
+            nchunk = self.plain_nchunk(ncell)
+            tri_d = self.triangles_device(tri)
+            nbytes = L.size_out()
+            L.call('bsk_triples_plain_workspace', min(T, 65535), nchunk, ctypes.byref(nbytes))
+            w = self.work.get('plain', nbytes.value)
+            for t0 in range(0, T, 65535):
+                t1 = min(T, t0 + 65535)
+                L.call('bsk_triples_plain', dt, L.ptr(sa.data), L.ptr(sb.data), L.ptr(sc.data),
+                       ncell, ncell, t1 - t0, L.ptr(tri_d[t0:t1]), nchunk, float(scale),
+                       L.ptr(out[t0:t1]), L.ptr(w), s)
+            return out
+        if path == PATH_TILED:
+            if nf != 1:
+                raise ValueError('the tiled path takes one shell set (auto spectra)')
+            dt = {L.FMT_F32: L.DT_F32, L.FMT_F64: L.DT_F64}[sa.fmt]
+            nbatch = (T + 511) // 512
+            nchunk = int(max(1, min(ncell // 32, 2048 // nbatch)))
+            tri_d = self.triangles_device(tri)
+            nbytes = L.size_out()
+            L.call('bsk_triples_tiled_workspace', T, 1, nchunk, ctypes.byref(nbytes))
+            w = self.work.get('tiled', nbytes.value)
+            L.call('bsk_triples_tiled', dt, L.ptr(sa.data), ncell, sa.nb, ncell, T, L.ptr(tri_d), 1,
+                   nchunk, float(scale), L.ptr(out), L.ptr(w), s)
+            return out
+        raise ValueError('unknown triple path %r' % (path,))
+
+    # ------------------------------------------------------------- grid info
+    def grid_info(self, tri, max_bytes=None, stream=None):
+        """(N_tri, kbar) for each triangle (i, j, k) of bin indices, FP64.
+
+        The unit and |k| shells are made in FP64 x-slab by x-slab and reduced
+        with the tiled FP64 kernel; N_tri is the rounded sum / N^3 (exact
+        integer), kbar = sum(|k| n n) / N^3 / N_tri (NaN where N_tri == 0).
+        """
+        tri = np.asarray(tri, dtype=np.int32).reshape(-1, 3)
+        T = len(tri)
+        if T == 0:
+            return np.zeros(0), np.zeros((0, 3))
+        if tri.min() < 0 or tri.max() >= self.nb:
+            raise IndexError('triangle bin index out of range')
+        N, nb = self.N, self.nb
+        if max_bytes is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            max_bytes = int(free * 0.5)
+        plane = N * N * 8 * 2 * nb
+        xs = int(max(1, min(N, max_bytes // max(plane, 1))))
+        while N % xs:
+            xs -= 1
+        s = stream_handle(stream)
+        nbytes = L.size_out()
+        L.call('bsk_shells_workspace', self._plan, L.DT_F64, nb, xs, ctypes.byref(nbytes))
+        wu = self.work.get('gi_u', nbytes.value)
+        wk = self.work.get('gi_k', nbytes.value)
+        scells = xs * N * N
+        buf = torch.empty((2 * nb, scells), dtype=torch.float64, device=self.device)
+        tri_d = self.triangles_device(tri)
+        nbatch = (T + 511) // 512
+        nchunk = int(max(1, min(scells // 32, 2048 // nbatch)))
+        L.call('bsk_triples_tiled_workspace', T, 4, nchunk, ctypes.byref(nbytes))
+        wt = self.work.get('gi_t', nbytes.value)
+        total = torch.zeros(4 * T, dtype=torch.float64, device=self.device)
+        part = torch.empty(4 * T, dtype=torch.float64, device=self.device)
+        L.call('bsk_shells', self._plan, L.DT_F64, L.KIND_UNIT, None, 0, nb, 0, xs, L.STAGE_X,
+               L.FMT_F64, None, None, 0, 0, L.ptr(wu), s)
+        L.call('bsk_shells', self._plan, L.DT_F64, L.KIND_KMAG, None, 0, nb, 0, xs, L.STAGE_X,
+               L.FMT_F64, None, None, 0, 0, L.ptr(wk), s)
+        for x0 in range(0, N, xs):
+            L.call('bsk_shells', self._plan, L.DT_F64, L.KIND_UNIT, None, 0, nb, x0, xs,
+                   L.STAGE_YZ, L.FMT_F64, L.ptr(buf), None, scells, 0, L.ptr(wu), s)
+            L.call('bsk_shells', self._plan, L.DT_F64, L.KIND_KMAG, None, 0, nb, x0, xs,
+                   L.STAGE_YZ, L.FMT_F64, L.ptr(buf[nb:]), None, scells, 0, L.ptr(wk), s)
+            L.call('bsk_triples_tiled', L.DT_F64, L.ptr(buf), scells, 2 * nb, scells, T,
+                   L.ptr(tri_d), 4, nchunk, 1.0, L.ptr(part), L.ptr(wt), s)
+            total += part
+        sums = (total / float(self.ncell)).view(T, 4).cpu().numpy()
+        ntri = np.rint(sums[:, 0])
+        kbar = np.full((T, 3), np.nan)
+        nz = ntri != 0
+        kbar[nz] = sums[nz, 1:4] / ntri[nz, None]
+        return ntri, kbar
+
+    # ---------------------------------------------------------------- sizing
+    def shell_bytes(self, fmt, nb=None):
+        nb = self.nb if nb is None else nb
+        per = 8 if fmt == L.FMT_F64 else 4
+        return nb * self.ncell * per
+
+
+def cic_paint(positions, nmesh, box_size, to_contrast=True, out=None, stream=None):
+    """CIC-paint (npart, 3) float32 device positions onto an nmesh^3 mesh."""
+    require_device()
+    pos = positions.contiguous()
+    if pos.dtype != torch.float32 or pos.dim() != 2 or pos.shape[1] != 3:
+        raise ValueError('positions must be a (npart, 3) float32 tensor')
+    if out is None:
+        out = torch.empty((nmesh,) * 3, dtype=torch.float32, device=pos.device)
+    L.call('bsk_cic_paint', int(nmesh), float(box_size), pos.shape[0], L.ptr(pos), L.ptr(out),
+           1 if to_contrast else 0, stream_handle(stream))
+    return out

@@ -1,0 +1,176 @@
+/* libbskhip -- C ABI of the gfx950 FFT-bispectrum path.
+ *
+ * The reference (sjforeman/bskit) is a pure-Python library with no FFI: the
+ * hot path is the FFTBispectrum class (bskit/main.py:1392-2161) calling
+ * nbodykit/pmesh for its FFTs and numpy for its triple sums.  Each entry
+ * point below replaces one of those calls; the citation on each names the
+ * reference code it stands in for.  The Python layer (bskit_amd.main)
+ * mirrors the reference's class and module functions on top of this ABI.
+ *
+ * Conventions
+ *   - Every data pointer is caller-owned device memory; nothing is freed
+ *     across the ABI.  A plan owns only small constant tables.
+ *   - All launches are asynchronous on the given stream (hipStream_t passed
+ *     as void*; NULL = default stream).  A plan must not be used by two host
+ *     threads at once.
+ *   - Return value: 0 on success, negative BSK_E* code on error; the message
+ *     of the last error on the calling thread is bsk_last_error().
+ *   - Grids are cubic: Nmesh^3 cells in a box of side box_size.  Supported
+ *     Nmesh: 16, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768.
+ *   - Shell membership: bin b holds the modes with n2lo[b] <= |n|^2 <= n2hi[b]
+ *     (n = k / k_f integer vector, components in [-N/2, N/2]); the integer
+ *     limits are the closed interval k_low <= |k| <= k_high with modes
+ *     within 1e-9 (relative) of an edge counted inside.
+ */
+#ifndef BSKHIP_H
+#define BSKHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSK_ABI_VERSION 1
+
+/* status codes */
+#define BSK_OK 0
+#define BSK_EINVAL (-1)
+#define BSK_ENOMEM (-2)
+#define BSK_EUNSUPPORTED (-3)
+#define BSK_EHIP (-4)
+
+/* element types of real fields / complex spectra */
+#define BSK_DT_F32 0
+#define BSK_DT_F64 1
+/* shell storage formats */
+#define BSK_FMT_F32 0
+#define BSK_FMT_BF16X2 1 /* two bf16 planes: hi = rne(v), lo = rne(v - hi) */
+#define BSK_FMT_F64 2
+/* shell kinds */
+#define BSK_KIND_DELTA 0 /* mask * delta(k)          (main.py:1846-1861)      */
+#define BSK_KIND_UNIT 1  /* mask                     (number_field :280-329)  */
+#define BSK_KIND_KMAG 2  /* |k| * mask               (k_field p=1 :227-277)   */
+/* triple-product modes (field roles of main.py:624-640) */
+#define BSK_MODE_AUTO 0 /* <AAA>: triangles taken with i >= j >= k      */
+#define BSK_MODE_AAB 1  /* <AAB>: symmetric in the first two sides      */
+#define BSK_MODE_ABC 2  /* <ABC>: no symmetry                           */
+
+typedef struct bsk_plan bsk_plan;
+
+/* Message of the last failed call on this thread (replaces the reference's
+ * Python exceptions at the ABI). */
+const char* bsk_last_error(void);
+int bsk_version(void);
+
+/* Integer |n|^2 limits of the closed shells [k_low[b], k_high[b]].
+ * Replaces the float mask predicate (kk<=kmax)&(kk>=kmin) of
+ * bskit/main.py:264-268, :314-320, :1850-1852. */
+int bsk_shell_thresholds(int nmesh, double box_size, int nbin, const double* k_low,
+                         const double* k_high, int* n2lo, int* n2hi);
+
+/* Plan for one (Nmesh, BoxSize, k bins) key.  Replaces the ParticleMesh set
+ * up per shell in k_field / number_field (bskit/main.py:271, :323) and the
+ * bin-edge list of measure_bispectrum_faster (:1828-1833). */
+int bsk_plan_create(bsk_plan** plan, int nmesh, double box_size, int nbin, const double* k_low,
+                    const double* k_high);
+int bsk_plan_destroy(bsk_plan* plan);
+/* Compact retained cube: |n_x|,|n_y| <= m (mx = my = min(2m+1, N) values,
+ * first signed frequency nlo) and 0 <= n_z < mz. */
+int bsk_plan_geometry(const bsk_plan* plan, int* m, int* mx, int* my, int* mz, int* nlo);
+int bsk_plan_thresholds(const bsk_plan* plan, int* n2lo, int* n2hi);
+/* Radix-plan variant (0 or 1) of FFT pass 0..5 (z r2c, y, x, masked x^-1,
+ * y^-1, z c2r); chosen by the autotuner. */
+int bsk_plan_set_variant(bsk_plan* plan, int pass, int variant);
+/* CIC window compensation applied in the forward transform: delta(k) is
+ * divided by prod_i (1 - 2/3 sin^2(pi n_i / nmesh_window))^(1/2).  Replaces
+ * CompensateCICShotnoiseNgrid (scripts/measure/measure_bs_fast.py:45-57,
+ * applied :222) and nbodykit's compensated CIC mesh (:211-217).
+ * nmesh_window <= 0 disables it. */
+int bsk_plan_set_compensation(bsk_plan* plan, int nmesh_window);
+
+/* Forward transform of a real N^3 field (C order, z fastest) into the
+ * compact cube delta(k) [my][mx][mz] (complex, interleaved), normalised by
+ * 1/N^3.  Replaces mesh.paint(mode='complex') in
+ * FFTBispectrum._paint_meshes (bskit/main.py:1608-1621).
+ * work: bsk_forward_workspace() bytes. */
+int bsk_forward_workspace(const bsk_plan* plan, int dtype, size_t* bytes);
+int bsk_forward(bsk_plan* plan, int dtype, const void* field, void* dk, void* work, void* stream);
+
+/* Shell fields of bins [shell0, shell0+nshell) on the x-slab
+ * [x0, x0+xslab): unnormalised inverse transform of the masked spectrum.
+ * stages: BSK_STAGE_X runs the masked inverse x transform of every
+ * retained (ky,kz) line into work; BSK_STAGE_YZ runs the y and z passes of
+ * the slab from that work (so one X stage serves many slabs); both = 3.
+ * Output shell b (relative to shell0) lands at
+ * out + (shell0+b)*shell_stride + out_cell0 (elements) with the slab's cells
+ * in C order; for BSK_FMT_BF16X2 out is the hi plane and out_lo the lo
+ * plane.  kind selects delta / unit / |k| shells (dk unused for the last
+ * two).  Replaces the per-bin view + mask action + paint(mode='real') of
+ * measure_bispectrum_faster (bskit/main.py:1846-1861) and number_field /
+ * k_field (:227-329) as used by measure_gridinfo_faster (:2006-2013).
+ * work: bsk_shells_workspace(nshell, xslab) bytes. */
+#define BSK_STAGE_X 1
+#define BSK_STAGE_YZ 2
+int bsk_shells_workspace(const bsk_plan* plan, int dtype, int nshell, int xslab, size_t* bytes);
+int bsk_shells(bsk_plan* plan, int dtype, int kind, const void* dk, int shell0, int nshell, int x0,
+               int xslab, int stages, int fmt, void* out, void* out_lo, long long shell_stride,
+               long long out_cell0, void* work, void* stream);
+
+/* out[t] = scale * sum_x A[i_t](x) B[j_t](x) C[k_t](x) for tri_dev = (i,j,k)
+ * per triangle (int32, device), one sweep over the cells per triangle in
+ * the shells' own precision.  Replaces the per-triangle
+ * np.sum(d_i*d_j*d_k) * L^6/N^3 + comm.gather/bcast of
+ * measure_bispectrum_faster (bskit/main.py:1871-1882).
+ * work: bsk_triples_plain_workspace(ntri, nchunk) bytes. */
+int bsk_triples_plain_workspace(int ntri, int nchunk, size_t* bytes);
+int bsk_triples_plain(int dtype, const void* fa, const void* fb, const void* fc,
+                      long long shell_stride, long long ncell, int ntri, const int* tri_dev,
+                      int nchunk, double scale, double* out_dev, void* work, void* stream);
+
+/* LDS-tiled triple sums over fields[0..nfield) (stride shell_stride).
+ * nsum = 1: out[t] = scale * sum_x F[i] F[j] F[k].
+ * nsum = 4: fields are nfield/2 unit shells then nfield/2 |k| shells and
+ *   out[4t+0] = scale * sum n_i n_j n_k, out[4t+1..3] = scale * sum of the
+ *   products with n_i, n_j, n_k replaced by the |k| shell in turn.
+ * Replaces the N_tri and k-mean sums of measure_gridinfo_faster
+ * (bskit/main.py:2024-2064).  work: bsk_triples_tiled_workspace() bytes. */
+int bsk_triples_tiled_workspace(int ntri, int nsum, int nchunk, size_t* bytes);
+int bsk_triples_tiled(int dtype, const void* fields, long long shell_stride, int nfield,
+                      long long ncell, int ntri, const int* tri_dev, int nsum, int nchunk,
+                      double scale, double* out_dev, void* work, void* stream);
+
+/* Split-BF16 MFMA triple sums: for each i, the GEMM (D o d_i) . D^T over
+ * the cells with the pair products split into hi+lo bf16 (hi*hi + hi*lo +
+ * lo*hi on v_mfma_f32_16x16x32_bf16, FP32 accumulation, FP64 final
+ * reduction).  Shells are BF16X2 planes (1..48 shells per set).  mode
+ * BSK_MODE_AUTO uses (hi_j,lo_j) for all three sides; BSK_MODE_AAB adds
+ * the k side; BSK_MODE_ABC uses all three sets.  slot_dev comes from
+ * bsk_mfma_slots for the same (nshell, mode, triangles).  Replaces the same
+ * loop as bsk_triples_plain (bskit/main.py:1871-1882) and, on unit shells,
+ * the N_tri sum (:2027-2031).
+ * work: bsk_triples_mfma_workspace(nshell, mode, nwg) bytes. */
+int bsk_mfma_tiles_per_wg(int nshell, int mode, int* tiles);
+int bsk_mfma_slots(int nshell, int mode, int ntri, const int* tri_host, int* slot_host);
+int bsk_triples_mfma_workspace(int nshell, int mode, int nwg, size_t* bytes);
+int bsk_triples_mfma(int mode, int nshell, const uint16_t* hi_i, const uint16_t* lo_i,
+                     const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
+                     const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
+                     int ntri, const int* slot_dev, double scale, double* out_dev, void* work,
+                     void* stream);
+
+/* Cloud-in-cell assignment of npart particles (pos: npart x 3 float, box
+ * units, C order) to the periodic mesh grid (nmesh^3 float, overwritten).
+ * Mesh node i sits at i*box/nmesh.  With to_contrast the result is
+ * delta = rho/rho_mean - 1.  Replaces nbodykit's
+ * CatalogSource.to_mesh(window='cic') painting as called by
+ * scripts/measure/measure_bs_fast.py:209-217 (the compensation is
+ * bsk_plan_set_compensation). */
+int bsk_cic_paint(int nmesh, double box_size, long long npart, const float* pos, float* grid,
+                  int to_contrast, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

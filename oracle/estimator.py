@@ -1,0 +1,127 @@
+"""FP64 numpy restatement of the reference's FFT bispectrum estimator.
+
+Test infrastructure only (see ``oracle/__init__``).
+
+Conventions restated from the reference:
+
+* forward transform of the input mesh, normalised by 1/N^3
+  (``bskit/main.py:1608-1621`` -> pmesh ``paint(mode='complex')``);
+* per k-bin closed-interval shell mask kL <= |k| <= kU
+  (``main.py:1850-1852``; unit shells ``:314-320``; |k| shells ``:264-268``)
+  followed by an *unnormalised* c2r (``paint(mode='real')``);
+* fast-path unnormalised B = L^6/N^3 * sum_x d_i d_j d_k * u^6
+  (``main.py:1875-1882``);
+* grid info N_tri = sum_x n_i n_j n_k / N^3 and
+  kbar_1 = sum_x kk_i n_j n_k / N^3 / N_tri (and cyclically), divided by u
+  (``main.py:2024-2064``);
+* slow path B = L^6/N^3 * sum_x d_i d_j d_k / N_tri * u^6 with the field roles
+  <AAA>, <AAB>, <ABC> of ``main.py:624-640``.
+
+Shell membership is decided on integer |n|^2 against per-edge thresholds
+(``shell_limits``): a mode on an edge within 1e-9 relative belongs to the bin
+(closed interval, ties in both neighbouring bins).  The reference's float
+predicate is ambiguous exactly on such ties; away from ties both agree.
+"""
+import numpy as np
+
+TIE_RTOL = 1e-9
+
+
+def kf_of(box):
+    return 2.0 * np.pi / float(box)
+
+
+def shell_limits(kL, kU, kf):
+    """Integer |n|^2 range [lo, hi] of the closed shell kL <= |k| <= kU."""
+    kL = np.atleast_1d(np.asarray(kL, dtype=np.float64))
+    kU = np.atleast_1d(np.asarray(kU, dtype=np.float64))
+    lo = np.empty(len(kL), dtype=np.int64)
+    hi = np.empty(len(kU), dtype=np.int64)
+    for b in range(len(kL)):
+        a = (kL[b] / kf) ** 2 if kL[b] > 0 else 0.0
+        c = (kU[b] / kf) ** 2 if kU[b] > 0 else -1.0
+        lo[b] = int(np.ceil(a * (1.0 - TIE_RTOL))) if a > 0 else 0
+        hi[b] = int(np.floor(c * (1.0 + TIE_RTOL))) if c >= 0 else -1
+    return lo, hi
+
+
+def n2_half(N):
+    """Integer |n|^2 on the r2c grid (N, N, N//2+1), components in [-N/2, N/2]."""
+    f = np.fft.fftfreq(N, 1.0 / N).astype(np.int64)
+    fz = np.arange(N // 2 + 1, dtype=np.int64)
+    return f[:, None, None] ** 2 + f[None, :, None] ** 2 + fz[None, None, :] ** 2
+
+
+def forward(field):
+    """delta(k) = (1/N^3) sum_x delta(x) e^{-ik.x} on the half grid, FP64."""
+    field = np.asarray(field, dtype=np.float64)
+    return np.fft.rfftn(field) / field.size
+
+
+def shell_field(dk_half, n2, lo, hi, N, kind='delta', kf=1.0):
+    """Unnormalised c2r of the masked half spectrum.
+
+    kind 'delta': mask * delta(k); 'unit': mask; 'kmag': |k| * mask.
+    """
+    m = (n2 >= lo) & (n2 <= hi)
+    if kind == 'delta':
+        spec = np.where(m, dk_half, 0.0)
+    elif kind == 'unit':
+        spec = m.astype(np.complex128)
+    elif kind == 'kmag':
+        spec = np.where(m, kf * np.sqrt(n2.astype(np.float64)), 0.0).astype(np.complex128)
+    else:
+        raise ValueError(kind)
+    return np.fft.irfftn(spec, s=(N, N, N)) * float(N) ** 3
+
+
+def shells(dk_half, N, lo, hi, kind='delta', kf=1.0):
+    n2 = n2_half(N)
+    return [shell_field(dk_half, n2, lo[b], hi[b], N, kind, kf) for b in range(len(lo))]
+
+
+def triple_sums(fa, fb, fc, tri):
+    """sum_x A_i B_j C_k for every (i, j, k) in tri, FP64."""
+    out = np.empty(len(tri), dtype=np.float64)
+    for t, (i, j, k) in enumerate(tri):
+        out[t] = np.sum(fa[i] * fb[j] * fc[k])
+    return out
+
+
+def grid_info(N, box, lo, hi, tri, units=1.0):
+    """(N_tri float64[T], kbar float64[T,3]) by the FFT method in FP64."""
+    kf = kf_of(box)
+    n = shells(None, N, lo, hi, 'unit')
+    kk = shells(None, N, lo, hi, 'kmag', kf)
+    ncell = float(N) ** 3
+    ntri = triple_sums(n, n, n, tri) / ncell
+    kbar = np.empty((len(tri), 3))
+    with np.errstate(divide='ignore', invalid='ignore'):
+        kbar[:, 0] = triple_sums(kk, n, n, tri) / ncell / ntri
+        kbar[:, 1] = triple_sums(n, kk, n, tri) / ncell / ntri
+        kbar[:, 2] = triple_sums(n, n, kk, tri) / ncell / ntri
+    return ntri, kbar / units
+
+
+def bispectrum_unnorm(fields, box, lo, hi, tri, units=1.0):
+    """Unnormalised B for 1, 2 or 3 real input meshes (roles of main.py:624-640).
+
+    fields: list of 1-3 real N^3 arrays.  Returns float64[T].
+    """
+    N = fields[0].shape[0]
+    ncell = float(N) ** 3
+    dks = [forward(f) for f in fields]
+    sh = [shells(dk, N, lo, hi, 'delta') for dk in dks]
+    if len(sh) == 1:
+        a, b, c = sh[0], sh[0], sh[0]
+    elif len(sh) == 2:
+        a, b, c = sh[0], sh[0], sh[1]
+    else:
+        a, b, c = sh[0], sh[1], sh[2]
+    return triple_sums(a, b, c, tri) * float(box) ** 6 / ncell * units ** 6
+
+
+def full_dk(field):
+    """Full complex delta(k) (FFT index order, 1/N^3) for the direct enumerator."""
+    field = np.asarray(field, dtype=np.float64)
+    return np.fft.fftn(field) / field.size

@@ -1,0 +1,223 @@
+"""Kernel-level parity of the HIP path against the FP64 oracle (GPU)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip('torch')
+
+from oracle import estimator as est  # noqa: E402
+from oracle import direct  # noqa: E402
+
+
+def _engine(N, box, k_low, k_high):
+    from bskit_amd.engine import GridEngine
+    return GridEngine(N, box, k_low, k_high)
+
+
+def _bins(nb, width, box):
+    kf = 2 * np.pi / box
+    lo = kf * (1 + width * np.arange(nb))
+    return lo, lo + width * kf
+
+
+def _field(N, seed, dtype):
+    rng = np.random.default_rng(seed)
+    f = rng.standard_normal((N, N, N))
+    return (f + 0.3 * (f * f - 1)).astype(dtype)
+
+
+def _compact(dk_half, eng):
+    """Gather numpy's half spectrum onto the engine's compact cube [ry][rx][kz]."""
+    N = eng.N
+    n = eng.nlo + np.arange(eng.MX)
+    idx = np.mod(n, N)
+    return dk_half[idx][:, idx][:, :, :eng.MZ].transpose(1, 0, 2)
+
+
+@pytest.mark.parametrize('N,dtype', [(32, np.float32), (48, np.float32), (64, np.float64),
+                                     (96, np.float32)])
+def test_forward_matches_numpy(N, dtype):
+    box = 500.0
+    lo, hi = _bins(4, 3, box)
+    eng = _engine(N, box, lo, hi)
+    f = _field(N, 1, dtype)
+    dk = eng.forward(torch.from_numpy(f).cuda()).cpu().numpy()
+    ref = _compact(est.forward(f), eng)
+    err = np.abs(dk - ref).max() / np.abs(ref).max()
+    print('forward', N, dtype.__name__, 'max rel err', err)
+    assert err < (2e-6 if dtype == np.float32 else 1e-13)
+
+
+@pytest.mark.parametrize('N', [16, 32, 48, 64])
+def test_forward_full_cube(N):
+    # bins reaching past Nyquist keep the whole axis (MX == N)
+    box = 100.0
+    kf = 2 * np.pi / box
+    eng = _engine(N, box, [kf], [kf * N])
+    assert eng.MX == N and eng.MZ == N // 2 + 1
+    f = _field(N, 2, np.float64)
+    dk = eng.forward(torch.from_numpy(f).cuda()).cpu().numpy()
+    ref = _compact(est.forward(f), eng)
+    assert np.abs(dk - ref).max() / np.abs(ref).max() < 1e-13
+
+
+@pytest.mark.parametrize('N,nb,width', [(32, 5, 2), (48, 6, 3), (64, 8, 2)])
+def test_delta_shells(N, nb, width):
+    from bskit_amd import _lib as L
+    box = 700.0
+    lo, hi = _bins(nb, width, box)
+    eng = _engine(N, box, lo, hi)
+    f64 = _field(N, 3, np.float64)
+    dk_ref = est.forward(f64)
+    ref = np.stack(est.shells(dk_ref, N, eng.n2lo, eng.n2hi, 'delta')).reshape(nb, -1)
+    scale = np.abs(ref).max()
+    # FP64 pipeline
+    d64 = eng.forward(torch.from_numpy(f64).cuda())
+    s64 = eng.shells(d64, fmt=L.FMT_F64, dtype=torch.float64).data.cpu().numpy()
+    assert np.abs(s64 - ref).max() / scale < 1e-12
+    # FP32 pipeline, F32 and split-BF16 storage
+    d32 = eng.forward(torch.from_numpy(f64.astype(np.float32)).cuda())
+    s32 = eng.shells(d32, fmt=L.FMT_F32).data.cpu().numpy()
+    sbf = eng.shells(d32, fmt=L.FMT_BF16X2).as_float().cpu().numpy()
+    e32 = np.abs(s32 - ref).max() / scale
+    ebf = np.abs(sbf - ref).max() / scale
+    print('shells N=%d fp32 %.2e bf16x2 %.2e' % (N, e32, ebf))
+    assert e32 < 3e-6
+    assert ebf < 3e-5
+
+
+@pytest.mark.parametrize('N,nb,width', [(32, 6, 2), (48, 5, 3)])
+def test_unit_and_kmag_shells(N, nb, width):
+    from bskit_amd import _lib as L
+    box = 300.0
+    lo, hi = _bins(nb, width, box)
+    eng = _engine(N, box, lo, hi)
+    kf = eng.kf
+    for kind, name in ((L.KIND_UNIT, 'unit'), (L.KIND_KMAG, 'kmag')):
+        ref = np.stack(est.shells(None, N, eng.n2lo, eng.n2hi, name, kf)).reshape(nb, -1)
+        got = eng.shells(None, kind=kind, fmt=L.FMT_F64, dtype=torch.float64).data.cpu().numpy()
+        assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12
+
+
+def _bcheck(got, ref, absum, tol):
+    # error relative to the sum of |terms| (cancellation-aware) and plain relative
+    err = np.abs(got - ref) / absum
+    return err.max() < tol, err.max()
+
+
+@pytest.mark.parametrize('N,nb,width', [(32, 6, 2), (48, 7, 2), (64, 12, 2), (64, 20, 1),
+                                        (32, 33, 0.5)])
+def test_triples_all_paths(N, nb, width):
+    from bskit_amd import _lib as L
+    from bskit_amd import bins
+    box = 1000.0
+    lo, hi = _bins(nb, width, box)
+    eng = _engine(N, box, lo, hi)
+    kLU = np.stack([lo, hi], axis=1)
+    tri = bins.closed_triangle_indices(kLU, 1)
+    f64 = _field(N, 5, np.float64)
+    shells_ref = est.shells(est.forward(f64), N, eng.n2lo, eng.n2hi, 'delta')
+    ref = est.triple_sums(shells_ref, shells_ref, shells_ref, tri)
+    absum = np.array([np.sum(np.abs(shells_ref[i] * shells_ref[j] * shells_ref[k]))
+                      for i, j, k in tri])
+    d32 = eng.forward(torch.from_numpy(f64.astype(np.float32)).cuda())
+    s32 = eng.shells(d32, fmt=L.FMT_F32)
+    sbf = eng.shells(d32, fmt=L.FMT_BF16X2)
+    plain = eng.triples([s32], tri, path='plain').cpu().numpy()
+    tiled = eng.triples([s32], tri, path='tiled').cpu().numpy()
+    mfma = eng.triples([sbf], tri, path='mfma').cpu().numpy()
+    ok_p, ep = _bcheck(plain, ref, absum, 2e-6)
+    ok_t, et = _bcheck(tiled, ref, absum, 2e-6)
+    ok_m, em = _bcheck(mfma, ref, absum, 5e-5)
+    print('N=%d nb=%d T=%d plain %.2e tiled %.2e mfma %.2e' % (N, nb, len(tri), ep, et, em))
+    assert ok_p and ok_t and ok_m
+
+
+def test_triples_fp64_plain():
+    from bskit_amd import _lib as L
+    from bskit_amd import bins
+    N, nb, box = 32, 6, 800.0
+    lo, hi = _bins(nb, 2, box)
+    eng = _engine(N, box, lo, hi)
+    tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), 1)
+    f64 = _field(N, 6, np.float64)
+    sh = est.shells(est.forward(f64), N, eng.n2lo, eng.n2hi, 'delta')
+    ref = est.triple_sums(sh, sh, sh, tri)
+    d = eng.forward(torch.from_numpy(f64).cuda())
+    s = eng.shells(d, fmt=L.FMT_F64, dtype=torch.float64)
+    for path in ('plain', 'tiled'):
+        got = eng.triples([s], tri, path=path).cpu().numpy()
+        assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-11, path
+
+
+@pytest.mark.parametrize('N,nb,width', [(32, 6, 2), (48, 8, 2)])
+def test_cross_modes(N, nb, width):
+    from bskit_amd import _lib as L
+    from bskit_amd import bins
+    box = 1000.0
+    lo, hi = _bins(nb, width, box)
+    eng = _engine(N, box, lo, hi)
+    kLU = np.stack([lo, hi], axis=1)
+    fa, fb, fc = (_field(N, s, np.float64) for s in (11, 12, 13))
+    sh = [est.shells(est.forward(f), N, eng.n2lo, eng.n2hi, 'delta') for f in (fa, fb, fc)]
+    sets_bf = [eng.shells(eng.forward(torch.from_numpy(f.astype(np.float32)).cuda()),
+                          fmt=L.FMT_BF16X2) for f in (fa, fb, fc)]
+    sets_32 = [eng.shells(eng.forward(torch.from_numpy(f.astype(np.float32)).cuda()),
+                          fmt=L.FMT_F32) for f in (fa, fb, fc)]
+    for nf in (2, 3):
+        tri = bins.closed_triangle_indices(kLU, nf)
+        sides = (sh[0], sh[0], sh[1]) if nf == 2 else (sh[0], sh[1], sh[2])
+        ref = est.triple_sums(*sides, tri)
+        absum = np.array([np.sum(np.abs(sides[0][i] * sides[1][j] * sides[2][k]))
+                          for i, j, k in tri])
+        got_m = eng.triples(sets_bf[:nf], tri, path='mfma').cpu().numpy()
+        got_p = eng.triples(sets_32[:nf], tri, path='plain').cpu().numpy()
+        ok_m, em = _bcheck(got_m, ref, absum, 5e-5)
+        ok_p, ep = _bcheck(got_p, ref, absum, 2e-6)
+        print('cross nf=%d T=%d mfma %.2e plain %.2e' % (nf, len(tri), em, ep))
+        assert ok_m and ok_p
+
+
+@pytest.mark.parametrize('N,nb,width', [(32, 6, 2), (48, 8, 2)])
+def test_grid_info_exact_counts(N, nb, width):
+    from bskit_amd import bins
+    box = 1000.0
+    lo, hi = _bins(nb, width, box)
+    eng = _engine(N, box, lo, hi)
+    tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), 1)
+    ntri, kbar = eng.grid_info(tri)
+    cnt, ksum, _ = direct.enumerate_triangles(N, eng.n2lo, eng.n2hi, tri)
+    assert np.array_equal(ntri.astype(np.int64), cnt)
+    nz = cnt > 0
+    kref = ksum[nz] / cnt[nz, None] * eng.kf
+    assert np.abs(kbar[nz] / kref - 1).max() < 1e-10
+    assert np.isnan(kbar[~nz]).all()
+
+
+def test_grid_info_slabs_match_resident():
+    from bskit_amd import bins
+    N, nb, box = 48, 6, 1000.0
+    lo, hi = _bins(nb, 2, box)
+    eng = _engine(N, box, lo, hi)
+    tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), 1)
+    a = eng.grid_info(tri)
+    b = eng.grid_info(tri, max_bytes=N * N * 8 * 2 * nb * 5)  # forces 4-plane slabs
+    assert np.array_equal(a[0], b[0])
+    assert np.allclose(a[1], b[1], rtol=1e-12, equal_nan=True)
+
+
+@pytest.mark.parametrize('N,nb,width', [(32, 6, 2), (32, 10, 1)])
+def test_mfma_unit_shell_counts_exact(N, nb, width):
+    """N_tri from the split-BF16 MFMA path on unit shells rounds to the direct count."""
+    from bskit_amd import _lib as L
+    from bskit_amd import bins
+    box = 1000.0
+    lo, hi = _bins(nb, width, box)
+    eng = _engine(N, box, lo, hi)
+    tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), 1)
+    unit = eng.shells(None, kind=L.KIND_UNIT, fmt=L.FMT_BF16X2, dtype=torch.float32)
+    s = eng.triples([unit], tri, path='mfma', scale=1.0 / N ** 3).cpu().numpy()
+    cnt, _, _ = direct.enumerate_triangles(N, eng.n2lo, eng.n2hi, tri)
+    print('max |count - mfma|', np.abs(s - cnt).max(), 'max count', cnt.max())
+    assert np.array_equal(np.rint(s).astype(np.int64), cnt)
