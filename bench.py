@@ -1,0 +1,298 @@
+#!/usr/bin/env python
+"""Benchmark of the FFT bispectrum path on one MI355X.
+
+Workload (the metric's configuration): 256^3 lattice particles displaced by
+the Zel'dovich field of a seeded Gaussian (P(k) = 2e4 (k/0.1)^-1.5,
+sigma_disp ~ 5 Mpc/h), CIC-painted on the GPU to a 512^3 mesh in
+L = 1000 Mpc/h with CIC window compensation; 32 k bins of width 2 k_f from
+k_f, measured on the union of the bin sets all triangles (3624),
+equilateral (32), isosceles m=2 and squeezed with |k1| = k_f (a 33rd shell
+[k_f, k_f]; 31 triangles): T = 3655 triangles over 33 shells.
+
+One step ("one grid") = forward FFT of the HBM-resident painted field +
+33 shell fields + all T triangle sums + normalisation by the cached N_tri.
+Grid info (N_tri, k-means) is computed once (FP64, exact counts) and
+reported separately.  Two paths are timed: split-BF16 MFMA (the value) and
+the reference-structured FP32 sweep per triangle; both are compared with
+the FP64 path for the relative B error.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = ('grids/sec at 512^3, 32 bins (split-BF16 MFMA vs FP32 plain) + p50/p99 job latency')
+HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(args, torch, np):
+    from bskit_amd import bins as B
+    from bskit_amd import synth
+    from bskit_amd.engine import cic_paint
+    N, L, nb = args.nmesh, args.box, args.nbin
+    kf = 2 * math.pi / L
+    kLU = B.generate_bin_edge_list(kf, kf + nb * 2 * kf + kf, 2 * kf)
+    assert len(kLU) == nb
+    k_low = np.concatenate([kLU[:, 0], [kf]])
+    k_high = np.concatenate([kLU[:, 1], [kf]])
+    special = nb  # shell [k_f, k_f]: |k1| = k_f exactly
+    sets = {
+        'all': B.closed_triangle_indices(kLU, 1),
+        'equilateral': B.generate_equilateral_triangle_bin_list(kf, kf + nb * 2 * kf + kf, 2 * kf,
+                                                                return_indices=True),
+        'isosceles_m2': B.generate_isosceles_triangle_bin_list(kf, kf + nb * 2 * kf + kf, 2 * kf,
+                                                               isos_mult=2, return_indices=True),
+        'squeezed_kf': np.array([[special, j, j] for j in range(1, nb)]),
+    }
+    canon = set()
+    tri = []
+    for name, t in sets.items():
+        for row in np.asarray(t).tolist():
+            key = tuple(sorted(row, reverse=True))
+            if key not in canon:
+                canon.add(key)
+                tri.append(key)
+    tri = np.array(tri, dtype=np.int32)
+    t0 = time.perf_counter()
+    pos = synth.zeldovich_particles(args.npart_side, L, seed=args.seed, sigma_disp=5.0)
+    t_pos = time.perf_counter() - t0
+    pos_d = torch.from_numpy(pos).cuda()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    field = cic_paint(pos_d, N, L)
+    torch.cuda.synchronize()
+    t_cic = time.perf_counter() - t0
+    del pos_d
+    return dict(N=N, L=L, kf=kf, k_low=k_low, k_high=k_high, tri=tri, sets=sets, field=field,
+                field_host=None, t_pos=t_pos, t_cic=t_cic, pos=pos)
+
+
+class StageTimer:
+    """HIP-event timing of the stages of one step on torch's current stream."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.ev = {}
+
+    def mark(self, name):
+        e = self.torch.cuda.Event(enable_timing=True)
+        e.record()
+        self.ev.setdefault(name, []).append(e)
+
+    def durations(self, a, b):
+        return [x.elapsed_time(y) for x, y in zip(self.ev[a], self.ev[b])]
+
+
+def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
+    from bskit_amd import _lib as L
+    N, L_ = wl['N'], wl['L']
+    scale = L_ ** 6 / float(N) ** 3
+    shells = eng.alloc_shells(fmt, eng.nb)
+    out = torch.empty(len(wl['tri']), dtype=torch.float64, device='cuda')
+    dtype = torch.float64 if fmt == L.FMT_F64 else torch.float32
+    field = wl['field'].to(dtype)
+
+    def step(t):
+        if t is not None:
+            t.mark('start')
+        dk = eng.forward(field)
+        if t is not None:
+            t.mark('fwd')
+        eng.shells(dk, fmt=fmt, dtype=dtype, out=shells)
+        if t is not None:
+            t.mark('shells')
+        eng.triples([shells], wl['tri'], path=path, scale=scale, out=out)
+        if t is not None:
+            t.mark('triples')
+        b = out / ntri_dev
+        if t is not None:
+            t.mark('end')
+        return b
+
+    for _ in range(warmup):
+        step(None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b = step(timer)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    return b, elapsed
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--plain-steps', type=int, default=2)
+    ap.add_argument('--nmesh', type=int, default=512)
+    ap.add_argument('--nbin', type=int, default=32)
+    ap.add_argument('--npart-side', type=int, default=256)
+    ap.add_argument('--box', type=float, default=1000.0)
+    ap.add_argument('--seed', type=int, default=1)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline sample')
+    ap.add_argument('--no-fp64', action='store_true', help='skip the FP64 accuracy reference')
+    ap.add_argument('--no-plain', action='store_true', help='skip the FP32 plain path')
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
+        print('bench.py measures one GPU: the FFT bispectrum path is sharded by job across GPUs '
+              '(no cross-GPU traffic), so run one process per GPU; --gpus %d is not supported.'
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from bskit_amd import _lib as L
+    from bskit_amd.engine import GridEngine
+
+    if not torch.cuda.is_available():
+        print('bench.py needs a GPU', file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.init()
+    log('[bench] building workload N=%d nbin=%d' % (args.nmesh, args.nbin))
+    wl = build_workload(args, torch, np)
+    T = len(wl['tri'])
+    eng = GridEngine(wl['N'], wl['L'], wl['k_low'], wl['k_high'], cic_window=wl['N'])
+    log('[bench] T=%d shells=%d compact m=%d' % (T, eng.nb, eng.m))
+
+    t0 = time.perf_counter()
+    ntri, kbar = eng.grid_info(wl['tri'])
+    torch.cuda.synchronize()
+    t_gridinfo = time.perf_counter() - t0
+    log('[bench] grid info %.2fs, N_tri range %d..%d' % (t_gridinfo, ntri.min(), ntri.max()))
+    ntri_dev = torch.from_numpy(np.where(ntri > 0, ntri, np.nan)).cuda()
+
+    timer = StageTimer(torch)
+    b_mfma, el_mfma = run_path(eng, wl, 'mfma', L.FMT_BF16X2, args.steps, args.warmup, torch,
+                               ntri_dev, timer)
+    ms_step = el_mfma / args.steps * 1e3
+    grids_mfma = args.steps / el_mfma
+    log('[bench] mfma: %.3f ms/grid, %.2f grids/s' % (ms_step, grids_mfma))
+    stage_ms = {
+        'forward': float(np.mean(timer.durations('start', 'fwd'))),
+        'shells': float(np.mean(timer.durations('fwd', 'shells'))),
+        'triples_mfma': float(np.mean(timer.durations('shells', 'triples'))),
+        'normalise': float(np.mean(timer.durations('triples', 'end'))),
+    }
+    log('[bench] stages (ms): %s' % stage_ms)
+    b_mfma = b_mfma.cpu().numpy()
+
+    res_plain = None
+    if not args.no_plain:
+        tp = StageTimer(torch)
+        b_plain, el_plain = run_path(eng, wl, 'plain', L.FMT_F32, args.plain_steps, 1, torch,
+                                     ntri_dev, tp)
+        res_plain = {'grids_per_s': args.plain_steps / el_plain,
+                     'ms_per_grid': el_plain / args.plain_steps * 1e3,
+                     'triples_ms': float(np.mean(tp.durations('shells', 'triples')))}
+        b_plain = b_plain.cpu().numpy()
+        log('[bench] fp32 plain: %.1f ms/grid' % res_plain['ms_per_grid'])
+
+    acc = {}
+    if not args.no_fp64:
+        b64, _ = run_path(eng, wl, 'plain', L.FMT_F64, 1, 0, torch, ntri_dev)
+        b64 = b64.cpu().numpy()
+        ok = np.isfinite(b64) & (b64 != 0)
+        rel_m = np.abs(b_mfma[ok] / b64[ok] - 1)
+        acc['bf16x3_max_rel_err'] = float(rel_m.max())
+        acc['bf16x3_median_rel_err'] = float(np.median(rel_m))
+        acc['bf16x3_frac_bins_above_1e-3'] = float(np.mean(rel_m > 1e-3))
+        if res_plain is not None:
+            rel_p = np.abs(b_plain[ok] / b64[ok] - 1)
+            acc['fp32_max_rel_err'] = float(rel_p.max())
+        log('[bench] accuracy vs FP64: %s' % acc)
+
+    # roofline of the dominant kernel (the MFMA triple sum over all shells)
+    N = wl['N']
+    ncell = float(N) ** 3
+    nsh = eng.nb
+    shell_bytes = nsh * 4 * ncell               # split-BF16 hi+lo, read once
+    flops = 2.0 * T * ncell
+    t_tri = stage_ms['triples_mfma'] * 1e-3
+    H = N * N * (N // 2 + 1)
+    t_sh = stage_ms['shells'] * 1e-3
+    shells_alg = 8.0 * H + nsh * 4 * ncell       # delta(k) read + shells written
+    dominant = 'triples_mfma' if t_tri >= t_sh else 'shells'
+    if dominant == 'triples_mfma':
+        ai = flops / shell_bytes
+        bound = 'hbm' if ai * HBM_PEAK_GBS * 1e9 < BF16_PEAK_TFLOPS * 1e12 else 'mfma'
+        achieved = shell_bytes / t_tri / 1e9
+        roof = {'kernel': 'k_triple_mfma<3,0,8,2> + k_reduce_mfma', 'bound': bound,
+                'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                'useful_tflops': flops / t_tri / 1e12, 'arith_intensity_flop_per_byte': ai}
+    else:
+        achieved = shells_alg / t_sh / 1e9
+        roof = {'kernel': 'shell passes (k_inv_x_mask, k_inv_y, k_inv_z)', 'bound': 'hbm',
+                'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': achieved / HBM_PEAK_GBS, 'traffic': None}
+    stage_gbs = {'shells_GBps': shells_alg / t_sh / 1e9,
+                 'forward_GBps': (4.0 * ncell + 8.0 * H) / (stage_ms['forward'] * 1e-3) / 1e9}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import baseline
+        from oracle import estimator as est
+        field_h = wl['field'].cpu().numpy()
+        lo, hi = est.shell_limits(wl['k_low'], wl['k_high'], wl['kf'])
+        log('[bench] cpu baseline sample...')
+        r = baseline.time_sample(field_h, lo, hi, wl['tri'], n_shell_sample=2, n_tri_sample=6)
+        cpu = {'value': 1.0 / r['seconds_per_grid'], 'unit': 'grids/s', 'cores': r['cores'],
+               'kind': 'port',
+               'sample': ('reference-structured numpy/scipy path on %d host threads: full forward '
+                          'r2c (%.1fs), %d shell c2r (%.2fs each), %d triangle sums (%.2fs each); '
+                          'extrapolated to %d shells and %d triangles'
+                          % (r['cores'], r['forward_s'], r['nshell_sampled'], r['shell_s'],
+                             r['ntri_sampled'], r['triangle_s'], nsh, T))}
+        log('[bench] cpu: %.1f s/grid' % r['seconds_per_grid'])
+
+    line = {
+        'metric': METRIC,
+        'value': grids_mfma,
+        'unit': 'grids/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': ms_step,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'bf16',
+        'data': 'synthetic',
+        'config': {'workload': ('%d^3 CIC mesh of %d^3 Zel\'dovich particles, L=%g Mpc/h, %d bins '
+                                'of 2k_f from k_f; all+equilateral+isosceles(m=2)+squeezed(|k1|=k_f)'
+                                % (N, args.npart_side, wl['L'], args.nbin)),
+                   'nmesh': N, 'nbin': args.nbin, 'nshell': nsh, 'ntri': T,
+                   'path': 'split-BF16 MFMA (hi*hi+hi*lo+lo*hi), FP32 accumulate, FP64 reduce',
+                   'parallelism': 'single GPU'},
+        'fp32_plain': res_plain,
+        'speedup_vs_fp32_plain': (grids_mfma / res_plain['grids_per_s']) if res_plain else None,
+        'stage_ms': stage_ms,
+        'stage_GBps': stage_gbs,
+        'gridinfo_once_s': t_gridinfo,
+        'accuracy_vs_fp64': acc,
+        'latency': None,
+        'roofline': roof,
+        'cpu_baseline': cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -54,6 +54,8 @@ _SIGS = {
     'bsk_triples_mfma': [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _c_int, _c_int,
                          _vp, _c_double, _vp, _vp, _vp],
     'bsk_cic_paint': [_c_int, _c_double, _c_ll, _vp, _vp, _c_int, _vp],
+    'bsk_direct_triangles': [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp,
+                             _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp],
 }
 _RESTYPE = {'bsk_last_error': ctypes.c_char_p}
 

@@ -231,7 +231,7 @@ class GridEngine:
         ncell = self.ncell if ncell is None else ncell
         return int(max(1, min(4096, ncell // 65536)))
 
-    def triples(self, sets, tri, path=PATH_MFMA, scale=1.0, out=None, stream=None):
+    def triples(self, sets, tri, path=PATH_MFMA, scale=1.0, out=None, stream=None, nwg=None):
         """scale * sum_x A_i B_j C_k for every (i, j, k) row of tri.
 
         sets: [A] (auto), [A, B] (<AAB>: sides A, A, B) or [A, B, C].
@@ -261,7 +261,7 @@ class GridEngine:
                 if x.fmt != L.FMT_BF16X2:
                     raise ValueError('the MFMA path needs BF16X2 shells')
             mode = _MODE[nf]
-            nwg = self.mfma_nwg(ncell)
+            nwg = self.mfma_nwg(ncell) if nwg is None else int(nwg)
             slots = self.mfma_slots(tri, sa.nb, mode)
             nbytes = L.size_out()
             L.call('bsk_triples_mfma_workspace', sa.nb, mode, nwg, ctypes.byref(nbytes))

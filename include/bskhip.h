@@ -170,6 +170,20 @@ int bsk_triples_mfma(int mode, int nshell, const uint16_t* hi_i, const uint16_t*
 int bsk_cic_paint(int nmesh, double box_size, long long npart, const float* pos, float* grid,
                   int to_contrast, void* stream);
 
+/* Direct enumeration of closed triangles from explicit mode lists (int32
+ * (nx,ny,nz) triples, components in [-N/2, N/2)): for every k1 in A and k2
+ * in B, k3 = -(k1+k2) (wrapped mod N when periodic, else rejected when off
+ * the grid) is accepted when n2lo_c <= |n3|^2 <= n2hi_c.  With dk_a/b/c
+ * (FP64 compact cubes from bsk_forward, geometry mx, mz, nlo) the complex
+ * sum of dA(k1) dB(k2) dC(k3) is formed too.  partial: nblocks x 6 doubles
+ * (count, sum|k1|, sum|k2|, sum|k3| in k_f units, Re, Im) to be summed by
+ * the caller.  Replaces the brute-force loops of bk_binned
+ * (bskit/main.py:834-1004). */
+int bsk_direct_triangles(int nmesh, int periodic, const int* modes_a, int na, const int* modes_b,
+                         int nb, int n2lo_c, int n2hi_c, const void* dk_a, const void* dk_b,
+                         const void* dk_c, int mx, int mz, int nlo, int nblocks, double* partial,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

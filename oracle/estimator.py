@@ -72,7 +72,7 @@ def shell_field(dk_half, n2, lo, hi, N, kind='delta', kf=1.0):
         spec = np.where(m, kf * np.sqrt(n2.astype(np.float64)), 0.0).astype(np.complex128)
     else:
         raise ValueError(kind)
-    return np.fft.irfftn(spec, s=(N, N, N)) * float(N) ** 3
+    return np.fft.irfftn(spec, s=(N, N, N), axes=(0, 1, 2)) * float(N) ** 3
 
 
 def shells(dk_half, N, lo, hi, kind='delta', kf=1.0):

@@ -1,0 +1,233 @@
+"""Host-side logic (CPU): bins, output formats, combiner, library ABI surface."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import bskit_amd
+from bskit_amd import bins as pb
+from bskit_amd import io as bio
+from oracle import bins as ob
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_product_bins_match_oracle_random():
+    rng = np.random.default_rng(11)
+    n = 0
+    for _ in range(150):
+        kmin = rng.uniform(0.001, 0.05)
+        dk = rng.uniform(0.002, 0.03)
+        nb = int(rng.integers(1, 12))
+        kmax = kmin + dk * (nb + rng.uniform(0.2, 1.8))
+        nl = int(rng.integers(0, 4))
+        dkh = dk * rng.uniform(1, 3) if nl else -1.
+        e1 = ob.bin_edges(kmin, kmax, dk, nl, dkh)
+        assert np.array_equal(e1, pb.generate_bin_edge_list(kmin, kmax, dk, nl, dkh))
+        if not len(e1):
+            continue
+        n += 1
+        for nf in (1, 2, 3):
+            a = ob.all_triangles(kmin, kmax, dk, nf, nl, dkh, return_indices=True)
+            b = pb.generate_triangle_bin_list(kmin, kmax, dk, num_fields=nf, num_lowk_bins=nl,
+                                              dk_high=dkh, return_indices=True)
+            assert np.array_equal(a, b)
+            assert np.array_equal(ob.all_triangles(kmin, kmax, dk, nf, nl, dkh),
+                                  pb.generate_triangle_bin_list(kmin, kmax, dk, num_fields=nf,
+                                                                num_lowk_bins=nl, dk_high=dkh))
+        for m in (1.0, 2.0, 3.0):
+            assert np.array_equal(ob.isosceles(kmin, kmax, dk, m, 0.1, nl, dkh, True),
+                                  pb.generate_isosceles_triangle_bin_list(kmin, kmax, dk, m, 0.1,
+                                                                          nl, dkh, True))
+    assert n > 100
+
+
+def test_fixture_triangle_list(golden_dir):
+    ref = np.loadtxt(os.path.join(golden_dir, 'Lbox1000_512_kf_3kf_3lowkbins.dat'))
+    e = pb.generate_triangle_bin_list(0.00314, 0.1, 0.00628, num_lowk_bins=3, dk_high=0.01884)
+    assert np.abs(e - ref[:, 4:10]).max() < 5e-10
+
+
+def test_config_triangle_counts():
+    kf = 2 * np.pi / 1000.
+    for nb, w, T, T2 in [(8, 2, 98, 231), (16, 3, 564, 1469), (32, 2, 3624, 10041)]:
+        kmax = kf + nb * w * kf + w * kf / 2
+        assert len(pb.generate_bin_edge_list(kf, kmax, w * kf)) == nb
+        assert len(pb.generate_triangle_bin_list(kf, kmax, w * kf)) == T
+        assert len(pb.generate_triangle_bin_list(kf, kmax, w * kf, num_fields=2)) == T2
+
+
+def test_output_formats_match_fixture_lines(golden_dir):
+    ref_lines = open(os.path.join(golden_dir, 'Lbox1000_512_kf_3kf_3lowkbins.dat')).read().splitlines()
+    ref = np.loadtxt(os.path.join(golden_dir, 'Lbox1000_512_kf_3kf_3lowkbins.dat'))
+    for r, line in zip(ref[:5], ref_lines[:5]):
+        s = bio.format_row(bio.FAST_GRIDINFO, int(r[0]), r[4:10], r[1:4], ntri=r[10])
+        assert s.rstrip('\n') == line
+    ub = open(os.path.join(golden_dir, 'test_grid_512_1_unnormbs_kf_3kf_3lowbins.dat')).read().splitlines()
+    arr = np.loadtxt(os.path.join(golden_dir, 'test_grid_512_1_unnormbs_kf_3kf_3lowbins.dat'))
+    for r, line in zip(arr[:5], ub[:5]):
+        assert bio.format_row(bio.FAST_B, int(r[0]), r[1:7], B=r[7]).rstrip('\n') == line
+    sl = open(os.path.join(golden_dir, 'test_grid_512_1_bs_kf_3kf_3lowbins_slow.dat')).read().splitlines()
+    arr = np.loadtxt(os.path.join(golden_dir, 'test_grid_512_1_bs_kf_3kf_3lowbins_slow.dat'))
+    for r, line in zip(arr[:5], sl[:5]):
+        s = bio.format_row(bio.SLOW_FULL, int(r[0]), r[4:10], r[1:4], B=r[10], ntri=r[11])
+        assert s.rstrip('\n') == line
+
+
+def test_combiner_reproduces_reference_relation(tmp_path, golden_dir):
+    """combine(grid info, unnormalised B) matches the reference's combined file."""
+    gi = os.path.join(golden_dir, 'Lbox1000_512_kf_3kf_3lowkbins.dat')
+    bu = os.path.join(golden_dir, 'test_grid_512_1_unnormbs_kf_3kf_3lowbins.dat')
+    out = tmp_path / 'comb.dat'
+    comb = bio.combine(gi, bu, 0.1, str(out))
+    ref = np.loadtxt(os.path.join(golden_dir, 'test_grid_512_1_bs_comb_kf_3kf_3lowbins.dat'))
+    assert comb.shape == ref.shape
+    assert np.array_equal(comb[:, 0], ref[:, 0])
+    assert np.allclose(comb[:, 1:], ref[:, 1:], rtol=1e-6)
+    header = open(out).readline()
+    ref_header = open(os.path.join(golden_dir, 'test_grid_512_1_bs_comb_kf_3kf_3lowbins.dat')).readline()
+    assert header == ref_header
+
+
+def test_combiner_checks(tmp_path):
+    gi = tmp_path / 'gi.dat'
+    bu = tmp_path / 'b.dat'
+    gi.write_text('0 1 1 1 0.1 0.2 0.1 0.2 0.1 0.2 5\n1 1 1 1 0.2 0.3 0.1 0.2 0.1 0.2 5\n')
+    bu.write_text('0 0.1 0.2 0.1 0.2 0.1 0.2 10\n2 0.2 0.3 0.1 0.2 0.1 0.2 10\n')
+    with pytest.raises(ValueError, match='index mismatch'):
+        bio.combine(str(gi), str(bu), 1.0, str(tmp_path / 'o.dat'))
+    bu.write_text('0 0.1 0.2 0.1 0.2 0.1 0.2 10\n1 0.25 0.3 0.1 0.2 0.1 0.2 10\n')
+    with pytest.raises(ValueError, match='bound mismatch'):
+        bio.combine(str(gi), str(bu), 1.0, str(tmp_path / 'o.dat'))
+
+
+def test_append_rows_chunks_and_resume(tmp_path):
+    p = tmp_path / 'b.dat'
+    idx = np.arange(250)
+    edges = np.tile(np.array([1, 2, 1, 2, 1, 2.]), (250, 1))
+    bio.append_rows(str(p), bio.FAST_B, idx[:130], edges[:130], B=np.ones(130))
+    assert bio.resume_index(str(p)) == 130
+    bio.append_rows(str(p), bio.FAST_B, idx[130:], edges[130:], B=np.ones(120))
+    a = np.loadtxt(p)
+    assert np.array_equal(a[:, 0], idx)
+    # a partial trailing line (interrupted write) is ignored by resume_index
+    with open(p, 'a') as f:
+        f.write('250 1.0')
+    assert bio.resume_index(str(p)) == 250
+
+
+def test_save_bispectrum(tmp_path):
+    b = {'index': np.arange(2), 'k_edge': np.ones((2, 6)), 'k_mean': np.ones((2, 3)),
+         'B': np.array([1., 2.]), 'N_tri': np.array([3., 4.])}
+    p = tmp_path / 's.dat'
+    bio.save_bispectrum(str(p), b)
+    a = np.loadtxt(p)
+    assert a.shape == (2, 11)
+    assert open(p).readline().startswith('# k1_mean')
+
+
+def _declared_symbols():
+    hdr = open(os.path.join(ROOT, 'include', 'bskhip.h')).read()
+    return sorted(set(re.findall(r'^\s*(?:int|const char\*)\s+(bsk_\w+)\(', hdr, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib_path = os.path.join(ROOT, 'bskit_amd', 'libbskhip.so')
+    if not os.path.exists(lib_path):
+        pytest.skip('libbskhip.so not built')
+    out = subprocess.check_output(['nm', '-D', '--defined-only', lib_path]).decode()
+    exported = set(re.findall(r' T (bsk_\w+)', out))
+    declared = _declared_symbols()
+    assert declared, 'no declarations parsed'
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+    # the ctypes table covers the same set
+    from bskit_amd import _lib
+    assert sorted(_lib.exported_symbols()) == declared
+
+
+def test_library_loads_and_host_calls_work():
+    """Host-only ABI calls (no GPU work): thresholds, slots, workspace sizes."""
+    from bskit_amd import _lib as L
+    try:
+        lib = L.lib()
+    except (ImportError, OSError) as e:
+        pytest.skip('library not loadable here: %s' % e)
+    assert lib.bsk_version() == 1
+    lo = (ctypes.c_int * 2)()
+    hi = (ctypes.c_int * 2)()
+    kf = 2 * np.pi / 100.
+    L.call('bsk_shell_thresholds', 64, 100.0, 2, L.dbl_array([kf, 3 * kf]),
+           L.dbl_array([3 * kf, 5 * kf]), lo, hi)
+    assert list(lo) == [1, 9] and list(hi) == [9, 25]
+    tiles = ctypes.c_int()
+    L.call('bsk_mfma_tiles_per_wg', 33, 0, ctypes.byref(tiles))
+    assert tiles.value == 16 * (1 + 3 + 6)
+    tri = np.array([[2, 1, 0], [0, 1, 2], [1, 1, 1]], dtype=np.int32)
+    slots = np.zeros(3, dtype=np.int32)
+    L.call('bsk_mfma_slots', 3, 0, 3, tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+           slots.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    assert slots[0] == slots[1]  # symmetric spectra: any ordering maps to one slot
+    n = ctypes.c_size_t()
+    L.call('bsk_triples_mfma_workspace', 33, 0, 10, ctypes.byref(n))
+    assert n.value == 10 * 160 * 256 * 4
+    with pytest.raises(L.BskError):
+        L.call('bsk_mfma_tiles_per_wg', 60, 0, ctypes.byref(tiles))
+
+
+def test_shell_threshold_abi_matches_oracle():
+    from bskit_amd import _lib as L
+    from oracle import estimator as est
+    try:
+        L.lib()
+    except (ImportError, OSError) as e:
+        pytest.skip('library not loadable here: %s' % e)
+    from bskit_amd.engine import shell_limits
+    rng = np.random.default_rng(3)
+    kf = 2 * np.pi / 1000.
+    lows = rng.uniform(0, 40, 200) * kf
+    highs = lows + rng.uniform(0, 10, 200) * kf
+    # exact ties at odd multiples of k_f
+    lows[:20] = kf * (1 + 2 * np.arange(20))
+    highs[:20] = lows[:20] + 2 * kf
+    a = shell_limits(512, 1000., lows, highs)
+    b = est.shell_limits(lows, highs, kf)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_api_surface_matches_reference_names():
+    names = ['FFTBispectrum', 'k_field', 'number_field', 'compute_Nbin', 'compute_k_means_on_grid',
+             'compute_bk_FFT_value', 'bk_FFT_full', 'bk_FFT_grid_info', 'bk_FFT_unnormalized_value',
+             'bk_binned', 'generate_bin_edge_list', 'generate_equilateral_triangle_bin_list',
+             'generate_squeezed_triangle_bin_list', 'generate_isosceles_triangle_bin_list',
+             'generate_triangle_bin_list', 'geometric_k_mean']
+    for n in names:
+        assert hasattr(bskit_amd, n), n
+    import inspect
+    sig = inspect.signature(bskit_amd.FFTBispectrum.__init__)
+    ref_params = ['source', 'Nmesh', 'BoxSize', 'dk', 'kmin', 'kmax', 'num_lowk_bins', 'dk_high',
+                  'dmu', 'mu_min', 'mu_max', 'pos_units_mpcoverh', 'k_edges', 'second', 'third',
+                  'triangle_type', 'isos_mult', 'isos_tol', 'squeezed_bin_index',
+                  'for_grid_info_only']
+    got = [p for p in sig.parameters if p != 'self'][:len(ref_params)]
+    assert got == ref_params
+    for meth in ('measure_bispectrum', 'measure_bispectrum_faster', 'measure_gridinfo_faster'):
+        params = list(inspect.signature(getattr(bskit_amd.FFTBispectrum, meth)).parameters)
+        assert params[1:7] == ['imin', 'imax', 'kmeas_min', 'kmeas_max', 'out_file', 'verbose']
+
+
+def test_constructor_errors_without_gpu():
+    field = np.zeros((16, 16, 16), dtype=np.float32)
+    with pytest.raises(ValueError, match='Must specify either'):
+        bskit_amd.FFTBispectrum(field, BoxSize=100., for_grid_info_only=True)
+    with pytest.raises(ValueError, match='Need second source'):
+        bskit_amd.FFTBispectrum(field, BoxSize=100., kmin=0.1, kmax=1., third=field,
+                                for_grid_info_only=True)
+    with pytest.raises(ValueError, match='BoxSize is required'):
+        bskit_amd.FFTBispectrum(field, kmin=0.1, kmax=1., for_grid_info_only=True)
+    with pytest.raises(NotImplementedError):
+        bskit_amd.FFTBispectrum(np.zeros((16, 16, 8), np.float32), BoxSize=100., kmin=0.1,
+                                kmax=1., for_grid_info_only=True)

@@ -1,0 +1,172 @@
+"""Pin the oracle against the reference's own fixtures and closed forms (CPU)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import bins as ob
+from oracle import direct
+from oracle import estimator as est
+
+FIX = dict(kmin=0.00314, kmax=0.1, dk=0.00628, num_lowk_bins=3, dk_high=0.01884)
+
+
+def _load(golden_dir, name):
+    return np.loadtxt(os.path.join(golden_dir, name))
+
+
+def test_bin_edges_fixture(golden_dir):
+    ref = _load(golden_dir, 'Lbox1000_512_kf_3kf_3lowkbins.dat')
+    e = ob.bin_edges(FIX['kmin'], FIX['kmax'], FIX['dk'], FIX['num_lowk_bins'], FIX['dk_high'])
+    assert len(e) == 7
+    tri = ob.all_triangles(FIX['kmin'], FIX['kmax'], FIX['dk'], 1, FIX['num_lowk_bins'],
+                           FIX['dk_high'])
+    assert tri.shape == (59, 6)
+    assert np.abs(tri - ref[:, 4:10]).max() < 5e-10
+    assert np.array_equal(ref[:, 0].astype(int), np.arange(59))
+
+
+def test_equilateral_fixture(golden_dir):
+    ref = _load(golden_dir, 'test_grid_512_1_cross_test_grid_512_2_bs_eq_kf_3kf_3lowbins_slow.dat')
+    tri = ob.equilateral(FIX['kmin'], FIX['kmax'], FIX['dk'], FIX['num_lowk_bins'], FIX['dk_high'])
+    assert np.abs(tri - ref[:, 4:10]).max() < 5e-10
+    other = _load(golden_dir, 'test_grid_512_2_cross_test_grid_512_1_bs_eq_kf_3kf_3lowbins_slow.dat')
+    # both cross orderings share N_tri and k-means (grid info is field independent)
+    assert np.array_equal(ref[:, 11], other[:, 11])
+    assert np.array_equal(ref[:, 1:4], other[:, 1:4])
+
+
+def test_bin_quirk_q1():
+    # kmax = kmin + n*dk gives n-1 bins (np.arange excludes the stop)
+    assert len(ob.bin_edges(0.01, 0.01 + 5 * 0.02, 0.02)) == 4
+    assert len(ob.bin_edges(0.01, 0.01 + 5 * 0.02 + 0.01, 0.02)) == 5
+
+
+def test_bin_errors():
+    with pytest.raises(ValueError):
+        ob.bin_edges(0., 1., 0.1)
+    with pytest.raises(ValueError):
+        ob.bin_edges(0.1, 1., 0.1, num_lowk_bins=2)
+    with pytest.raises(ValueError):
+        ob.isosceles(0.1, 1., 0.1, isos_mult=0.5)
+    with pytest.raises(NotImplementedError):
+        ob.all_triangles(0.1, 1., 0.1, mu_min=0.)
+
+
+def test_closed_forms_special_lists():
+    kf = 2 * np.pi / 1000.
+    e = ob.bin_edges(kf, kf + 10 * 2 * kf + kf, 2 * kf)
+    sq = ob.squeezed(kf, kf + 10 * 2 * kf + kf, 2 * kf, squeezed_bin_index=2, return_indices=True)
+    assert np.array_equal(sq, np.array([[2, j, j] for j in range(3, 10)]))
+    iso = ob.isosceles(kf, kf + 10 * 2 * kf + kf, 2 * kf, isos_mult=2, return_indices=True)
+    km = np.array([ob.geometric_k_mean(a, b) for a, b in e])
+    for l, s, s2 in iso:
+        assert s == s2
+        assert abs(km[l] - km[s] / 2) / (km[s] / 2) < 0.1
+    # geometric mean of a thin shell -> its centre
+    assert abs(ob.geometric_k_mean(1.0, 1.0 + 1e-6) - 1.0) < 1e-6
+
+
+def test_direct_counts_gridinfo_fixture_512(golden_dir):
+    """Exact enumeration reproduces N_tri and k-means of the 512^3 fixture."""
+    ref = _load(golden_dir, 'Lbox1000_512_kf_3kf_3lowkbins.dat')
+    e = ob.bin_edges(FIX['kmin'], FIX['kmax'], FIX['dk'], FIX['num_lowk_bins'], FIX['dk_high'])
+    idx = ob.all_triangles(FIX['kmin'], FIX['kmax'], FIX['dk'], 1, FIX['num_lowk_bins'],
+                           FIX['dk_high'], return_indices=True)
+    kf = est.kf_of(1000.)
+    lo, hi = est.shell_limits(e[:, 0], e[:, 1], kf)
+    cnt, ks, _ = direct.enumerate_triangles(512, lo, hi, idx)
+    assert np.array_equal(cnt, ref[:, 10].astype(np.int64))
+    kbar = ks / cnt[:, None] * kf
+    assert np.abs(kbar / ref[:, 1:4] - 1).max() < 5e-7
+
+
+def test_direct_counts_gridinfo_fixture_256(golden_dir):
+    """Block 1 of the 256^3 (L=500) subbox fixture, including empty bins."""
+    ref = _load(golden_dir, 'Lbox1000_512_kf_3kf_3lowkbins_subbox0_block1.dat')
+    e = ob.bin_edges(FIX['kmin'], FIX['kmax'], FIX['dk'], FIX['num_lowk_bins'], FIX['dk_high'])
+    idx = ob.all_triangles(FIX['kmin'], FIX['kmax'], FIX['dk'], 1, FIX['num_lowk_bins'],
+                           FIX['dk_high'], return_indices=True)
+    kf = est.kf_of(500.)
+    lo, hi = est.shell_limits(e[:, 0], e[:, 1], kf)
+    cnt, ks, _ = direct.enumerate_triangles(256, lo, hi, idx)
+    # the fixture's N_tri are integers up to FP noise (row 29 holds -9.06e-14)
+    assert np.abs(cnt - ref[:, 10]).max() < 1e-9
+    ok = cnt > 0
+    assert np.abs(ks[ok] / cnt[ok, None] * kf / ref[ok, 1:4] - 1).max() < 5e-7
+    # empty bins are NaN in the reference when its sum is exactly 0.0
+    zero_exact = ref[:, 10] == 0.0
+    assert np.isnan(ref[zero_exact, 1:4]).all()
+    assert (cnt[zero_exact] == 0).all()
+
+
+def test_fft_counts_equal_direct_counts_small():
+    """FFT estimator counts (FP64 numpy restatement) equal exact enumeration."""
+    N, box = 24, 1000.
+    kf = est.kf_of(box)
+    e = ob.bin_edges(kf, kf + 5 * 2 * kf + kf, 2 * kf)
+    idx = ob.all_triangles(kf, kf + 5 * 2 * kf + kf, 2 * kf, return_indices=True)
+    lo, hi = est.shell_limits(e[:, 0], e[:, 1], kf)
+    nt, kbar = est.grid_info(N, box, lo, hi, idx)
+    cnt, ks, _ = direct.enumerate_triangles(N, lo, hi, idx)
+    assert np.abs(nt - cnt).max() < 1e-6
+    ok = cnt > 0
+    assert np.allclose(kbar[ok], ks[ok] / cnt[ok, None] * kf, rtol=1e-10)
+
+
+def test_periodic_versus_open_closure():
+    """Above ~N/3 the FFT (periodic) count exceeds the non-periodic closure count."""
+    N = 16
+    lo, hi = np.array([25]), np.array([49])  # 5 <= |n| <= 7
+    cnt, _, _ = direct.enumerate_triangles(N, lo, hi, np.array([[0, 0, 0]]))
+    assert cnt[0] == 209586
+    nt, _ = est.grid_info(N, 2 * np.pi, lo, hi, np.array([[0, 0, 0]]))
+    assert abs(nt[0] - cnt[0]) < 1e-6
+
+
+def test_bispectrum_oracle_matches_direct_sum():
+    """FFT-estimator B sums equal brute-force sums of delta(k1)delta(k2)delta(k3)."""
+    N, box = 16, 500.
+    kf = est.kf_of(box)
+    rng = np.random.default_rng(4)
+    g = rng.standard_normal((N, N, N))
+    field = g + 0.4 * (g * g - 1)
+    e = ob.bin_edges(kf, kf + 4 * 1 * kf + kf / 2, kf)
+    idx = ob.all_triangles(kf, kf + 4 * kf + kf / 2, kf, return_indices=True)
+    lo, hi = est.shell_limits(e[:, 0], e[:, 1], kf)
+    b_fft = est.bispectrum_unnorm([field], box, lo, hi, idx)
+    dkf = est.full_dk(field)
+    _, _, bsum = direct.enumerate_triangles(N, lo, hi, idx, dkf)
+    b_direct = bsum.real * box ** 6
+    assert np.abs(bsum.imag).max() < 1e-10 * np.abs(bsum.real).max()
+    assert np.allclose(b_fft, b_direct, rtol=1e-9, atol=1e-12 * np.abs(b_direct).max())
+
+
+def test_bispectrum_closed_form_cosines():
+    """A field of three cosine modes closing one triangle has a hand-computable B."""
+    N, box = 16, 100.
+    kf = est.kf_of(box)
+    x = np.arange(N) * box / N
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    # modes k1=(2,0,0), k2=(0,3,0), k3=(-2,-3,0): k1+k2+k3 = 0
+    a1, a2, a3 = 0.7, -1.1, 0.5
+    f = (a1 * np.cos(kf * 2 * X) + a2 * np.cos(kf * 3 * Y) + a3 * np.cos(kf * (-2 * X - 3 * Y)))
+    # delta(k) = a/2 at +-k; shells: |k1|^2 = 4, |k2|^2 = 9, |k3|^2 = 13
+    lo, hi = np.array([4, 9, 13]), np.array([4, 9, 13])
+    tri = np.array([[0, 1, 2]])
+    b = est.bispectrum_unnorm([f], box, lo, hi, tri)
+    # sum over closed triples: (k1,k2,k3) and (-k1,-k2,-k3): 2 * (a1/2)(a2/2)(a3/2)
+    expect = 2 * (a1 / 2) * (a2 / 2) * (a3 / 2) * box ** 6
+    assert np.isclose(b[0], expect, rtol=1e-10)
+    # invariants: B(a f) = a^3 B
+    assert np.isclose(est.bispectrum_unnorm([2.5 * f], box, lo, hi, tri)[0], 2.5 ** 3 * b[0],
+                      rtol=1e-10)
+
+
+def test_shell_limits_ties():
+    kf = 1.0
+    lo, hi = est.shell_limits([3.0, 5.0], [5.0, 7.0], kf)
+    # |n| = 5 lies on the shared edge and belongs to both bins
+    assert hi[0] == 25 and lo[1] == 25
+    lo2, hi2 = est.shell_limits([3.0 * (1 + 1e-13)], [5.0 * (1 - 1e-13)], kf)
+    assert lo2[0] == 9 and hi2[0] == 25
