@@ -389,66 +389,125 @@ int bsk_triples_tiled(int dtype, const void* fields, long long shell_stride, int
   return bsk_reduce_tiled_launch(part, ntri, nsum, nchunk, scale, out_dev, s);
 }
 
-int bsk_mfma_tiles_per_wg(int nshell, int mode, int* tiles) {
-  const int nblk = (nshell + 15) / 16;
-  if (nblk < 1 || nblk > 3 || mode < 0 || mode > 2) {
-    bsk_set_error("mfma: 1..48 shells and mode 0..2 supported (got %d shells, mode %d)", nshell,
-                  mode);
-    return BSK_EUNSUPPORTED;
-  }
-  *tiles = bsk_mfma_tiles(nblk, mode);
-  return BSK_OK;
+namespace {
+
+// Tile index of (i, jb, kb) in a workgroup's block tiles (see mfma.hip).
+int block_tile(int nblk, int mode, int i, int jb, int kb) {
+  const int ib = i / 16, ii = i % 16;
+  int before = 0;
+  for (int b = 0; b < ib; ++b)
+    before += mode == 0 ? (b + 1) * (b + 2) / 2 : (mode == 1 ? (b + 1) * nblk : nblk * nblk);
+  const int nti = mode == 0 ? (ib + 1) * (ib + 2) / 2 : (mode == 1 ? (ib + 1) * nblk : nblk * nblk);
+  const int tin = mode == 0 ? jb * (jb + 1) / 2 + kb : jb * nblk + kb;
+  return 16 * before + ii * nti + tin;
 }
 
-int bsk_mfma_slots(int nshell, int mode, int ntri, const int* tri, int* slot) {
-  const int nblk = (nshell + 15) / 16;
-  if (nblk < 1 || nblk > 3 || mode < 0 || mode > 2 || ntri < 0 || (ntri && (!tri || !slot))) {
-    bsk_set_error("bsk_mfma_slots: invalid argument");
+// Canonical side order of a triangle for the mode's symmetry.
+void canonical(int mode, int& i, int& j, int& k) {
+  if (mode == 0) {
+    int v[3] = {i, j, k};
+    std::sort(v, v + 3);
+    i = v[2]; j = v[1]; k = v[0];
+  } else if (mode == 1 && j > i) {
+    std::swap(i, j);
+  }
+}
+
+}  // namespace
+
+int bsk_mfma_layout(int nshell, int mode, int ntri, const int* tri, int* slot, int* layout,
+                    int* tiles_per_wg) {
+  if (nshell < 1 || mode < 0 || mode > 2 || ntri < 0 || (ntri && (!tri || !slot)) || !layout ||
+      !tiles_per_wg) {
+    bsk_set_error("bsk_mfma_layout: invalid argument");
     return BSK_EINVAL;
   }
+  for (int t = 0; t < ntri; ++t)
+    for (int e = 0; e < 3; ++e)
+      if (tri[3 * t + e] < 0 || tri[3 * t + e] >= nshell) {
+        bsk_set_error("bsk_mfma_layout: triangle %d has a shell index out of range", t);
+        return BSK_EINVAL;
+      }
+  for (int e = 0; e < BSK_MFMA_LAYOUT_INTS; ++e) layout[e] = 0;
+  // auto spectra whose last few shells only appear as the largest side:
+  // full blocks for the rest plus a handful of extra tiles
+  int nblk = (nshell + 15) / 16;
+  int nextra = 0;
+  int xt[kMfmaMaxExtra][3];
+  if (mode == 0 && nshell > 16 && nshell % 16 != 0 && nshell / 16 <= 2) {
+    const int nb = nshell / 16, lim = 16 * nb;
+    bool ok = true;
+    for (int t = 0; t < ntri && ok; ++t) {
+      int i = tri[3 * t], j = tri[3 * t + 1], k = tri[3 * t + 2];
+      canonical(mode, i, j, k);
+      if (i < lim) continue;
+      if (j >= lim) { ok = false; break; }
+      int f = -1;
+      for (int x = 0; x < nextra; ++x)
+        if (xt[x][0] == i && xt[x][1] == j / 16 && xt[x][2] == k / 16) f = x;
+      if (f < 0) {
+        if (nextra == kMfmaMaxExtra) { ok = false; break; }
+        xt[nextra][0] = i; xt[nextra][1] = j / 16; xt[nextra][2] = k / 16;
+        ++nextra;
+      }
+    }
+    if (ok) {
+      nblk = nb;
+    } else {
+      nextra = 0;
+    }
+  }
+  if (!nextra && !bsk_mfma_supported(nblk, mode)) {
+    bsk_set_error("mfma: %d shells in mode %d exceed the kernel (48 shells, 32 for <ABC>)",
+                  nshell, mode);
+    return BSK_EUNSUPPORTED;
+  }
+  const int tiles = bsk_mfma_tiles(nblk, mode);
   for (int t = 0; t < ntri; ++t) {
     int i = tri[3 * t], j = tri[3 * t + 1], k = tri[3 * t + 2];
-    if (i < 0 || j < 0 || k < 0 || i >= nshell || j >= nshell || k >= nshell) {
-      bsk_set_error("bsk_mfma_slots: triangle %d has a shell index out of range", t);
-      return BSK_EINVAL;
+    canonical(mode, i, j, k);
+    int g;
+    if (i >= 16 * nblk) {
+      g = -1;
+      for (int x = 0; x < nextra; ++x)
+        if (xt[x][0] == i && xt[x][1] == j / 16 && xt[x][2] == k / 16) g = tiles + x;
+    } else {
+      g = block_tile(nblk, mode, i, j / 16, k / 16);
     }
-    if (mode == 0) {  // symmetric: order i >= j >= k
-      int v[3] = {i, j, k};
-      std::sort(v, v + 3);
-      i = v[2]; j = v[1]; k = v[0];
-    } else if (mode == 1) {  // symmetric in the first two sides
-      if (j > i) std::swap(i, j);
-    }
-    const int ib = i / 16, ii = i % 16, jb = j / 16, kb = k / 16;
-    int before = 0, nti = 0, tin = 0;
-    for (int b = 0; b < ib; ++b)
-      before += mode == 0 ? (b + 1) * (b + 2) / 2 : (mode == 1 ? (b + 1) * nblk : nblk * nblk);
-    nti = mode == 0 ? (ib + 1) * (ib + 2) / 2 : (mode == 1 ? (ib + 1) * nblk : nblk * nblk);
-    tin = mode == 0 ? jb * (jb + 1) / 2 + kb : jb * nblk + kb;
-    const int gtile = 16 * before + ii * nti + tin;
-    slot[t] = gtile * 256 + (j % 16) * 16 + (k % 16);
+    slot[t] = g * 256 + (j % 16) * 16 + (k % 16);
   }
+  layout[0] = nblk;
+  layout[1] = nextra;
+  for (int x = 0; x < nextra; ++x)
+    for (int e = 0; e < 3; ++e) layout[2 + 3 * x + e] = xt[x][e];
+  *tiles_per_wg = tiles + nextra;
   return BSK_OK;
 }
 
-int bsk_triples_mfma_workspace(int nshell, int mode, int nwg, size_t* bytes) {
-  int tiles = 0;
-  if (int rc = bsk_mfma_tiles_per_wg(nshell, mode, &tiles)) return rc;
-  *bytes = sizeof(float) * 256 * (size_t)tiles * std::max(nwg, 1);
+int bsk_triples_mfma_workspace(int tiles_per_wg, int nwg, size_t* bytes) {
+  if (tiles_per_wg <= 0 || nwg <= 0 || !bytes) {
+    bsk_set_error("bsk_triples_mfma_workspace: invalid argument");
+    return BSK_EINVAL;
+  }
+  *bytes = sizeof(float) * 256 * (size_t)tiles_per_wg * nwg;
   return BSK_OK;
 }
 
 int bsk_triples_mfma(int mode, int nshell, const uint16_t* hi_i, const uint16_t* lo_i,
                      const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
                      const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
-                     int ntri, const int* slot_dev, double scale, double* out_dev, void* work,
-                     void* stream) {
-  int tiles = 0;
-  if (int rc = bsk_mfma_tiles_per_wg(nshell, mode, &tiles)) return rc;
-  if (!hi_j || !lo_j || !slot_dev || !out_dev || !work || nwg <= 0 ||
+                     const int* layout, int ntri, const int* slot_dev, double scale,
+                     double* out_dev, void* work, void* stream) {
+  if (!layout || !hi_j || !lo_j || !slot_dev || !out_dev || !work || nwg <= 0 ||
       (mode >= 1 && (!hi_k || !lo_k)) || (mode == 2 && (!hi_i || !lo_i)) ||
-      shell_stride < ncell) {
+      shell_stride < ncell || mode < 0 || mode > 2) {
     bsk_set_error("bsk_triples_mfma: invalid argument");
+    return BSK_EINVAL;
+  }
+  const int nblk = layout[0], nextra = layout[1];
+  if (nblk < 1 || nextra < 0 || nextra > kMfmaMaxExtra || (nextra && mode != 0) ||
+      16 * nblk + (nextra ? 16 : 0) < nshell || (!nextra && 16 * nblk < nshell)) {
+    bsk_set_error("bsk_triples_mfma: layout does not match %d shells", nshell);
     return BSK_EINVAL;
   }
   MfmaArgs a;
@@ -464,13 +523,23 @@ int bsk_triples_mfma(int mode, int nshell, const uint16_t* hi_i, const uint16_t*
   }
   a.ncell = ncell;
   a.mode = mode;
-  a.nblk = (nshell + 15) / 16;
+  a.nblk = nblk;
   a.nwg = nwg;
   a.partial = reinterpret_cast<float*>(work);
+  a.nextra = nextra;
+  for (int x = 0; x < 16; ++x)
+    for (int e = 0; e < 3; ++e) a.xt[x][e] = x < nextra ? layout[2 + 3 * x + e] : 0;
+  for (int x = 0; x < nextra; ++x)
+    if (a.xt[x][0] < 16 * nblk || a.xt[x][0] >= nshell || a.xt[x][1] < 0 ||
+        a.xt[x][1] >= nblk || a.xt[x][2] < 0 || a.xt[x][2] >= nblk) {
+      bsk_set_error("bsk_triples_mfma: extra tile %d out of range", x);
+      return BSK_EINVAL;
+    }
   hipStream_t s = as_stream(stream);
   int rc = bsk_triple_mfma_launch(a, s);
   if (rc) return rc;
-  return bsk_reduce_mfma_launch(a.partial, nwg, tiles, ntri, slot_dev, scale, out_dev, s);
+  return bsk_reduce_mfma_launch(a.partial, nwg, bsk_mfma_tiles(nblk, mode) + nextra, ntri,
+                                slot_dev, scale, out_dev, s);
 }
 
 }  // extern "C"

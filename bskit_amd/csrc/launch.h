@@ -53,9 +53,13 @@ struct MfmaArgs {
   int mode;                // 0 auto (i>=j>=k), 1 <AAB> (j<=i, all k), 2 <ABC> (all)
   int nblk;                // 16-shell blocks (1..3)
   int nwg;                 // workgroups (cell ranges)
-  float* partial;          // [nwg][tiles][256]
+  float* partial;          // [nwg][tiles + nextra][256]
+  int nextra;              // extra tiles (auto mode): rows i >= 16*nblk
+  int xt[16][3];           // extra tile (i, jb, kb)
 };
 int bsk_mfma_tiles(int nblk, int mode);  // tile slots per workgroup
+int bsk_mfma_supported(int nblk, int mode);
+constexpr int kMfmaMaxExtra = 8;  // extra tiles per launch (one slot per wave)
 int bsk_triple_mfma_launch(const MfmaArgs& a, hipStream_t s);
 int bsk_reduce_mfma_launch(const float* partial, int nwg, int tiles_per_wg, int ntri,
                            const int* slot_dev, double scale, double* out, hipStream_t s);

@@ -210,22 +210,40 @@ class GridEngine:
             self._tri_cache[key] = t
         return t
 
-    def mfma_slots(self, tri, nshell, mode):
+    def mfma_layout(self, tri, nshell, mode):
+        """(layout int array, device slots, tiles per workgroup) of an MFMA launch."""
         tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
-        key = (tri.tobytes(), 'slot', nshell, mode)
-        t = self._tri_cache.get(key)
-        if t is None:
+        key = (tri.tobytes(), 'layout', nshell, mode)
+        r = self._tri_cache.get(key)
+        if r is None:
             slots = np.zeros(len(tri), dtype=np.int32)
-            L.call('bsk_mfma_slots', nshell, mode, len(tri),
+            layout = (ctypes.c_int * L.MFMA_LAYOUT_INTS)()
+            tiles = ctypes.c_int()
+            L.call('bsk_mfma_layout', nshell, mode, len(tri),
                    tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                   slots.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
-            t = torch.from_numpy(slots).to(self.device)
-            self._tri_cache[key] = t
-        return t
+                   slots.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), layout,
+                   ctypes.byref(tiles))
+            r = (layout, torch.from_numpy(slots).to(self.device), tiles.value)
+            self._tri_cache[key] = r
+        return r
 
     def mfma_nwg(self, ncell=None):
-        niter = (self.ncell if ncell is None else ncell) // 64
-        return int(max(1, min(2048, niter // 256)))
+        """Workgroups of the MFMA kernel: ~4 per CU at large sizes, >= 8 chunks each."""
+        nchunk = (self.ncell if ncell is None else ncell) // 128
+        return int(max(1, min(1024, nchunk // 8)))
+
+    @staticmethod
+    def mfma_supported(nshell, nfield, tri):
+        """Whether the MFMA kernel covers this shell count / field roles / triangles."""
+        tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
+        slots = np.zeros(len(tri), dtype=np.int32)
+        layout = (ctypes.c_int * L.MFMA_LAYOUT_INTS)()
+        tiles = ctypes.c_int()
+        rc = L.lib().bsk_mfma_layout(int(nshell), _MODE[nfield], len(tri),
+                                     tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                     slots.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), layout,
+                                     ctypes.byref(tiles))
+        return rc == L.OK
 
     def plain_nchunk(self, ncell=None):
         ncell = self.ncell if ncell is None else ncell
@@ -262,16 +280,16 @@ class GridEngine:
                     raise ValueError('the MFMA path needs BF16X2 shells')
             mode = _MODE[nf]
             nwg = self.mfma_nwg(ncell) if nwg is None else int(nwg)
-            slots = self.mfma_slots(tri, sa.nb, mode)
+            layout, slots, tiles = self.mfma_layout(tri, sa.nb, mode)
             nbytes = L.size_out()
-            L.call('bsk_triples_mfma_workspace', sa.nb, mode, nwg, ctypes.byref(nbytes))
+            L.call('bsk_triples_mfma_workspace', tiles, nwg, ctypes.byref(nbytes))
             w = self.work.get('mfma', nbytes.value)
             hi_i, lo_i = sa.data, sa.lo
             hi_j, lo_j = sb.data, sb.lo
             hi_k, lo_k = sc.data, sc.lo
             L.call('bsk_triples_mfma', mode, sa.nb, L.ptr(hi_i), L.ptr(lo_i), L.ptr(hi_j),
-                   L.ptr(lo_j), L.ptr(hi_k), L.ptr(lo_k), ncell, ncell, nwg, T, L.ptr(slots),
-                   float(scale), L.ptr(out), L.ptr(w), s)
+                   L.ptr(lo_j), L.ptr(hi_k), L.ptr(lo_k), ncell, ncell, nwg, layout, T,
+                   L.ptr(slots), float(scale), L.ptr(out), L.ptr(w), s)
             return out
         if path == PATH_PLAIN:
             dt = {L.FMT_F32: L.DT_F32, L.FMT_F64: L.DT_F64}.get(sa.fmt)

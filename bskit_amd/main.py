@@ -275,12 +275,13 @@ class FFTBispectrum:
         win = max([m.compensation_window for m in self._meshes()] + [0])
         return get_engine(self._N, self._L, kLU, self._device, win)
 
-    def _shellsets(self, eng, kLU):
+    def _shellsets(self, eng, kLU, tri):
         if not self.attrs.get('painted') or self._fields is None:
             raise RuntimeError('meshes are not painted (for_grid_info_only=True)')
         fmt = {'bf16x3': L.FMT_BF16X2, 'fp32': L.FMT_F32, 'fp64': L.FMT_F64}[self.precision]
-        if fmt == L.FMT_BF16X2 and len(kLU) > _MFMA_MAX_SHELLS:
-            warnings.warn('more than %d bins: using the FP32 path' % _MFMA_MAX_SHELLS)
+        if fmt == L.FMT_BF16X2 and not GridEngine.mfma_supported(len(kLU), len(self._fields), tri):
+            warnings.warn('%d bins with %d fields exceed the MFMA kernel (48 shells, 32 for '
+                          '<ABC>): using the FP32 path' % (len(kLU), len(self._fields)))
             fmt = L.FMT_F32
         key = _bins_key(kLU)
         sets = []
@@ -301,7 +302,7 @@ class FFTBispectrum:
 
     def _unnorm_sums(self, kLU, tri, scale):
         eng = self._engine(kLU)
-        sets, path = self._shellsets(eng, kLU)
+        sets, path = self._shellsets(eng, kLU, tri)
         out = eng.triples(sets, tri, path=path, scale=scale)
         return out.cpu().numpy()
 

@@ -143,22 +143,27 @@ int bsk_triples_tiled(int dtype, const void* fields, long long shell_stride, int
 
 /* Split-BF16 MFMA triple sums: for each i, the GEMM (D o d_i) . D^T over
  * the cells with the pair products split into hi+lo bf16 (hi*hi + hi*lo +
- * lo*hi on v_mfma_f32_16x16x32_bf16, FP32 accumulation, FP64 final
- * reduction).  Shells are BF16X2 planes (1..48 shells per set).  mode
- * BSK_MODE_AUTO uses (hi_j,lo_j) for all three sides; BSK_MODE_AAB adds
- * the k side; BSK_MODE_ABC uses all three sets.  slot_dev comes from
- * bsk_mfma_slots for the same (nshell, mode, triangles).  Replaces the same
- * loop as bsk_triples_plain (bskit/main.py:1871-1882) and, on unit shells,
- * the N_tri sum (:2027-2031).
- * work: bsk_triples_mfma_workspace(nshell, mode, nwg) bytes. */
-int bsk_mfma_tiles_per_wg(int nshell, int mode, int* tiles);
-int bsk_mfma_slots(int nshell, int mode, int ntri, const int* tri_host, int* slot_host);
-int bsk_triples_mfma_workspace(int nshell, int mode, int nwg, size_t* bytes);
+ * lo*hi on v_mfma_f32_16x16x32_bf16, FP32 accumulation over short chains,
+ * FP64 second level, FP64 final reduction).  Shells are BF16X2 planes.
+ * mode BSK_MODE_AUTO uses (hi_j,lo_j) for all three sides; BSK_MODE_AAB adds
+ * the k side; BSK_MODE_ABC uses all three sets.
+ * bsk_mfma_layout (host) plans a launch for a triangle list: layout[0] =
+ * 16-shell blocks, layout[1] = extra tiles (auto spectra whose last shells
+ * only appear as the largest side, up to 8 tiles), layout[2+3t..] their
+ * (i, jb, kb); it returns per-triangle partial slots and the tiles per
+ * workgroup.  Supports up to 48 shells (32 for <ABC>, 32 + extra shells for
+ * auto).  Replaces the per-triangle loop of measure_bispectrum_faster
+ * (bskit/main.py:1871-1882) and, on unit shells, the N_tri sum (:2027-2031).
+ * work: bsk_triples_mfma_workspace(tiles_per_wg, nwg) bytes. */
+#define BSK_MFMA_LAYOUT_INTS 26
+int bsk_mfma_layout(int nshell, int mode, int ntri, const int* tri_host, int* slot_host,
+                    int* layout, int* tiles_per_wg);
+int bsk_triples_mfma_workspace(int tiles_per_wg, int nwg, size_t* bytes);
 int bsk_triples_mfma(int mode, int nshell, const uint16_t* hi_i, const uint16_t* lo_i,
                      const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
                      const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
-                     int ntri, const int* slot_dev, double scale, double* out_dev, void* work,
-                     void* stream);
+                     const int* layout, int ntri, const int* slot_dev, double scale,
+                     double* out_dev, void* work, void* stream);
 
 /* Cloud-in-cell assignment of npart particles (pos: npart x 3 float, box
  * units, C order) to the periodic mesh grid (nmesh^3 float, overwritten).

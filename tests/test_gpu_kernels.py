@@ -228,3 +228,34 @@ def test_mfma_unit_shell_counts_exact(N, nb, width):
     assert cnt.max() < 2 ** 17
     assert np.abs(s - cnt).max() < 0.25
     assert np.array_equal(np.rint(s).astype(np.int64), cnt)
+
+
+@pytest.mark.parametrize('N,nb', [(32, 33), (64, 17)])
+def test_mfma_extra_tiles(N, nb):
+    """Shells past the last full 16-block that only appear as the largest side
+    ride on extra tiles (the bench's [k_f, k_f] squeezed shell)."""
+    from bskit_amd import _lib as L
+    from bskit_amd import bins
+    box = 1000.0
+    kf = 2 * np.pi / box
+    width = 0.5 if nb > 32 else 1.0
+    lo = kf * (1 + width * np.arange(nb - 1))
+    hi = lo + width * kf
+    lo = np.concatenate([lo, [kf]])
+    hi = np.concatenate([hi, [kf]])
+    eng = _engine(N, box, lo, hi)
+    base = bins.closed_triangle_indices(np.stack([lo[:-1], hi[:-1]], 1), 1)
+    extra = np.array([[nb - 1, j, j] for j in range(1, nb - 1)] + [[nb - 1, 5, 2]])
+    tri = np.concatenate([base, extra])
+    layout, _, tiles = eng.mfma_layout(tri, nb, 0)
+    assert layout[1] > 0, 'expected the extra-tile layout'
+    f64 = _field(N, 21, np.float64)
+    sh = est.shells(est.forward(f64), N, eng.n2lo, eng.n2hi, 'delta')
+    ref = est.triple_sums(sh, sh, sh, tri)
+    absum = np.array([np.sum(np.abs(sh[i] * sh[j] * sh[k])) for i, j, k in tri])
+    sbf = eng.shells(eng.forward(torch.from_numpy(f64.astype(np.float32)).cuda()),
+                     fmt=L.FMT_BF16X2)
+    got = eng.triples([sbf], tri, path='mfma').cpu().numpy()
+    ok, err = _bcheck(got, ref, absum, 5e-5)
+    print('extra tiles N=%d nb=%d layout=(%d,%d) err %.2e' % (N, nb, layout[0], layout[1], err))
+    assert ok

@@ -164,18 +164,33 @@ def test_library_loads_and_host_calls_work():
            L.dbl_array([3 * kf, 5 * kf]), lo, hi)
     assert list(lo) == [1, 9] and list(hi) == [9, 25]
     tiles = ctypes.c_int()
-    L.call('bsk_mfma_tiles_per_wg', 33, 0, ctypes.byref(tiles))
-    assert tiles.value == 16 * (1 + 3 + 6)
+    layout = (ctypes.c_int * L.MFMA_LAYOUT_INTS)()
     tri = np.array([[2, 1, 0], [0, 1, 2], [1, 1, 1]], dtype=np.int32)
     slots = np.zeros(3, dtype=np.int32)
-    L.call('bsk_mfma_slots', 3, 0, 3, tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-           slots.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.call('bsk_mfma_layout', 3, 0, 3, tri.ctypes.data_as(ip), slots.ctypes.data_as(ip), layout,
+           ctypes.byref(tiles))
     assert slots[0] == slots[1]  # symmetric spectra: any ordering maps to one slot
+    assert layout[0] == 1 and layout[1] == 0 and tiles.value == 16
+    # 33 shells where shell 32 only appears as the largest side: 2 blocks + extra tiles
+    tri = np.array([[32, j, j] for j in range(1, 32)] + [[20, 10, 3]], dtype=np.int32)
+    slots = np.zeros(len(tri), dtype=np.int32)
+    L.call('bsk_mfma_layout', 33, 0, len(tri), tri.ctypes.data_as(ip), slots.ctypes.data_as(ip),
+           layout, ctypes.byref(tiles))
+    assert layout[0] == 2 and layout[1] == 2
+    assert tiles.value == 16 * (1 + 3) + 2
+    assert sorted([tuple(layout[2 + 3 * t:5 + 3 * t]) for t in range(2)]) == [(32, 0, 0), (32, 1, 1)]
+    # a triangle with shell 32 as a middle side forces three full blocks
+    tri2 = np.concatenate([tri, [[32, 32, 1]]]).astype(np.int32)
+    slots2 = np.zeros(len(tri2), dtype=np.int32)
+    L.call('bsk_mfma_layout', 33, 0, len(tri2), tri2.ctypes.data_as(ip),
+           slots2.ctypes.data_as(ip), layout, ctypes.byref(tiles))
+    assert layout[0] == 3 and layout[1] == 0 and tiles.value == 16 * 10
     n = ctypes.c_size_t()
-    L.call('bsk_triples_mfma_workspace', 33, 0, 10, ctypes.byref(n))
+    L.call('bsk_triples_mfma_workspace', 160, 10, ctypes.byref(n))
     assert n.value == 10 * 160 * 256 * 4
     with pytest.raises(L.BskError):
-        L.call('bsk_mfma_tiles_per_wg', 60, 0, ctypes.byref(tiles))
+        L.call('bsk_mfma_layout', 60, 0, 0, None, None, layout, ctypes.byref(tiles))
 
 
 def test_shell_threshold_abi_matches_oracle():
