@@ -98,8 +98,8 @@ class StageTimer:
 
 def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
     from bskit_amd import _lib as L
-    N, L_ = wl['N'], wl['L']
-    scale = L_ ** 6 / float(N) ** 3
+    # L^6/N^3 sum over the N^3 mesh == L^6/S^3 sum over the S^3 shell mesh
+    scale = wl['L'] ** 6 / float(eng.ncell)
     shells = eng.alloc_shells(fmt, eng.nb)
     out = torch.empty(len(wl['tri']), dtype=torch.float64, device='cuda')
     dtype = torch.float64 if fmt == L.FMT_F64 else torch.float32
@@ -169,7 +169,7 @@ def main():
     wl = build_workload(args, torch, np)
     T = len(wl['tri'])
     eng = GridEngine(wl['N'], wl['L'], wl['k_low'], wl['k_high'], cic_window=wl['N'])
-    log('[bench] T=%d shells=%d compact m=%d' % (T, eng.nb, eng.m))
+    log('[bench] T=%d shells=%d compact m=%d shell mesh S=%d' % (T, eng.nb, eng.m, eng.S))
 
     t0 = time.perf_counter()
     ntri, kbar = eng.grid_info(wl['tri'])
@@ -220,12 +220,12 @@ def main():
 
     # roofline of the dominant kernel (the MFMA triple sum over all shells)
     N = wl['N']
-    ncell = float(N) ** 3
+    ncell = float(eng.ncell)                     # shell-mesh cells S^3
     nsh = eng.nb
     shell_bytes = nsh * 4 * ncell               # split-BF16 hi+lo, read once
     flops = 2.0 * T * ncell
     t_tri = stage_ms['triples_mfma'] * 1e-3
-    H = N * N * (N // 2 + 1)
+    H = eng.MX * eng.MY * eng.MZ                 # compact delta(k) cube
     t_sh = stage_ms['shells'] * 1e-3
     shells_alg = 8.0 * H + nsh * 4 * ncell       # delta(k) read + shells written
     dominant = 'triples_mfma' if t_tri >= t_sh else 'shells'
@@ -243,7 +243,7 @@ def main():
                 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                 'frac': achieved / HBM_PEAK_GBS, 'traffic': None}
     stage_gbs = {'shells_GBps': shells_alg / t_sh / 1e9,
-                 'forward_GBps': (4.0 * ncell + 8.0 * H) / (stage_ms['forward'] * 1e-3) / 1e9}
+                 'forward_GBps': (4.0 * N ** 3 + 8.0 * H) / (stage_ms['forward'] * 1e-3) / 1e9}
 
     cpu = None
     if not args.no_cpu:
@@ -278,7 +278,7 @@ def main():
         'config': {'workload': ('%d^3 CIC mesh of %d^3 Zel\'dovich particles, L=%g Mpc/h, %d bins '
                                 'of 2k_f from k_f; all+equilateral+isosceles(m=2)+squeezed(|k1|=k_f)'
                                 % (N, args.npart_side, wl['L'], args.nbin)),
-                   'nmesh': N, 'nbin': args.nbin, 'nshell': nsh, 'ntri': T,
+                   'nmesh': N, 'shell_mesh': eng.S, 'nbin': args.nbin, 'nshell': nsh, 'ntri': T,
                    'path': 'split-BF16 MFMA (hi*hi+hi*lo+lo*hi), FP32 accumulate, FP64 reduce',
                    'parallelism': 'single GPU'},
         'fp32_plain': res_plain,

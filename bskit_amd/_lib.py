@@ -21,6 +21,7 @@ KIND_DELTA, KIND_UNIT, KIND_KMAG = 0, 1, 2
 MODE_AUTO, MODE_AAB, MODE_ABC = 0, 1, 2
 STAGE_X, STAGE_YZ = 1, 2
 MFMA_LAYOUT_INTS = 26
+ABI_VERSION = 2
 
 _c_int, _c_ll, _c_double, _vp = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 _c_size_p = ctypes.POINTER(ctypes.c_size_t)
@@ -36,6 +37,8 @@ _SIGS = {
     'bsk_plan_destroy': [_vp],
     'bsk_plan_geometry': [_vp, _int_p, _int_p, _int_p, _int_p, _int_p],
     'bsk_plan_thresholds': [_vp, _int_p, _int_p],
+    'bsk_plan_shell_mesh': [_vp, _int_p],
+    'bsk_plan_set_shell_mesh': [_vp, ctypes.c_int],
     'bsk_plan_set_variant': [_vp, _c_int, _c_int],
     'bsk_plan_set_compensation': [_vp, _c_int],
     'bsk_forward_workspace': [_vp, _c_int, _c_size_p],
@@ -95,6 +98,9 @@ def lib():
                 fn = getattr(h, name)
                 fn.argtypes = args
                 fn.restype = _RESTYPE.get(name, _c_int)
+            if h.bsk_version() != ABI_VERSION:
+                raise ImportError('%s has ABI version %d, this package needs %d; rebuild it'
+                                  % (LIB_PATH, h.bsk_version(), ABI_VERSION))
             _lib = h
     return _lib
 

@@ -23,7 +23,8 @@ void bsk_set_error(const char* fmt, ...) {
 }
 
 struct bsk_plan {
-  int N;
+  int N;  // input mesh
+  int S;  // shell mesh: the inverse transforms and triple sums run on S^3 cells
   double box;
   int nbin;
   std::vector<int> n2lo, n2hi;
@@ -34,6 +35,8 @@ struct bsk_plan {
   int* d_hi = nullptr;
   cplx<float>* tw_f = nullptr;
   cplx<double>* tw_d = nullptr;
+  cplx<float>* tws_f = nullptr;  // exp(-2 pi i t / S) when S != N
+  cplx<double>* tws_d = nullptr;
   float* comp_f = nullptr;
   double* comp_d = nullptr;
 };
@@ -43,6 +46,43 @@ namespace {
 const double kTieRtol = 1e-9;
 
 hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// The shell fields are band limited: every retained mode has integer
+// components |n_a| <= m.  A product of three of them only holds frequencies
+// with components in [-3m, 3m], so on any periodic mesh of S > 3m points per
+// axis the sum over cells of d_i d_j d_k picks up exactly the zero frequency:
+//   sum_{x in S^3} d_i d_j d_k = S^3 sum_{q1+q2+q3=0} F_i(q1) F_j(q2) F_k(q3),
+// the same sum the N^3 mesh gives up to the factor (N/S)^3.  The shells are
+// therefore evaluated on the smallest supported S > 3m when that is below N.
+int auto_shell_mesh(int N, int m) {
+  if (2 * m + 1 >= N) return N;
+  const int s = bsk_fft_size_above(3 * m);
+  return (s > 0 && s < N) ? s : N;
+}
+
+template <typename T>
+int make_twiddles(int n, cplx<T>** out) {
+  std::vector<cplx<T>> h(n);
+  for (int t = 0; t < n; ++t) {
+    const double a = -2.0 * M_PI * (double)t / (double)n;
+    h[t] = {(T)cos(a), (T)sin(a)};
+  }
+  BSK_HIP_CHECK(hipMalloc(out, sizeof(cplx<T>) * n));
+  BSK_HIP_CHECK(hipMemcpy(*out, h.data(), sizeof(cplx<T>) * n, hipMemcpyHostToDevice));
+  return BSK_OK;
+}
+
+int set_shell_mesh(bsk_plan* p, int S) {
+  (void)hipFree(p->tws_f);
+  (void)hipFree(p->tws_d);
+  p->tws_f = nullptr;
+  p->tws_d = nullptr;
+  p->S = S;
+  if (S == p->N) return BSK_OK;
+  int rc = make_twiddles<float>(S, &p->tws_f);
+  if (!rc) rc = make_twiddles<double>(S, &p->tws_d);
+  return rc;
+}
 
 template <typename T>
 int forward_t(bsk_plan* p, const T* field, cplx<T>* dk, void* work, hipStream_t s,
@@ -63,7 +103,7 @@ template <typename T>
 int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int x0, int xs,
              int stages, int fmt, void* out, void* out_lo, long long stride, long long cell0,
              void* work, hipStream_t s, const cplx<T>* tw) {
-  const int N = p->N;
+  const int N = p->S;
   const Geom g = p->g;
   cplx<T>* d1 = reinterpret_cast<cplx<T>*>(work);
   cplx<T>* d2 = d1 + (size_t)nsh * g.MY * N * g.MZ;
@@ -84,6 +124,9 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
                       nsh, s);
 }
 
+const cplx<float>* shell_tw(const bsk_plan* p, float) { return p->tws_f ? p->tws_f : p->tw_f; }
+const cplx<double>* shell_tw(const bsk_plan* p, double) { return p->tws_d ? p->tws_d : p->tw_d; }
+
 int check_plan(const bsk_plan* p) {
   if (!p) {
     bsk_set_error("null plan");
@@ -93,6 +136,28 @@ int check_plan(const bsk_plan* p) {
 }
 
 }  // namespace
+
+bool bsk_fft_size_supported(int N) {
+  switch (N) {
+#define X(n) case n:
+    BSK_FFT_SIZES(X)
+#undef X
+    return true;
+    default:
+      return false;
+  }
+}
+
+int bsk_fft_size_above(int n) {
+  static const int sizes[] = {
+#define X(v) v,
+      BSK_FFT_SIZES(X)
+#undef X
+  };
+  for (int v : sizes)
+    if (v > n) return v;
+  return 0;
+}
 
 extern "C" {
 
@@ -129,8 +194,8 @@ int bsk_plan_create(bsk_plan** out, int nmesh, double box_size, int nbin, const 
   }
   *out = nullptr;
   if (!bsk_fft_size_supported(nmesh)) {
-    bsk_set_error("bsk_plan_create: Nmesh=%d is not a supported FFT size "
-                  "(16,32,48,64,96,128,192,256,384,512,768)", nmesh);
+    bsk_set_error("bsk_plan_create: Nmesh=%d is not a supported FFT size (even, 5-smooth, "
+                  "16..768, half length with a radix plan)", nmesh);
     return BSK_EUNSUPPORTED;
   }
   bsk_plan* p = new bsk_plan();
@@ -162,28 +227,25 @@ int bsk_plan_create(bsk_plan** out, int nmesh, double box_size, int nbin, const 
   }
   g.MZ = std::min(m + 1, N / 2 + 1);
   p->g = g;
+  p->S = N;
   for (int i = 0; i < 6; ++i) p->variant[i] = 0;
 
-  std::vector<cplx<float>> twf(N);
-  std::vector<cplx<double>> twd(N);
-  for (int t = 0; t < N; ++t) {
-    const double a = -2.0 * M_PI * (double)t / (double)N;
-    twd[t] = {cos(a), sin(a)};
-    twf[t] = {(float)twd[t].x, (float)twd[t].y};
-  }
   hipError_t e = hipSuccess;
   e = hipMalloc(&p->d_lo, sizeof(int) * nbin);
   if (e == hipSuccess) e = hipMalloc(&p->d_hi, sizeof(int) * nbin);
-  if (e == hipSuccess) e = hipMalloc(&p->tw_f, sizeof(cplx<float>) * N);
-  if (e == hipSuccess) e = hipMalloc(&p->tw_d, sizeof(cplx<double>) * N);
   if (e == hipSuccess) e = hipMemcpy(p->d_lo, p->n2lo.data(), sizeof(int) * nbin, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_hi, p->n2hi.data(), sizeof(int) * nbin, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(p->tw_f, twf.data(), sizeof(cplx<float>) * N, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemcpy(p->tw_d, twd.data(), sizeof(cplx<double>) * N, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     bsk_set_error("bsk_plan_create: %s", hipGetErrorString(e));
     bsk_plan_destroy(p);
     return e == hipErrorOutOfMemory ? BSK_ENOMEM : BSK_EHIP;
+  }
+  rc = make_twiddles<float>(N, &p->tw_f);
+  if (!rc) rc = make_twiddles<double>(N, &p->tw_d);
+  if (!rc) rc = set_shell_mesh(p, auto_shell_mesh(N, m));
+  if (rc) {
+    bsk_plan_destroy(p);
+    return rc;
   }
   *out = p;
   return BSK_OK;
@@ -195,6 +257,8 @@ int bsk_plan_destroy(bsk_plan* p) {
   (void)hipFree(p->d_hi);
   (void)hipFree(p->tw_f);
   (void)hipFree(p->tw_d);
+  (void)hipFree(p->tws_f);
+  (void)hipFree(p->tws_d);
   (void)hipFree(p->comp_f);
   (void)hipFree(p->comp_d);
   delete p;
@@ -209,6 +273,27 @@ int bsk_plan_geometry(const bsk_plan* p, int* m, int* mx, int* my, int* mz, int*
   if (mz) *mz = p->g.MZ;
   if (nlo) *nlo = p->g.nlo;
   return BSK_OK;
+}
+
+int bsk_plan_shell_mesh(const bsk_plan* p, int* shell_mesh) {
+  if (int rc = check_plan(p)) return rc;
+  if (!shell_mesh) {
+    bsk_set_error("bsk_plan_shell_mesh: null output");
+    return BSK_EINVAL;
+  }
+  *shell_mesh = p->S;
+  return BSK_OK;
+}
+
+int bsk_plan_set_shell_mesh(bsk_plan* p, int shell_mesh) {
+  if (int rc = check_plan(p)) return rc;
+  const int S = shell_mesh == 0 ? auto_shell_mesh(p->N, p->m) : shell_mesh;
+  if (S != p->N && (!bsk_fft_size_supported(S) || S <= 3 * p->m || S > p->N)) {
+    bsk_set_error("bsk_plan_set_shell_mesh: %d is neither Nmesh=%d nor a supported size in "
+                  "(3m, Nmesh) with m=%d", shell_mesh, p->N, p->m);
+    return BSK_EINVAL;
+  }
+  return set_shell_mesh(p, S);
 }
 
 int bsk_plan_thresholds(const bsk_plan* p, int* n2lo, int* n2hi) {
@@ -279,8 +364,8 @@ int bsk_forward(bsk_plan* p, int dtype, const void* field, void* dk, void* work,
 int bsk_shells_workspace(const bsk_plan* p, int dtype, int nshell, int xslab, size_t* bytes) {
   if (int rc = check_plan(p)) return rc;
   const size_t c = dtype == BSK_DT_F64 ? 16 : 8;
-  const size_t N = p->N;
-  *bytes = c * (size_t)nshell * p->g.MZ * ((size_t)p->g.MY * N + (size_t)xslab * N);
+  const size_t S = p->S;
+  *bytes = c * (size_t)nshell * p->g.MZ * ((size_t)p->g.MY * S + (size_t)xslab * S);
   return BSK_OK;
 }
 
@@ -288,10 +373,10 @@ int bsk_shells(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int
                int xslab, int stages, int fmt, void* out, void* out_lo, long long shell_stride,
                long long out_cell0, void* work, void* stream) {
   if (int rc = check_plan(p)) return rc;
-  const long long slab_cells = (long long)xslab * p->N * p->N;
+  const long long slab_cells = (long long)xslab * p->S * p->S;
   const bool yz = (stages & BSK_STAGE_YZ) != 0;
   if (nshell <= 0 || shell0 < 0 || shell0 + nshell > p->nbin || x0 < 0 || xslab <= 0 ||
-      x0 + xslab > p->N || !work || stages < 1 || stages > 3 ||
+      x0 + xslab > p->S || !work || stages < 1 || stages > 3 ||
       ((stages & BSK_STAGE_X) && kind == BSK_KIND_DELTA && !dk) ||
       (kind < BSK_KIND_DELTA || kind > BSK_KIND_KMAG) ||
       (yz && (!out || out_cell0 < 0 || shell_stride < out_cell0 + slab_cells))) {
@@ -312,7 +397,8 @@ int bsk_shells(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int
       return BSK_EINVAL;
     }
     return shells_t<float>(p, kind, (const cplx<float>*)dk, shell0, nshell, x0, xslab, stages, fmt,
-                           out, out_lo, shell_stride, out_cell0, work, as_stream(stream), p->tw_f);
+                           out, out_lo, shell_stride, out_cell0, work, as_stream(stream),
+                           shell_tw(p, 0.f));
   }
   if (dtype == BSK_DT_F64) {
     if (fmt != BSK_FMT_F64) {
@@ -321,7 +407,7 @@ int bsk_shells(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int
     }
     return shells_t<double>(p, kind, (const cplx<double>*)dk, shell0, nshell, x0, xslab, stages,
                             fmt, out, nullptr, shell_stride, out_cell0, work, as_stream(stream),
-                            p->tw_d);
+                            shell_tw(p, 0.0));
   }
   bsk_set_error("bsk_shells: dtype must be BSK_DT_F32 or BSK_DT_F64");
   return BSK_EINVAL;

@@ -81,6 +81,38 @@ struct Dft<T, 8, INV> {
   }
 };
 
+template <typename T, bool INV>
+struct Dft<T, 5, INV> {
+  static BSK_DEV void run(cplx<T>* v) {
+    const T c1 = (T)0.30901699437494742410;   // cos(2 pi / 5)
+    const T c2 = (T)-0.80901699437494742410;  // cos(4 pi / 5)
+    const T s1 = (T)0.95105651629515357212;   // sin(2 pi / 5)
+    const T s2 = (T)0.58778525229247312917;   // sin(4 pi / 5)
+    const cplx<T> x0 = v[0];
+    const cplx<T> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+    const cplx<T> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    const cplx<T> a1 = {x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y};
+    const cplx<T> a2 = {x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y};
+    const cplx<T> b1 = mul_mi<T, INV>({s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y});
+    const cplx<T> b2 = mul_mi<T, INV>({s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y});
+    v[0] = cadd(x0, cadd(t1, t2));
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+  }
+};
+
+// LDS index of element i of a pencil: one padding slot after every 8 complex
+// values, so the stride-R stores of a Stockham stage and the stride-LD
+// accesses across pencils fall in distinct banks.
+BSK_DEV int lpad(int i) { return i + (i >> 3); }
+// padded pencil stride for length L (odd, > lpad(L-1))
+template <int L>
+constexpr int pencil_ld() {
+  return L + L / 8 + 1;
+}
+
 // ---------------------------------------------------------------- one stage
 template <typename T, int L, int R, int NS, bool INV, int P, int W, int LD, int TWN>
 BSK_DEV void stockham_stage(cplx<T>* buf, const cplx<T>* tw) {
@@ -94,9 +126,9 @@ BSK_DEV void stockham_stage(cplx<T>* buf, const cplx<T>* tw) {
     const int b = tid + q * W;
     if ((TOT % W) == 0 || b < TOT) {
       const int p = b / NB, j = b - p * NB;
-      const cplx<T>* src = buf + p * LD + j;
+      const cplx<T>* src = buf + p * LD;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = src[r * NB];
+      for (int r = 0; r < R; ++r) v[q][r] = src[lpad(j + r * NB)];
     }
   }
   __syncthreads();
@@ -115,9 +147,10 @@ BSK_DEV void stockham_stage(cplx<T>* buf, const cplx<T>* tw) {
         }
       }
       Dft<T, R, INV>::run(v[q]);
-      cplx<T>* dst = buf + p * LD + (j / NS) * NS * R + k;
+      cplx<T>* dst = buf + p * LD;
+      const int o = (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
+      for (int r = 0; r < R; ++r) dst[lpad(o + r * NS)] = v[q][r];
     }
   }
   __syncthreads();
@@ -147,15 +180,36 @@ struct Plan;
 BSK_PLAN(8, 0, 8) BSK_PLAN(8, 1, 4, 2)
 BSK_PLAN(12, 0, 4, 3) BSK_PLAN(12, 1, 2, 2, 3)
 BSK_PLAN(16, 0, 4, 4) BSK_PLAN(16, 1, 8, 2)
+BSK_PLAN(20, 0, 4, 5) BSK_PLAN(20, 1, 2, 2, 5)
 BSK_PLAN(24, 0, 8, 3) BSK_PLAN(24, 1, 4, 2, 3)
 BSK_PLAN(32, 0, 8, 4) BSK_PLAN(32, 1, 4, 4, 2)
+BSK_PLAN(36, 0, 4, 3, 3) BSK_PLAN(36, 1, 2, 2, 3, 3)
+BSK_PLAN(40, 0, 8, 5) BSK_PLAN(40, 1, 4, 2, 5)
 BSK_PLAN(48, 0, 8, 2, 3) BSK_PLAN(48, 1, 4, 4, 3)
+BSK_PLAN(60, 0, 4, 3, 5) BSK_PLAN(60, 1, 2, 2, 3, 5)
 BSK_PLAN(64, 0, 8, 8) BSK_PLAN(64, 1, 4, 4, 4)
+BSK_PLAN(72, 0, 8, 3, 3) BSK_PLAN(72, 1, 4, 2, 3, 3)
+BSK_PLAN(80, 0, 8, 2, 5) BSK_PLAN(80, 1, 4, 4, 5)
 BSK_PLAN(96, 0, 8, 4, 3) BSK_PLAN(96, 1, 4, 4, 2, 3)
+BSK_PLAN(100, 0, 4, 5, 5) BSK_PLAN(100, 1, 2, 2, 5, 5)
+BSK_PLAN(108, 0, 4, 3, 3, 3) BSK_PLAN(108, 1, 2, 2, 3, 3, 3)
+BSK_PLAN(120, 0, 8, 3, 5) BSK_PLAN(120, 1, 4, 2, 3, 5)
 BSK_PLAN(128, 0, 8, 4, 4) BSK_PLAN(128, 1, 4, 4, 4, 2)
+BSK_PLAN(144, 0, 8, 2, 3, 3) BSK_PLAN(144, 1, 4, 4, 3, 3)
+BSK_PLAN(160, 0, 8, 4, 5) BSK_PLAN(160, 1, 4, 4, 2, 5)
 BSK_PLAN(192, 0, 8, 8, 3) BSK_PLAN(192, 1, 4, 4, 4, 3)
+BSK_PLAN(200, 0, 8, 5, 5) BSK_PLAN(200, 1, 4, 2, 5, 5)
+BSK_PLAN(216, 0, 8, 3, 3, 3) BSK_PLAN(216, 1, 4, 2, 3, 3, 3)
+BSK_PLAN(240, 0, 8, 2, 3, 5) BSK_PLAN(240, 1, 4, 4, 3, 5)
 BSK_PLAN(256, 0, 8, 8, 4) BSK_PLAN(256, 1, 4, 4, 4, 4)
+BSK_PLAN(288, 0, 8, 4, 3, 3) BSK_PLAN(288, 1, 4, 4, 2, 3, 3)
+BSK_PLAN(320, 0, 8, 8, 5) BSK_PLAN(320, 1, 4, 4, 4, 5)
 BSK_PLAN(384, 0, 8, 8, 2, 3) BSK_PLAN(384, 1, 8, 4, 4, 3)
+BSK_PLAN(400, 0, 8, 2, 5, 5) BSK_PLAN(400, 1, 4, 4, 5, 5)
+BSK_PLAN(432, 0, 8, 2, 3, 3, 3) BSK_PLAN(432, 1, 4, 4, 3, 3, 3)
+BSK_PLAN(480, 0, 8, 4, 3, 5) BSK_PLAN(480, 1, 4, 4, 2, 3, 5)
 BSK_PLAN(512, 0, 8, 8, 8) BSK_PLAN(512, 1, 4, 4, 4, 8)
+BSK_PLAN(576, 0, 8, 8, 3, 3) BSK_PLAN(576, 1, 4, 4, 4, 3, 3)
+BSK_PLAN(640, 0, 8, 8, 2, 5) BSK_PLAN(640, 1, 8, 4, 4, 5)
 BSK_PLAN(768, 0, 8, 8, 4, 3) BSK_PLAN(768, 1, 4, 4, 4, 4, 3)
 #undef BSK_PLAN

@@ -1,18 +1,24 @@
 // Host launchers for the FFT passes: runtime (N, variant, precision) ->
-// compile-time kernel instance.
+// compile-time kernel instance.  The file is compiled once per pass
+// (BSK_PASS = 0..5: forward z, y, x; inverse x, y, z) so the instances build
+// in parallel.
 #include "fft_passes.h"
 #include "launch.h"
+
+#ifndef BSK_PASS
+#error "compile with -DBSK_PASS=0..5"
+#endif
 
 namespace {
 
 constexpr int kW = 256;
 
 // Pencils per workgroup for the strided x/y passes: 16 consecutive kz values
-// (128 B in FP32) per grid point; FP64 and the longest axis use 8 to stay
+// (128 B in FP32) per grid point; FP64 and lengths above 640 use 8 to stay
 // inside the LDS budget.
 template <typename T, int N>
 constexpr int pxy() {
-  return (sizeof(T) == 8 || N >= 768) ? 8 : 16;
+  return (sizeof(T) == 8 || N > 640) ? 8 : 16;
 }
 // Rows per workgroup for the contiguous z passes: about 4096 complex values.
 template <typename T, int NH>
@@ -23,8 +29,29 @@ constexpr int pz() {
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
 
-#define BSK_SIZES(X) X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768)
+}  // namespace
 
+#define BSK_SWITCH(T, N, V, CALL)                                                    \
+  switch (N) {                                                                       \
+    BSK_FFT_SIZES(CALL)                                                              \
+    default:                                                                         \
+      bsk_set_error("unsupported FFT size %d", N);                                   \
+      return BSK_EUNSUPPORTED;                                                       \
+  }                                                                                  \
+  BSK_LAUNCH_CHECK();                                                                \
+  return BSK_OK;
+
+// one case: variant 1 exists for FP32 only
+#define BSK_VCASE(n, EXPR0, EXPR1)                          \
+  case n:                                                   \
+    if constexpr (sizeof(T) == 4) {                         \
+      if (V == 1) { EXPR1; break; }                         \
+    }                                                       \
+    EXPR0;                                                  \
+    break;
+
+#if BSK_PASS == 0
+namespace {
 template <typename T, int N, int V>
 void fwd_z_n(const T* in, cplx<T>* out, const cplx<T>* tw, long long nrows, int MZ,
              hipStream_t s) {
@@ -32,12 +59,45 @@ void fwd_z_n(const T* in, cplx<T>* out, const cplx<T>* tw, long long nrows, int 
   hipLaunchKernelGGL((k_fwd_z<T, NH, V, P, kW>), dim3(cdiv(nrows, P)), dim3(kW), 0, s, in, out,
                      tw, nrows, MZ);
 }
+}  // namespace
+template <typename T>
+int bsk_fwd_z(int N, int V, const T* in, cplx<T>* out, const cplx<T>* tw, long long nrows, int MZ,
+              hipStream_t s) {
+#define C(n) BSK_VCASE(n, (fwd_z_n<T, n, 0>(in, out, tw, nrows, MZ, s)), \
+                       (fwd_z_n<T, n, 1>(in, out, tw, nrows, MZ, s)))
+  BSK_SWITCH(T, N, V, C)
+#undef C
+}
+template int bsk_fwd_z<float>(int, int, const float*, cplx<float>*, const cplx<float>*, long long,
+                              int, hipStream_t);
+template int bsk_fwd_z<double>(int, int, const double*, cplx<double>*, const cplx<double>*,
+                               long long, int, hipStream_t);
+#endif
+
+#if BSK_PASS == 1
+namespace {
 template <typename T, int N, int V>
 void fwd_y_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, hipStream_t s) {
   constexpr int P = pxy<T, N>();
   hipLaunchKernelGGL((k_fwd_y<T, N, V, P, kW>), dim3(N, cdiv(g.MZ, P)), dim3(kW), 0, s, in, out,
                      tw, g);
 }
+}  // namespace
+template <typename T>
+int bsk_fwd_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g,
+              hipStream_t s) {
+#define C(n) BSK_VCASE(n, (fwd_y_n<T, n, 0>(in, out, tw, g, s)), (fwd_y_n<T, n, 1>(in, out, tw, g, s)))
+  BSK_SWITCH(T, N, V, C)
+#undef C
+}
+template int bsk_fwd_y<float>(int, int, const cplx<float>*, cplx<float>*, const cplx<float>*, Geom,
+                              hipStream_t);
+template int bsk_fwd_y<double>(int, int, const cplx<double>*, cplx<double>*, const cplx<double>*,
+                               Geom, hipStream_t);
+#endif
+
+#if BSK_PASS == 2
+namespace {
 template <typename T, int N, int V>
 void fwd_x_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, T scale, const T* comp,
              hipStream_t s) {
@@ -45,6 +105,23 @@ void fwd_x_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, T scale
   hipLaunchKernelGGL((k_fwd_x<T, N, V, P, kW>), dim3(g.MY, cdiv(g.MZ, P)), dim3(kW), 0, s, in,
                      out, tw, g, scale, comp);
 }
+}  // namespace
+template <typename T>
+int bsk_fwd_x(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, T scale,
+              const T* comp, hipStream_t s) {
+#define C(n) BSK_VCASE(n, (fwd_x_n<T, n, 0>(in, out, tw, g, scale, comp, s)), \
+                       (fwd_x_n<T, n, 1>(in, out, tw, g, scale, comp, s)))
+  BSK_SWITCH(T, N, V, C)
+#undef C
+}
+template int bsk_fwd_x<float>(int, int, const cplx<float>*, cplx<float>*, const cplx<float>*, Geom,
+                              float, const float*, hipStream_t);
+template int bsk_fwd_x<double>(int, int, const cplx<double>*, cplx<double>*, const cplx<double>*,
+                               Geom, double, const double*, hipStream_t);
+#endif
+
+#if BSK_PASS == 3
+namespace {
 template <typename T, int N, int V>
 void inv_x_n(const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g, const int* lo,
              const int* hi, int shell0, int nsh, int kind, T kf, hipStream_t s) {
@@ -52,6 +129,23 @@ void inv_x_n(const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g, const i
   hipLaunchKernelGGL((k_inv_x_mask<T, N, V, P, kW>), dim3(g.MY, cdiv(g.MZ, P), nsh), dim3(kW),
                      0, s, dk, out, tw, g, lo, hi, shell0, kind, kf);
 }
+}  // namespace
+template <typename T>
+int bsk_inv_x(int N, int V, const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g,
+              const int* lo, const int* hi, int shell0, int nsh, int kind, T kf, hipStream_t s) {
+#define C(n) BSK_VCASE(n, (inv_x_n<T, n, 0>(dk, out, tw, g, lo, hi, shell0, nsh, kind, kf, s)), \
+                       (inv_x_n<T, n, 1>(dk, out, tw, g, lo, hi, shell0, nsh, kind, kf, s)))
+  BSK_SWITCH(T, N, V, C)
+#undef C
+}
+template int bsk_inv_x<float>(int, int, const cplx<float>*, cplx<float>*, const cplx<float>*, Geom,
+                              const int*, const int*, int, int, int, float, hipStream_t);
+template int bsk_inv_x<double>(int, int, const cplx<double>*, cplx<double>*, const cplx<double>*,
+                               Geom, const int*, const int*, int, int, int, double, hipStream_t);
+#endif
+
+#if BSK_PASS == 4
+namespace {
 template <typename T, int N, int V>
 void inv_y_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0, int xs, int nsh,
              hipStream_t s) {
@@ -59,6 +153,23 @@ void inv_y_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0,
   hipLaunchKernelGGL((k_inv_y<T, N, V, P, kW>), dim3(xs, cdiv(g.MZ, P), nsh), dim3(kW), 0, s, in,
                      out, tw, g, x0, xs);
 }
+}  // namespace
+template <typename T>
+int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0,
+              int xs, int nsh, hipStream_t s) {
+#define C(n) BSK_VCASE(n, (inv_y_n<T, n, 0>(in, out, tw, g, x0, xs, nsh, s)), \
+                       (inv_y_n<T, n, 1>(in, out, tw, g, x0, xs, nsh, s)))
+  BSK_SWITCH(T, N, V, C)
+#undef C
+}
+template int bsk_inv_y<float>(int, int, const cplx<float>*, cplx<float>*, const cplx<float>*, Geom,
+                              int, int, int, hipStream_t);
+template int bsk_inv_y<double>(int, int, const cplx<double>*, cplx<double>*, const cplx<double>*,
+                               Geom, int, int, int, hipStream_t);
+#endif
+
+#if BSK_PASS == 5
+namespace {
 template <typename T, int N, int V>
 void inv_z_n(const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
              long long nrows, long long cell0, long long stride, int fmt, int nsh, hipStream_t s) {
@@ -66,147 +177,18 @@ void inv_z_n(const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int 
   hipLaunchKernelGGL((k_inv_z<T, NH, V, P, kW>), dim3(cdiv(nrows, P), nsh), dim3(kW), 0, s, in,
                      out, out_lo, tw, MZ, nrows, cell0, stride, fmt);
 }
-
 }  // namespace
-
-bool bsk_fft_size_supported(int N) {
-  switch (N) {
-#define X(n) case n:
-    BSK_SIZES(X)
-#undef X
-    return true;
-    default:
-      return false;
-  }
-}
-
-template <typename T>
-int bsk_fwd_z(int N, int V, const T* in, cplx<T>* out, const cplx<T>* tw, long long nrows, int MZ,
-              hipStream_t s) {
-  switch (N) {
-#define X(n)                                                                      \
-  case n:                                                                         \
-    if constexpr (sizeof(T) == 4) { if (V == 1) { fwd_z_n<T, n, 1>(in, out, tw, nrows, MZ, s); break; } } \
-    fwd_z_n<T, n, 0>(in, out, tw, nrows, MZ, s);                             \
-    break;
-    BSK_SIZES(X)
-#undef X
-    default:
-      bsk_set_error("unsupported FFT size %d", N);
-      return BSK_EUNSUPPORTED;
-  }
-  BSK_LAUNCH_CHECK();
-  return BSK_OK;
-}
-
-template <typename T>
-int bsk_fwd_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g,
-              hipStream_t s) {
-  switch (N) {
-#define X(n)                                                                      \
-  case n:                                                                         \
-    if constexpr (sizeof(T) == 4) { if (V == 1) { fwd_y_n<T, n, 1>(in, out, tw, g, s); break; } } \
-    fwd_y_n<T, n, 0>(in, out, tw, g, s);                                     \
-    break;
-    BSK_SIZES(X)
-#undef X
-    default:
-      bsk_set_error("unsupported FFT size %d", N);
-      return BSK_EUNSUPPORTED;
-  }
-  BSK_LAUNCH_CHECK();
-  return BSK_OK;
-}
-
-template <typename T>
-int bsk_fwd_x(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, T scale,
-              const T* comp, hipStream_t s) {
-  switch (N) {
-#define X(n)                                                                      \
-  case n:                                                                         \
-    if constexpr (sizeof(T) == 4) { if (V == 1) { fwd_x_n<T, n, 1>(in, out, tw, g, scale, comp, s); break; } } \
-    fwd_x_n<T, n, 0>(in, out, tw, g, scale, comp, s);                        \
-    break;
-    BSK_SIZES(X)
-#undef X
-    default:
-      bsk_set_error("unsupported FFT size %d", N);
-      return BSK_EUNSUPPORTED;
-  }
-  BSK_LAUNCH_CHECK();
-  return BSK_OK;
-}
-
-template <typename T>
-int bsk_inv_x(int N, int V, const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g,
-              const int* lo, const int* hi, int shell0, int nsh, int kind, T kf, hipStream_t s) {
-  switch (N) {
-#define X(n)                                                                             \
-  case n:                                                                                \
-    if constexpr (sizeof(T) == 4) { if (V == 1) { inv_x_n<T, n, 1>(dk, out, tw, g, lo, hi, shell0, nsh, kind, kf, s); break; } } \
-    inv_x_n<T, n, 0>(dk, out, tw, g, lo, hi, shell0, nsh, kind, kf, s);             \
-    break;
-    BSK_SIZES(X)
-#undef X
-    default:
-      bsk_set_error("unsupported FFT size %d", N);
-      return BSK_EUNSUPPORTED;
-  }
-  BSK_LAUNCH_CHECK();
-  return BSK_OK;
-}
-
-template <typename T>
-int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0,
-              int xs, int nsh, hipStream_t s) {
-  switch (N) {
-#define X(n)                                                                      \
-  case n:                                                                         \
-    if constexpr (sizeof(T) == 4) { if (V == 1) { inv_y_n<T, n, 1>(in, out, tw, g, x0, xs, nsh, s); break; } } \
-    inv_y_n<T, n, 0>(in, out, tw, g, x0, xs, nsh, s);                        \
-    break;
-    BSK_SIZES(X)
-#undef X
-    default:
-      bsk_set_error("unsupported FFT size %d", N);
-      return BSK_EUNSUPPORTED;
-  }
-  BSK_LAUNCH_CHECK();
-  return BSK_OK;
-}
-
 template <typename T>
 int bsk_inv_z(int N, int V, const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
               long long nrows, long long cell0, long long stride, int fmt, int nsh,
               hipStream_t s) {
-  switch (N) {
-#define X(n)                                                                             \
-  case n:                                                                                \
-    if constexpr (sizeof(T) == 4) { if (V == 1) { inv_z_n<T, n, 1>(in, out, out_lo, tw, MZ, nrows, cell0, stride, fmt, nsh, s); break; } } \
-    inv_z_n<T, n, 0>(in, out, out_lo, tw, MZ, nrows, cell0, stride, fmt, nsh, s);   \
-    break;
-    BSK_SIZES(X)
-#undef X
-    default:
-      bsk_set_error("unsupported FFT size %d", N);
-      return BSK_EUNSUPPORTED;
-  }
-  BSK_LAUNCH_CHECK();
-  return BSK_OK;
+#define C(n) BSK_VCASE(n, (inv_z_n<T, n, 0>(in, out, out_lo, tw, MZ, nrows, cell0, stride, fmt, nsh, s)), \
+                       (inv_z_n<T, n, 1>(in, out, out_lo, tw, MZ, nrows, cell0, stride, fmt, nsh, s)))
+  BSK_SWITCH(T, N, V, C)
+#undef C
 }
-
-#define BSK_INST(T)                                                                        \
-  template int bsk_fwd_z<T>(int, int, const T*, cplx<T>*, const cplx<T>*, long long, int,    \
-                            hipStream_t);                                                  \
-  template int bsk_fwd_y<T>(int, int, const cplx<T>*, cplx<T>*, const cplx<T>*, Geom,        \
-                            hipStream_t);                                                  \
-  template int bsk_fwd_x<T>(int, int, const cplx<T>*, cplx<T>*, const cplx<T>*, Geom, T,     \
-                            const T*, hipStream_t);                                        \
-  template int bsk_inv_x<T>(int, int, const cplx<T>*, cplx<T>*, const cplx<T>*, Geom,        \
-                            const int*, const int*, int, int, int, T, hipStream_t);        \
-  template int bsk_inv_y<T>(int, int, const cplx<T>*, cplx<T>*, const cplx<T>*, Geom, int,   \
-                            int, int, hipStream_t);                                        \
-  template int bsk_inv_z<T>(int, int, const cplx<T>*, void*, void*, const cplx<T>*, int,     \
-                            long long, long long, long long, int, int, hipStream_t);
-BSK_INST(float)
-BSK_INST(double)
+template int bsk_inv_z<float>(int, int, const cplx<float>*, void*, void*, const cplx<float>*, int,
+                              long long, long long, long long, int, int, hipStream_t);
+template int bsk_inv_z<double>(int, int, const cplx<double>*, void*, void*, const cplx<double>*,
+                               int, long long, long long, long long, int, int, hipStream_t);
+#endif

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
                                              const cplx<T>* __restrict__ twg,
                                              long long nrows, int MZ) {
   constexpr int N = 2 * NH;
-  constexpr int LD = NH + 1;
+  constexpr int LD = pencil_ld<NH>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
     const long long row = row0 + p;
     cplx<T> v = {(T)0, (T)0};
     if (row < nrows) v = src[row * NH + n];
-    buf[p * LD + n] = v;
+    buf[p * LD + lpad(n)] = v;
   }
   __syncthreads();
   fft_lds<T, NH, false, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
@@ -53,8 +53,8 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
     const int p = e / MZ, k = e - p * MZ;
     const long long row = row0 + p;
     if (row >= nrows) continue;
-    cplx<T> zk = buf[p * LD + (k % NH)];
-    cplx<T> zc = cconj(buf[p * LD + ((NH - k) % NH)]);
+    cplx<T> zk = buf[p * LD + lpad((k % NH))];
+    cplx<T> zc = cconj(buf[p * LD + lpad(((NH - k) % NH))]);
     cplx<T> xe = cscale(cadd(zk, zc), (T)0.5);
     cplx<T> d = csub(zk, zc);
     cplx<T> xo = {(T)0.5 * d.y, (T)-0.5 * d.x};  // (zk - zc) / (2i)
@@ -67,7 +67,7 @@ template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
                                              cplx<T>* __restrict__ out,
                                              const cplx<T>* __restrict__ twg, Geom g) {
-  constexpr int LD = N + 1;
+  constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
     const int y = e / P, p = e - y * P, kz = kz0 + p;
     cplx<T> v = {(T)0, (T)0};
     if (kz < g.MZ) v = in[(x * N + y) * g.MZ + kz];
-    buf[p * LD + y] = v;
+    buf[p * LD + lpad(y)] = v;
   }
   __syncthreads();
   fft_lds<T, N, false, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
     const int r = e / P, p = e - r * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
     const int idx = (g.nlo + r + N) % N;
-    out[(x * g.MY + r) * g.MZ + kz] = buf[p * LD + idx];
+    out[(x * g.MY + r) * g.MZ + kz] = buf[p * LD + lpad(idx)];
   }
 }
 
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
                                              cplx<T>* __restrict__ out,
                                              const cplx<T>* __restrict__ twg, Geom g,
                                              T scale, const T* __restrict__ comp) {
-  constexpr int LD = N + 1;
+  constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
     const int x = e / P, p = e - x * P, kz = kz0 + p;
     cplx<T> v = {(T)0, (T)0};
     if (kz < g.MZ) v = in[((long long)x * g.MY + ry) * g.MZ + kz];
-    buf[p * LD + x] = v;
+    buf[p * LD + lpad(x)] = v;
   }
   __syncthreads();
   fft_lds<T, N, false, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
     const int nx = g.nlo + r;
     T s = scale;
     if (comp) s *= comp[nx + N / 2] * comp[ny + N / 2] * comp[kz + N / 2];
-    out[(ry * g.MX + r) * g.MZ + kz] = cscale(buf[p * LD + (nx + N) % N], s);
+    out[(ry * g.MX + r) * g.MZ + kz] = cscale(buf[p * LD + lpad((nx + N) % N)], s);
   }
 }
 
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
                                                   const int* __restrict__ n2lo,
                                                   const int* __restrict__ n2hi, int shell0,
                                                   int kind, T kf) {
-  constexpr int LD = N + 1;
+  constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
           v.x = kf * sqrt((T)n2);
       }
     }
-    buf[p * LD + xi] = v;
+    buf[p * LD + lpad(xi)] = v;
   }
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
   for (int e = tid; e < P * N; e += W) {
     const int x = e / P, p = e - x * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
-    out[(obase + x) * g.MZ + kz] = buf[p * LD + x];
+    out[(obase + x) * g.MZ + kz] = buf[p * LD + lpad(x)];
   }
 }
 
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
                                              cplx<T>* __restrict__ out,
                                              const cplx<T>* __restrict__ twg, Geom g, int x0,
                                              int xs) {
-  constexpr int LD = N + 1;
+  constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
     cplx<T> v = {(T)0, (T)0};
     if (kz < g.MZ && ry >= 0 && ry < g.MY)
       v = in[((bl * g.MY + ry) * N + x) * g.MZ + kz];
-    buf[p * LD + yi] = v;
+    buf[p * LD + lpad(yi)] = v;
   }
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
   for (int e = tid; e < P * N; e += W) {
     const int y = e / P, p = e - y * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
-    out[(obase + y) * g.MZ + kz] = buf[p * LD + y];
+    out[(obase + y) * g.MZ + kz] = buf[p * LD + lpad(y)];
   }
 }
 
@@ -219,11 +219,15 @@ BSK_DEV void store_pair(int fmt, void* out, void* out_lo, long long off, T a, T 
 template <>
 BSK_DEV void store_pair<float>(int fmt, void* out, void* out_lo, long long off, float a, float b) {
   if (fmt == BSK_FMT_BF16X2) {
-    uint32_t ha = bf16_bits_rne(a), hb = bf16_bits_rne(b);
-    float ra = a - __uint_as_float(ha << 16), rb = b - __uint_as_float(hb << 16);
-    uint32_t la = bf16_bits_rne(ra), lb = bf16_bits_rne(rb);
-    reinterpret_cast<uint32_t*>(out)[off >> 1] = ha | (hb << 16);
-    reinterpret_cast<uint32_t*>(out_lo)[off >> 1] = la | (lb << 16);
+    // hi = rne_bf16(v), lo = rne_bf16(v - hi) with v_cvt_pk_bf16_f32
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+    const f32x2 v = {a, b};
+    const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+    const f32x2 r = {a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u)};
+    const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
+    reinterpret_cast<uint32_t*>(out)[off >> 1] = hi;
+    reinterpret_cast<uint32_t*>(out_lo)[off >> 1] = lo;
   } else {
     reinterpret_cast<float2*>(out)[off >> 1] = make_float2(a, b);
   }
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
                                              int MZ, long long nrows, long long cell0,
                                              long long shell_stride, int fmt) {
   constexpr int N = 2 * NH;
-  constexpr int LD = NH + 1;
+  constexpr int LD = pencil_ld<NH>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
@@ -265,7 +269,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
       cplx<T> wd = cmul(w, d);
       z = {s.x - wd.y, s.y + wd.x};  // s + i * w * d
     }
-    buf[p * LD + k] = z;
+    buf[p * LD + lpad(k)] = z;
   }
   __syncthreads();
   fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
@@ -273,7 +277,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
     const int p = e / NH, n = e - p * NH;
     const long long row = row0 + p;
     if (row >= nrows) continue;
-    const cplx<T> z = buf[p * LD + n];
+    const cplx<T> z = buf[p * LD + lpad(n)];
     const long long off = bl * shell_stride + cell0 + row * N + 2 * n;
     store_pair<T>(fmt, out, out_lo, off, z.x, z.y);
   }

@@ -3,7 +3,16 @@
 #include "common.h"
 #include "fft_passes.h"
 
+// Mesh sizes with compiled FFT kernels: even 5-smooth lengths whose half
+// also has a radix plan (the z passes run N/2-point complex transforms).
+#define BSK_FFT_SIZES(X)                                                                  \
+  X(16) X(24) X(32) X(40) X(48) X(64) X(72) X(80) X(96) X(120) X(128) X(144) X(160)       \
+  X(192) X(200) X(216) X(240) X(256) X(288) X(320) X(384) X(400) X(432) X(480) X(512)    \
+  X(576) X(640) X(768)
+
 bool bsk_fft_size_supported(int N);
+// Smallest supported mesh size strictly greater than n (0 if none).
+int bsk_fft_size_above(int n);
 
 template <typename T>
 int bsk_fwd_z(int N, int V, const T* in, cplx<T>* out, const cplx<T>* tw, long long nrows, int MZ,

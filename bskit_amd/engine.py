@@ -99,9 +99,19 @@ class ShellSet:
 
 
 class GridEngine:
-    """Plan + kernels for one (Nmesh, BoxSize, bins) key on one device."""
+    """Plan + kernels for one (Nmesh, BoxSize, bins) key on one device.
 
-    def __init__(self, nmesh, box_size, k_low, k_high, device=None, cic_window=0):
+    The forward transform runs on the N^3 input mesh; shells and triple sums
+    run on the shell mesh S^3 (``self.S``, ``self.ncell = S^3``).  S is the
+    smallest supported size above 3m (m = largest retained |n| component),
+    or N: the cell sums on S^3 equal (S/N)^3 times those on N^3 exactly
+    (no product of three shells holds a frequency that aliases to zero on
+    either mesh), so every sum is scaled by 1/S^3 where the reference
+    divides by N^3.  ``shell_mesh=N`` evaluates on the reference's own mesh.
+    """
+
+    def __init__(self, nmesh, box_size, k_low, k_high, device=None, cic_window=0,
+                 shell_mesh=None):
         self.device = require_device(device)
         self.N = int(nmesh)
         self.box = float(box_size)
@@ -110,7 +120,6 @@ class GridEngine:
         self.nb = len(self.k_low)
         if self.nb == 0:
             raise ValueError('at least one k bin is needed')
-        self.ncell = self.N ** 3
         h = ctypes.c_void_p()
         with torch.cuda.device(self.device):
             L.call('bsk_plan_create', ctypes.byref(h), self.N, self.box, self.nb,
@@ -124,6 +133,7 @@ class GridEngine:
         L.call('bsk_plan_thresholds', self._plan, lo, hi)
         self.n2lo = np.array(lo[:], dtype=np.int64)
         self.n2hi = np.array(hi[:], dtype=np.int64)
+        self._set_shell_mesh(0 if shell_mesh is None else int(shell_mesh))
         self.work = Workspace(self.device)
         self._tri_cache = {}
         self.cic_window = 0
@@ -142,6 +152,18 @@ class GridEngine:
             self.close()
         except Exception:
             pass
+
+    def _set_shell_mesh(self, shell_mesh):
+        L.call('bsk_plan_set_shell_mesh', self._plan, int(shell_mesh))
+        s = ctypes.c_int()
+        L.call('bsk_plan_shell_mesh', self._plan, ctypes.byref(s))
+        self.S = s.value
+        self.ncell = self.S ** 3
+
+    def set_shell_mesh(self, shell_mesh=0):
+        """Shell mesh size (0 = automatic, Nmesh = the reference's own mesh)."""
+        self._set_shell_mesh(shell_mesh)
+        self._tri_cache.clear()
 
     def set_compensation(self, nmesh_window):
         L.call('bsk_plan_set_compensation', self._plan, int(nmesh_window))
@@ -175,7 +197,7 @@ class GridEngine:
     # ---------------------------------------------------------------- shells
     def shells(self, dk=None, kind=L.KIND_DELTA, fmt=L.FMT_F32, dtype=torch.float32,
                shell0=0, nshell=None, out=None, stream=None):
-        """All (or a range of) shell fields of one spectrum, full grid."""
+        """All (or a range of) shell fields of one spectrum on the full S^3 mesh."""
         nshell = self.nb - shell0 if nshell is None else nshell
         dt = _dt(dtype)
         if kind == L.KIND_DELTA:
@@ -185,9 +207,9 @@ class GridEngine:
         if out is None:
             out = self.alloc_shells(fmt, self.nb)
         nbytes = L.size_out()
-        L.call('bsk_shells_workspace', self._plan, dt, nshell, self.N, ctypes.byref(nbytes))
+        L.call('bsk_shells_workspace', self._plan, dt, nshell, self.S, ctypes.byref(nbytes))
         w = self.work.get('shell', nbytes.value)
-        L.call('bsk_shells', self._plan, dt, kind, L.ptr(dk), shell0, nshell, 0, self.N,
+        L.call('bsk_shells', self._plan, dt, kind, L.ptr(dk), shell0, nshell, 0, self.S,
                L.STAGE_X | L.STAGE_YZ, fmt, L.ptr(out.data), L.ptr(out.lo), self.ncell, 0,
                L.ptr(w), stream_handle(stream))
         return out
@@ -326,8 +348,8 @@ class GridEngine:
         """(N_tri, kbar) for each triangle (i, j, k) of bin indices, FP64.
 
         The unit and |k| shells are made in FP64 x-slab by x-slab and reduced
-        with the tiled FP64 kernel; N_tri is the rounded sum / N^3 (exact
-        integer), kbar = sum(|k| n n) / N^3 / N_tri (NaN where N_tri == 0).
+        with the tiled FP64 kernel; N_tri is the rounded sum / S^3 (exact
+        integer), kbar = sum(|k| n n) / S^3 / N_tri (NaN where N_tri == 0).
         """
         tri = np.asarray(tri, dtype=np.int32).reshape(-1, 3)
         T = len(tri)
@@ -335,7 +357,7 @@ class GridEngine:
             return np.zeros(0), np.zeros((0, 3))
         if tri.min() < 0 or tri.max() >= self.nb:
             raise IndexError('triangle bin index out of range')
-        N, nb = self.N, self.nb
+        N, nb = self.S, self.nb
         if max_bytes is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             max_bytes = int(free * 0.5)

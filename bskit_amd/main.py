@@ -55,14 +55,21 @@ def _bins_key(kLU):
     return tuple((float(a), float(b)) for a, b in np.asarray(kLU, dtype=np.float64))
 
 
-def get_engine(nmesh, box, kLU, device=None, cic_window=0):
-    """Shared GridEngine per (Nmesh, BoxSize, bins, window) key (LRU)."""
+def get_engine(nmesh, box, kLU, device=None, cic_window=0, shell_mesh=None):
+    """Shared GridEngine per (Nmesh, BoxSize, bins, window, shell mesh) key (LRU).
+
+    shell_mesh None picks the smallest exact shell mesh; Nmesh keeps the
+    shells on the input mesh (needed where the shell fields themselves are
+    returned).
+    """
     dev = require_device(device)
-    key = (int(nmesh), float(box), _bins_key(kLU), str(dev), int(cic_window))
+    key = (int(nmesh), float(box), _bins_key(kLU), str(dev), int(cic_window),
+           None if shell_mesh is None else int(shell_mesh))
     eng = _ENGINES.pop(key, None)
     if eng is None:
         kLU = np.asarray(kLU, dtype=np.float64)
-        eng = GridEngine(nmesh, box, kLU[:, 0], kLU[:, 1], device=dev, cic_window=cic_window)
+        eng = GridEngine(nmesh, box, kLU[:, 0], kLU[:, 1], device=dev, cic_window=cic_window,
+                         shell_mesh=shell_mesh)
         eng.gridinfo_cache = {}
     _ENGINES[key] = eng
     while len(_ENGINES) > _ENGINE_LIMIT:
@@ -300,10 +307,12 @@ class FFTBispectrum:
         path = 'mfma' if fmt == L.FMT_BF16X2 else 'plain'
         return sets, path
 
-    def _unnorm_sums(self, kLU, tri, scale):
+    def _unnorm_sums(self, kLU, tri, factor):
+        """factor * sum_x d_i d_j d_k / ncell: the reference's L^6/N^3 sum
+        (main.py:1875-1882) evaluated on the engine's shell mesh."""
         eng = self._engine(kLU)
         sets, path = self._shellsets(eng, kLU, tri)
-        out = eng.triples(sets, tri, path=path, scale=scale)
+        out = eng.triples(sets, tri, path=path, scale=factor / float(eng.ncell))
         return out.cpu().numpy()
 
     def _append_b(self, index, edges, kmean, B, ntri):
@@ -341,12 +350,11 @@ class FFTBispectrum:
         if T:
             kLU, tri = _unique_bins(tri_edges * u)
             eng = self._engine(kLU)
-            N3 = float(self._N) ** 3
             if meas_type in ('full', 'grid_info'):
                 ntri, kmean = _grid_info(eng, tri)
                 kmean = kmean / u
             if meas_type in ('full', 'unnorm_b_value'):
-                s = self._unnorm_sums(kLU, tri, self._L ** 6 / N3 * u ** 6)
+                s = self._unnorm_sums(kLU, tri, self._L ** 6 * u ** 6)
                 if meas_type == 'full':
                     with np.errstate(divide='ignore', invalid='ignore'):
                         B = np.where(ntri != 0, s / np.where(ntri != 0, ntri, 1.0), np.nan)
@@ -380,7 +388,7 @@ class FFTBispectrum:
         u = float(self.attrs['pos_units_mpcoverh'])
         B = np.zeros(T)
         if T:
-            B = self._unnorm_sums(kLU, tri, self._L ** 6 / float(self._N) ** 3 * u ** 6)
+            B = self._unnorm_sums(kLU, tri, self._L ** 6 * u ** 6)
         if verbose > 0:
             for t in range(T):
                 print('FFT Bk: %d %d %g' % (t, index[t], B[t]))
@@ -440,7 +448,7 @@ def _box_n(box_size, n_mesh):
 
 def _single_shell(box_size, n_mesh, kmin, kmax, kind):
     box, N = _box_n(box_size, n_mesh)
-    eng = get_engine(N, box, [[kmin, kmax]])
+    eng = get_engine(N, box, [[kmin, kmax]], shell_mesh=N)
     s = eng.shells(None, kind=kind, fmt=L.FMT_F64, dtype=torch.float64)
     return s.data[0].view(N, N, N)
 

@@ -16,7 +16,9 @@
  *   - Return value: 0 on success, negative BSK_E* code on error; the message
  *     of the last error on the calling thread is bsk_last_error().
  *   - Grids are cubic: Nmesh^3 cells in a box of side box_size.  Supported
- *     Nmesh: 16, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768.
+ *     Nmesh: the even 5-smooth sizes 16, 24, 32, 40, 48, 64, 72, 80, 96, 120,
+ *     128, 144, 160, 192, 200, 216, 240, 256, 288, 320, 384, 400, 432, 480,
+ *     512, 576, 640, 768.
  *   - Shell membership: bin b holds the modes with n2lo[b] <= |n|^2 <= n2hi[b]
  *     (n = k / k_f integer vector, components in [-N/2, N/2]); the integer
  *     limits are the closed interval k_low <= |k| <= k_high with modes
@@ -32,7 +34,7 @@
 extern "C" {
 #endif
 
-#define BSK_ABI_VERSION 1
+#define BSK_ABI_VERSION 2
 
 /* status codes */
 #define BSK_OK 0
@@ -80,6 +82,17 @@ int bsk_plan_destroy(bsk_plan* plan);
  * first signed frequency nlo) and 0 <= n_z < mz. */
 int bsk_plan_geometry(const bsk_plan* plan, int* m, int* mx, int* my, int* mz, int* nlo);
 int bsk_plan_thresholds(const bsk_plan* plan, int* n2lo, int* n2hi);
+/* Shell mesh S: the shell fields and triple sums are evaluated on S^3 cells.
+ * Every shell mode has components |n_a| <= m, so a product of three shells
+ * has components in [-3m, 3m] and on any mesh with S > 3m the cell sum is
+ * S^3 times the sum over closed triangles q1+q2+q3 = 0, the same as on the
+ * N^3 mesh of the reference up to (N/S)^3.  Scale sums by 1/S^3 where the
+ * reference divides by N^3 (bskit/main.py:1875-1882, :2024-2064).  The
+ * default is the smallest supported S > 3m, or N when that is not smaller;
+ * set_shell_mesh(0) restores it, set_shell_mesh(N) gives the reference's own
+ * mesh. */
+int bsk_plan_shell_mesh(const bsk_plan* plan, int* shell_mesh);
+int bsk_plan_set_shell_mesh(bsk_plan* plan, int shell_mesh);
 /* Radix-plan variant (0 or 1) of FFT pass 0..5 (z r2c, y, x, masked x^-1,
  * y^-1, z c2r); chosen by the autotuner. */
 int bsk_plan_set_variant(bsk_plan* plan, int pass, int variant);
@@ -99,7 +112,8 @@ int bsk_forward_workspace(const bsk_plan* plan, int dtype, size_t* bytes);
 int bsk_forward(bsk_plan* plan, int dtype, const void* field, void* dk, void* work, void* stream);
 
 /* Shell fields of bins [shell0, shell0+nshell) on the x-slab
- * [x0, x0+xslab): unnormalised inverse transform of the masked spectrum.
+ * [x0, x0+xslab) of the shell mesh (S^3 cells, 0 <= x0 < S): unnormalised
+ * inverse transform of the masked spectrum.
  * stages: BSK_STAGE_X runs the masked inverse x transform of every
  * retained (ky,kz) line into work; BSK_STAGE_YZ runs the y and z passes of
  * the slab from that work (so one X stage serves many slabs); both = 3.

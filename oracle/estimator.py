@@ -80,6 +80,30 @@ def shells(dk_half, N, lo, hi, kind='delta', kf=1.0):
     return [shell_field(dk_half, n2, lo[b], hi[b], N, kind, kf) for b in range(len(lo))]
 
 
+def shells_on_mesh(dk_half, N, S, lo, hi, kind='delta', kf=1.0):
+    """The same masked spectra as ``shells`` evaluated on an S^3 mesh.
+
+    The modes of the N-mesh half spectrum with signed components
+    representable on S (|n_a| < S/2) are placed on the S-mesh half spectrum
+    and transformed there; modes outside that range must not be in any shell
+    (true whenever S > 2m).  For S > 3m the cell sums of triple products on
+    the S mesh times (N/S)^3 equal those on the N mesh (zero-mode identity).
+    At S == N this is ``shells`` itself.
+    """
+    if S == N:
+        return shells(dk_half, N, lo, hi, kind, kf)
+    f = np.fft.fftfreq(S, 1.0 / S).astype(np.int64)
+    n2 = n2_half(S)
+    for b in range(len(lo)):
+        if hi[b] >= (S // 2) ** 2:
+            raise ValueError('shell %d reaches the Nyquist plane of S=%d' % (b, S))
+    sub = None
+    if dk_half is not None:
+        ix = np.mod(f, N)
+        sub = dk_half[ix][:, ix][:, :, :S // 2 + 1]
+    return [shell_field(sub, n2, lo[b], hi[b], S, kind, kf) for b in range(len(lo))]
+
+
 def triple_sums(fa, fb, fc, tri):
     """sum_x A_i B_j C_k for every (i, j, k) in tri, FP64."""
     out = np.empty(len(tri), dtype=np.float64)

@@ -156,7 +156,7 @@ def test_library_loads_and_host_calls_work():
         lib = L.lib()
     except (ImportError, OSError) as e:
         pytest.skip('library not loadable here: %s' % e)
-    assert lib.bsk_version() == 1
+    assert lib.bsk_version() == L.ABI_VERSION
     lo = (ctypes.c_int * 2)()
     hi = (ctypes.c_int * 2)()
     kf = 2 * np.pi / 100.

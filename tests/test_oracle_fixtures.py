@@ -170,3 +170,34 @@ def test_shell_limits_ties():
     assert hi[0] == 25 and lo[1] == 25
     lo2, hi2 = est.shell_limits([3.0 * (1 + 1e-13)], [5.0 * (1 - 1e-13)], kf)
     assert lo2[0] == 9 and hi2[0] == 25
+
+
+def test_shell_mesh_identity():
+    """Cell sums of shell triples on an S^3 mesh with S > 3m equal the N^3
+    sums times (S/N)^3; at S <= 3m aliasing breaks the equality."""
+    N, box = 48, 700.
+    kf = est.kf_of(box)
+    rng = np.random.default_rng(9)
+    g = rng.standard_normal((N, N, N))
+    field = g + 0.3 * (g * g - 1)
+    e = ob.bin_edges(kf, kf + 5 * kf + kf / 2, kf)       # shells up to |n| = 6, m = 6
+    idx = ob.all_triangles(kf, kf + 5 * kf + kf / 2, kf, return_indices=True)
+    lo, hi = est.shell_limits(e[:, 0], e[:, 1], kf)
+    m = int(np.floor(np.sqrt(hi.max())))
+    assert m == 6
+    dk = est.forward(field)
+    full = est.shells(dk, N, lo, hi)
+    ref = est.triple_sums(full, full, full, idx) / N ** 3
+    assert np.array_equal(np.stack(est.shells_on_mesh(dk, N, N, lo, hi)), np.stack(full))
+    for S in (20, 24, 32):                               # S > 3m = 18
+        sh = est.shells_on_mesh(dk, N, S, lo, hi)
+        got = est.triple_sums(sh, sh, sh, idx) / S ** 3
+        assert np.allclose(got, ref, rtol=1e-10, atol=1e-13 * np.abs(ref).max()), S
+    # counts: unit shells on S reproduce the exact N-mesh counts
+    n_s = est.shells_on_mesh(None, N, 20, lo, hi, 'unit')
+    cnt, _, _ = direct.enumerate_triangles(N, lo, hi, idx)
+    assert np.array_equal(np.rint(est.triple_sums(n_s, n_s, n_s, idx) / 20 ** 3), cnt)
+    # below the bound the sums alias: S = 16 <= 3m differs for some triangle
+    sh = est.shells_on_mesh(dk, N, 16, lo, hi)
+    got = est.triple_sums(sh, sh, sh, idx) / 16 ** 3
+    assert not np.allclose(got, ref, rtol=1e-6)
