@@ -16,6 +16,13 @@ reported separately.  Two paths are timed: split-BF16 MFMA (the value) and
 the reference-structured FP32 sweep per triangle; both are compared with
 the FP64 path for the relative B error.
 
+Latency (the metric's p50/p99): a serving run of seeded fast-path jobs
+(N uniform over {128,192,256,384,512,768}, nb uniform over [8, 40], auto or
+cross <AAB> with probability 1/2) with Poisson arrivals at 80% of the
+capacity measured by a saturated run of the same jobs.  Batched cross: 64
+<AAB> pairs (32 at 128^3, 24 at 256^3, 8 at 384^3, nb in [8, 24]) replayed
+as one HIP graph.
+
 Prints one JSON line.
 """
 import argparse
@@ -133,11 +140,77 @@ def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
     return b, elapsed
 
 
+def run_serving(args, torch, np):
+    """p50/p99 latency of the job mix at args.load x measured capacity."""
+    from bskit_amd import serve
+    t0 = time.perf_counter()
+    pool = serve.make_pool(box=args.box, seed=args.seed + 100)
+    jobs = serve.make_jobs(args.jobs, seed=args.seed, box=args.box, pool=pool)
+    srv = serve.Server(nstreams=4)
+    for j in jobs:
+        srv.prepare(j)
+    log('[serve] %d jobs prepared (%d engines) in %.1fs' % (len(jobs), len(srv.engines),
+                                                            time.perf_counter() - t0))
+    # capacity: every job released at t = 0 (also warms workspaces and caches)
+    for j in jobs:
+        j.arrival = 0.0
+    sat = srv.run(jobs)
+    capacity = len(jobs) / sat['makespan_s']
+    log('[serve] saturated: %.2fs, capacity %.1f jobs/s' % (sat['makespan_s'], capacity))
+    serve.reset(jobs)
+    rate = args.load * capacity
+    serve.poisson_arrivals(jobs, rate, seed=args.seed)
+    res = srv.run(jobs, log=log)
+    st = serve.latency_stats(jobs)
+    gpu_s = sum(2 if j.nfield == 2 else 1 for j in jobs)
+    log('[serve] poisson %.1f jobs/s: p50 %.2f ms p99 %.2f ms' % (rate, st['p50_ms'],
+                                                                 st['p99_ms']))
+    out = dict(st)
+    out.update({'jobs': len(jobs), 'arrival_rate_jobs_per_s': rate,
+                'capacity_jobs_per_s': capacity, 'load': args.load,
+                'makespan_s': res['makespan_s'],
+                'throughput_jobs_per_s': len(jobs) / res['makespan_s'],
+                'peak_footprint_GB': res['peak_footprint_bytes'] / 1e9,
+                'hbm_budget_GB': srv.budget / 1e9, 'streams': len(srv.streams),
+                'fields': gpu_s,
+                'mix': 'N uniform {128,192,256,384,512,768}, nb uniform [8,40], '
+                       'auto/cross 1:1, fast-path B_unnorm per job copied to host; '
+                       'inputs: 2 seeded correlated field pairs per N resident in HBM'})
+    for j in jobs:
+        j.result = None
+    del pool, jobs, srv
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_batch(args, torch, np):
+    """64 cross <AAB> pairs as one HIP graph: pairs/s over several replays."""
+    from bskit_amd import batch
+    pairs = batch.config_pairs(seed=args.seed, box=args.box)
+    cb = batch.CrossBatch(pairs, nbranch=4)
+    cb.capture()
+    cb.replay()
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cb.replay()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / reps
+    res = {'pairs': len(pairs), 'triangles': cb.ntri, 'graph_ms': el * 1e3,
+           'pairs_per_s': len(pairs) / el, 'replays': reps, 'branches': cb.nbranch,
+           'layout': '32 @128^3, 24 @256^3, 8 @384^3, nb uniform [8,24], <AAB>'}
+    log('[batch] %d pairs in %.2f ms per graph replay' % (len(pairs), el * 1e3))
+    del cb, pairs
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=5)
-    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--plain-steps', type=int, default=2)
     ap.add_argument('--nmesh', type=int, default=512)
     ap.add_argument('--nbin', type=int, default=32)
@@ -147,6 +220,10 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline sample')
     ap.add_argument('--no-fp64', action='store_true', help='skip the FP64 accuracy reference')
     ap.add_argument('--no-plain', action='store_true', help='skip the FP32 plain path')
+    ap.add_argument('--no-serve', action='store_true', help='skip the serving (latency) run')
+    ap.add_argument('--no-batch', action='store_true', help='skip the graph-batched cross run')
+    ap.add_argument('--jobs', type=int, default=2000, help='jobs in the serving run')
+    ap.add_argument('--load', type=float, default=0.8, help='arrival rate / measured capacity')
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
@@ -245,6 +322,13 @@ def main():
     stage_gbs = {'shells_GBps': shells_alg / t_sh / 1e9,
                  'forward_GBps': (4.0 * N ** 3 + 8.0 * H) / (stage_ms['forward'] * 1e-3) / 1e9}
 
+    latency = None
+    if not args.no_serve:
+        latency = run_serving(args, torch, np)
+    batched = None
+    if not args.no_batch:
+        batched = run_batch(args, torch, np)
+
     cpu = None
     if not args.no_cpu:
         from oracle import baseline
@@ -287,7 +371,8 @@ def main():
         'stage_GBps': stage_gbs,
         'gridinfo_once_s': t_gridinfo,
         'accuracy_vs_fp64': acc,
-        'latency': None,
+        'latency': latency,
+        'batched_cross': batched,
         'roofline': roof,
         'cpu_baseline': cpu,
     }

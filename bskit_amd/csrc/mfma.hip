@@ -393,8 +393,10 @@ int launch_rt(const MfmaArgs& a, hipStream_t s) {
   constexpr int ACC = acc_of<NB, MODE, RT>();
   using K = MfmaK<NB, MODE, W, RT, kCK, kF, ACC, XS>;
   auto fn = k_triple_mfma<NB, MODE, W, RT, kCK, kF, ACC, XS>;
-  BSK_HIP_CHECK(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)K::LDS_BYTES));
+  // once per instance (not per launch: keeps launches capturable and cheap)
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)K::LDS_BYTES);
+  BSK_HIP_CHECK(attr);
   hipLaunchKernelGGL(fn, dim3(a.nwg), dim3(64 * W), K::LDS_BYTES, s, a);
   BSK_LAUNCH_CHECK();
   return BSK_OK;

@@ -226,13 +226,17 @@ int bsk_triple_tiled_launch(const T* f, long long stride, int nfield, long long 
   }
   const dim3 grid(nchunk, (ntri + kW * TPT - 1) / (kW * TPT));
   if (nsum == 1) {
-    BSK_HIP_CHECK(hipFuncSetAttribute((const void*)k_triple_tiled<T, 1, TPT, CC>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    static const hipError_t attr = hipFuncSetAttribute(
+        (const void*)k_triple_tiled<T, 1, TPT, CC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        160 * 1024);
+    BSK_HIP_CHECK(attr);
     hipLaunchKernelGGL((k_triple_tiled<T, 1, TPT, CC>), grid, dim3(kW), lds, s, f, stride, nfield,
                        ncell, ntri, tri, partial, nchunk);
   } else if (nsum == 4) {
-    BSK_HIP_CHECK(hipFuncSetAttribute((const void*)k_triple_tiled<T, 4, TPT, CC>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    static const hipError_t attr = hipFuncSetAttribute(
+        (const void*)k_triple_tiled<T, 4, TPT, CC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        160 * 1024);
+    BSK_HIP_CHECK(attr);
     hipLaunchKernelGGL((k_triple_tiled<T, 4, TPT, CC>), grid, dim3(kW), lds, s, f, stride, nfield,
                        ncell, ntri, tri, partial, nchunk);
   } else {

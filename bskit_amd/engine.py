@@ -177,7 +177,7 @@ class GridEngine:
         return 2.0 * math.pi / self.box
 
     # --------------------------------------------------------------- forward
-    def forward(self, field, stream=None):
+    def forward(self, field, stream=None, work=None, out=None):
         """delta(k) on the compact cube [MY, MX, MZ] (complex, 1/N^3 scaled)."""
         if field.device.type != 'cuda':
             raise ValueError('field must be a device tensor')
@@ -186,17 +186,22 @@ class GridEngine:
         field = field.contiguous()
         dt = _dt(field.dtype)
         ctype = torch.complex64 if dt == L.DT_F32 else torch.complex128
-        dk = torch.empty((self.MY, self.MX, self.MZ), dtype=ctype, device=self.device)
+        dk = out
+        if dk is None:
+            dk = torch.empty((self.MY, self.MX, self.MZ), dtype=ctype, device=self.device)
+        elif dk.dtype != ctype or tuple(dk.shape) != (self.MY, self.MX, self.MZ):
+            raise ValueError('out must be a %s [%d, %d, %d] tensor'
+                             % (ctype, self.MY, self.MX, self.MZ))
         nbytes = L.size_out()
         L.call('bsk_forward_workspace', self._plan, dt, ctypes.byref(nbytes))
-        w = self.work.get('fwd', nbytes.value)
+        w = (work or self.work).get('fwd', nbytes.value)
         L.call('bsk_forward', self._plan, dt, L.ptr(field), L.ptr(dk), L.ptr(w),
                stream_handle(stream))
         return dk
 
     # ---------------------------------------------------------------- shells
     def shells(self, dk=None, kind=L.KIND_DELTA, fmt=L.FMT_F32, dtype=torch.float32,
-               shell0=0, nshell=None, out=None, stream=None):
+               shell0=0, nshell=None, out=None, stream=None, work=None):
         """All (or a range of) shell fields of one spectrum on the full S^3 mesh."""
         nshell = self.nb - shell0 if nshell is None else nshell
         dt = _dt(dtype)
@@ -208,7 +213,7 @@ class GridEngine:
             out = self.alloc_shells(fmt, self.nb)
         nbytes = L.size_out()
         L.call('bsk_shells_workspace', self._plan, dt, nshell, self.S, ctypes.byref(nbytes))
-        w = self.work.get('shell', nbytes.value)
+        w = (work or self.work).get('shell', nbytes.value)
         L.call('bsk_shells', self._plan, dt, kind, L.ptr(dk), shell0, nshell, 0, self.S,
                L.STAGE_X | L.STAGE_YZ, fmt, L.ptr(out.data), L.ptr(out.lo), self.ncell, 0,
                L.ptr(w), stream_handle(stream))
@@ -271,7 +276,8 @@ class GridEngine:
         ncell = self.ncell if ncell is None else ncell
         return int(max(1, min(4096, ncell // 65536)))
 
-    def triples(self, sets, tri, path=PATH_MFMA, scale=1.0, out=None, stream=None, nwg=None):
+    def triples(self, sets, tri, path=PATH_MFMA, scale=1.0, out=None, stream=None, nwg=None,
+                work=None):
         """scale * sum_x A_i B_j C_k for every (i, j, k) row of tri.
 
         sets: [A] (auto), [A, B] (<AAB>: sides A, A, B) or [A, B, C].
@@ -305,7 +311,7 @@ class GridEngine:
             layout, slots, tiles = self.mfma_layout(tri, sa.nb, mode)
             nbytes = L.size_out()
             L.call('bsk_triples_mfma_workspace', tiles, nwg, ctypes.byref(nbytes))
-            w = self.work.get('mfma', nbytes.value)
+            w = (work or self.work).get('mfma', nbytes.value)
             hi_i, lo_i = sa.data, sa.lo
             hi_j, lo_j = sb.data, sb.lo
             hi_k, lo_k = sc.data, sc.lo
@@ -321,7 +327,7 @@ class GridEngine:
             tri_d = self.triangles_device(tri)
             nbytes = L.size_out()
             L.call('bsk_triples_plain_workspace', min(T, 65535), nchunk, ctypes.byref(nbytes))
-            w = self.work.get('plain', nbytes.value)
+            w = (work or self.work).get('plain', nbytes.value)
             for t0 in range(0, T, 65535):
                 t1 = min(T, t0 + 65535)
                 L.call('bsk_triples_plain', dt, L.ptr(sa.data), L.ptr(sb.data), L.ptr(sc.data),
@@ -337,7 +343,7 @@ class GridEngine:
             tri_d = self.triangles_device(tri)
             nbytes = L.size_out()
             L.call('bsk_triples_tiled_workspace', T, 1, nchunk, ctypes.byref(nbytes))
-            w = self.work.get('tiled', nbytes.value)
+            w = (work or self.work).get('tiled', nbytes.value)
             L.call('bsk_triples_tiled', dt, L.ptr(sa.data), ncell, sa.nb, ncell, T, L.ptr(tri_d), 1,
                    nchunk, float(scale), L.ptr(out), L.ptr(w), s)
             return out

@@ -1,0 +1,259 @@
+"""Request loop for fast-path bispectrum jobs on one GPU.
+
+A job is one fast-path measurement (bskit/main.py:1784-1938 with
+main.py:1875-1882): forward transform of one field (auto) or two fields
+(cross <AAB>, the roles of main.py:624-640), every shell of its bin list
+in split-BF16, and the unnormalised B of every closed triangle, copied back
+to host memory.  The reference measures one grid per MPI job; here many
+jobs share the GPU:
+
+* admission: a job starts only when its HBM footprint (shells, spectra,
+  workspaces) fits in the budget left by the jobs in flight (default
+  min(256 GB, 85% of free HBM));
+* continuous batching: admitted jobs are issued at once onto the stream
+  with the fewest jobs in flight (default-priority streams), so small and
+  large grids overlap on the device instead of waiting for a batch to drain;
+* per-key caches: one GridEngine (plan, thresholds, MFMA layouts) per
+  (Nmesh, BoxSize, bins) key, created before the timed run by ``prepare``.
+
+Every reduction in the path runs in a fixed order, so a job's result does
+not depend on which other jobs share the device.
+"""
+import collections
+import ctypes
+import dataclasses
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import bins as B
+from .engine import PATH_MFMA, PATH_PLAIN, GridEngine, Workspace, require_device
+
+MESH_SIZES = (128, 192, 256, 384, 512, 768)
+
+
+@dataclasses.dataclass
+class Job:
+    jid: int
+    nmesh: int
+    box: float
+    kLU: np.ndarray
+    nfield: int                  # 1: auto <AAA>; 2: cross <AAB>
+    fields: list                 # device float32 N^3 tensors
+    arrival: float = 0.0         # seconds after the start of the run
+    # filled by the server
+    tri: np.ndarray = None
+    path: str = None
+    footprint: int = 0
+    result: torch.Tensor = None  # pinned host float64 [T]
+    issued: float = None
+    done: float = None
+    stream: int = None
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+def _bins_key(kLU):
+    return tuple((float(a), float(b)) for a, b in np.asarray(kLU, dtype=np.float64))
+
+
+class Server:
+    """Admission + continuous batching of Jobs on one device."""
+
+    def __init__(self, device=None, hbm_budget=None, nstreams=4, poll_s=2e-5):
+        self.device = require_device(device)
+        free, _ = torch.cuda.mem_get_info(self.device)
+        self.budget = int(hbm_budget) if hbm_budget else int(min(256e9, 0.85 * free))
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(nstreams)]
+        self.works = [Workspace(self.device) for _ in range(nstreams)]
+        self.poll_s = poll_s
+        self.engines = {}
+        self._tri = {}
+
+    # ------------------------------------------------------------ host side
+    def engine(self, nmesh, box, kLU):
+        key = (int(nmesh), float(box), _bins_key(kLU))
+        eng = self.engines.get(key)
+        if eng is None:
+            kLU = np.asarray(kLU, dtype=np.float64)
+            eng = GridEngine(nmesh, box, kLU[:, 0], kLU[:, 1], device=self.device)
+            self.engines[key] = eng
+        return eng
+
+    def footprint(self, eng, nfield):
+        """HBM bytes a job holds while in flight."""
+        nb = eng.nb
+        sz = L.size_out()
+        L.call('bsk_forward_workspace', eng._plan, L.DT_F32, ctypes.byref(sz))
+        fwd = sz.value
+        L.call('bsk_shells_workspace', eng._plan, L.DT_F32, nb, eng.S, ctypes.byref(sz))
+        shw = sz.value
+        shells = nfield * nb * eng.ncell * 4
+        spectra = nfield * eng.MX * eng.MY * eng.MZ * 8
+        return int(fwd + shw + shells + spectra)
+
+    def prepare(self, job):
+        """Engine, triangles, path and footprint of a job (host work, untimed)."""
+        eng = self.engine(job.nmesh, job.box, job.kLU)
+        key = (_bins_key(job.kLU), job.nfield)
+        tri = self._tri.get(key)
+        if tri is None:
+            tri = B.closed_triangle_indices(np.asarray(job.kLU), job.nfield).astype(np.int32)
+            self._tri[key] = tri
+        job.tri = tri
+        job.path = PATH_MFMA if GridEngine.mfma_supported(eng.nb, job.nfield, tri) else PATH_PLAIN
+        if job.path == PATH_MFMA and len(tri):
+            mode = {1: L.MODE_AUTO, 2: L.MODE_AAB}[job.nfield]
+            eng.mfma_layout(tri, eng.nb, mode)
+        eng.triangles_device(tri)
+        job.footprint = self.footprint(eng, job.nfield)
+        if job.footprint > self.budget:
+            raise MemoryError('job %d needs %.1f GB, above the %.1f GB budget'
+                              % (job.jid, job.footprint / 1e9, self.budget / 1e9))
+        return job
+
+    # ---------------------------------------------------------- device side
+    def _issue(self, job, si):
+        """Enqueue the whole job on stream si; returns its completion event."""
+        eng = self.engine(job.nmesh, job.box, job.kLU)
+        s = self.streams[si]
+        w = self.works[si]
+        fmt = L.FMT_BF16X2 if job.path == PATH_MFMA else L.FMT_F32
+        with torch.cuda.stream(s):
+            sets = []
+            for f in job.fields[:job.nfield]:
+                dk = eng.forward(f, work=w)
+                sets.append(eng.shells(dk, fmt=fmt, out=eng.alloc_shells(fmt, eng.nb), work=w))
+            out = eng.triples(sets, job.tri, path=job.path, scale=job.box ** 6 / float(eng.ncell),
+                              work=w)
+            host = torch.empty(len(job.tri), dtype=torch.float64, pin_memory=True)
+            host.copy_(out, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(s)
+        job.result = host
+        job.stream = si
+        return ev
+
+    def run_one(self, job):
+        """Run a single job alone and wait for it (reference result)."""
+        if job.tri is None:
+            self.prepare(job)
+        ev = self._issue(job, 0)
+        ev.synchronize()
+        return job.result.clone()
+
+    def run(self, jobs, log=None):
+        """Serve jobs by arrival time; fills issued/done (seconds from start)."""
+        for j in jobs:
+            if j.tri is None:
+                self.prepare(j)
+        pending = collections.deque(sorted(jobs, key=lambda j: (j.arrival, j.jid)))
+        inflight = []
+        per_stream = [0] * len(self.streams)
+        used = 0
+        peak = 0
+        t0 = time.perf_counter()
+        nextlog = 1.0
+        while pending or inflight:
+            now = time.perf_counter() - t0
+            keep = []
+            for j, ev in inflight:
+                if ev.query():
+                    j.done = now
+                    used -= j.footprint
+                    per_stream[j.stream] -= 1
+                else:
+                    keep.append((j, ev))
+            inflight = keep
+            while pending and pending[0].arrival <= now:
+                j = pending[0]
+                if inflight and used + j.footprint > self.budget:
+                    break
+                pending.popleft()
+                si = int(np.argmin(per_stream))
+                j.issued = time.perf_counter() - t0
+                ev = self._issue(j, si)
+                per_stream[si] += 1
+                used += j.footprint
+                peak = max(peak, used)
+                inflight.append((j, ev))
+            if log is not None and now >= nextlog:
+                ndone = sum(1 for j in jobs if j.done is not None)
+                log('[serve] t=%.1fs done %d/%d in flight %d' % (now, ndone, len(jobs),
+                                                               len(inflight)))
+                nextlog += 1.0
+            if pending and not inflight:
+                wait = pending[0].arrival - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 1e-3))
+            else:
+                time.sleep(self.poll_s)
+        makespan = time.perf_counter() - t0
+        return {'makespan_s': makespan, 'peak_footprint_bytes': peak}
+
+
+# ---------------------------------------------------------------- workloads
+def serving_bins(nmesh, nbin, kf):
+    """nbin bins from k_f: width 2 k_f while the last edge stays below 0.45 N k_f,
+    else width k_f (edges by generate_bin_edge_list, quirk-aware kmax)."""
+    w = 2 if (2 * nbin + 1) <= 0.45 * nmesh else 1
+    kLU = B.generate_bin_edge_list(kf, kf + nbin * w * kf + w * kf / 2.0, w * kf)
+    return kLU[:nbin]
+
+
+def make_jobs(njobs, seed=0, box=1000.0, sizes=MESH_SIZES, nbin_range=(8, 40), pool=None,
+              device=None):
+    """Seeded job mix: N uniform over sizes, nb uniform over nbin_range, auto or
+    cross with probability 1/2.  pool[N] = list of (A, B) device field pairs;
+    job j uses pair j mod len(pool[N])."""
+    rng = np.random.default_rng(seed)
+    kf = 2 * math.pi / box
+    jobs = []
+    for jid in range(njobs):
+        n = int(rng.choice(sizes))
+        nb = int(rng.integers(nbin_range[0], nbin_range[1] + 1))
+        nf = 1 if rng.random() < 0.5 else 2
+        pair = pool[n][jid % len(pool[n])]
+        jobs.append(Job(jid=jid, nmesh=n, box=box, kLU=serving_bins(n, nb, kf), nfield=nf,
+                        fields=list(pair)))
+    return jobs
+
+
+def make_pool(sizes=MESH_SIZES, npair=2, box=1000.0, seed=100, device='cuda'):
+    """Correlated local-quadratic field pairs per mesh size, resident in HBM."""
+    from . import synth
+    pool = {}
+    for i, n in enumerate(sizes):
+        pool[n] = [synth.correlated_pair_device(n, box, seed + 1000 * i + 2 * p,
+                                                seed + 1000 * i + 2 * p + 1, device)
+                   for p in range(npair)]
+    return pool
+
+
+def poisson_arrivals(jobs, rate, seed=0):
+    """Exponential inter-arrival times at `rate` jobs/s."""
+    rng = np.random.default_rng(seed + 7)
+    t = 0.0
+    for j in jobs:
+        t += rng.exponential(1.0 / rate)
+        j.arrival = t
+    return jobs
+
+
+def reset(jobs):
+    for j in jobs:
+        j.result = None
+        j.issued = None
+        j.done = None
+        j.stream = None
+
+
+def latency_stats(jobs):
+    lat = np.array([j.latency for j in jobs], dtype=np.float64) * 1e3
+    return {'p50_ms': float(np.percentile(lat, 50)), 'p99_ms': float(np.percentile(lat, 99)),
+            'mean_ms': float(lat.mean()), 'max_ms': float(lat.max())}

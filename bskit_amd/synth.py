@@ -133,3 +133,56 @@ def cic_paint_numpy(pos, N, box):
              (f[:, 2] if c & 1 else 1 - f[:, 2]))
         grid += np.bincount((ix * N + iy) * N + iz, weights=w, minlength=N ** 3)
     return grid.reshape(N, N, N)
+
+
+# ---------------------------------------------------------------------------
+# Device generators (serving and batch workloads).  Same power spectrum and
+# normalisation as above, but the white noise comes from torch's seeded
+# device generator (Philox), so the fields differ from the PCG64 ones.  The
+# transforms here prepare inputs and are not part of the measured path.
+# ---------------------------------------------------------------------------
+def _device_kmag(N, box, device):
+    import torch
+    kf = 2.0 * np.pi / box
+    f = torch.fft.fftfreq(N, 1.0 / N, device=device, dtype=torch.float64)
+    fz = torch.arange(N // 2 + 1, device=device, dtype=torch.float64)
+    return kf, f, fz
+
+
+def gaussian_field_device(N, box, seed, device='cuda', amp=P_AMP, k0=P_K0, index=P_INDEX):
+    """Seeded Gaussian field with P(k) = amp (k/k0)^index, float32 on device."""
+    import torch
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    w = torch.randn((N, N, N), generator=gen, device=device, dtype=torch.float32)
+    wk = torch.fft.rfftn(w)
+    del w
+    kf, f, fz = _device_kmag(N, box, device)
+    vol = float(box) ** 3
+    n3 = float(N) ** 3
+    for ix in range(N):
+        k = kf * torch.sqrt(f[ix] ** 2 + f[:, None] ** 2 + fz[None, :] ** 2)
+        p = torch.where(k > 0, amp * (k.clamp_min(1e-30) / k0) ** index, torch.zeros_like(k))
+        # r2c / N^3 * sqrt(P N^3 / V), then the unnormalised c2r (irfftn * N^3)
+        wk[ix] *= torch.sqrt(p * n3 / vol).to(torch.float32)
+    return torch.fft.irfftn(wk, s=(N, N, N))
+
+
+def local_quadratic_device(d, f_nl=0.5):
+    """d + f_nl (d^2 - <d^2>), float32 on device."""
+    d2 = d * d
+    return d + f_nl * (d2 - d2.mean())
+
+
+def quadratic_field_device(N, box, seed, device='cuda', f_nl=0.5):
+    return local_quadratic_device(gaussian_field_device(N, box, seed, device), f_nl)
+
+
+def correlated_pair_device(N, box, seed_a, seed_c, device='cuda', rho=0.8, f_nl=0.5):
+    """(A, B): A = Q(G_A), B = Q(rho G_A + sqrt(1 - rho^2) G_C) with Q the local
+    quadratic transform; rho = 0.8 gives weights 0.8 and 0.6."""
+    ga = gaussian_field_device(N, box, seed_a, device)
+    gc = gaussian_field_device(N, box, seed_c, device)
+    gb = rho * ga + float(np.sqrt(1.0 - rho * rho)) * gc
+    del gc
+    return local_quadratic_device(ga, f_nl), local_quadratic_device(gb, f_nl)
