@@ -20,8 +20,7 @@ FMT_F32, FMT_BF16X2, FMT_F64 = 0, 1, 2
 KIND_DELTA, KIND_UNIT, KIND_KMAG = 0, 1, 2
 MODE_AUTO, MODE_AAB, MODE_ABC = 0, 1, 2
 STAGE_X, STAGE_YZ = 1, 2
-MFMA_LAYOUT_INTS = 26
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _c_int, _c_ll, _c_double, _vp = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 _c_size_p = ctypes.POINTER(ctypes.c_size_t)
@@ -52,10 +51,15 @@ _SIGS = {
     'bsk_triples_tiled_workspace': [_c_int, _c_int, _c_int, _c_size_p],
     'bsk_triples_tiled': [_c_int, _vp, _c_ll, _c_int, _c_ll, _c_int, _vp, _c_int, _c_int, _c_double,
                           _vp, _vp, _vp],
-    'bsk_mfma_layout': [_c_int, _c_int, _c_int, _int_p, _int_p, _int_p, _int_p],
-    'bsk_triples_mfma_workspace': [_c_int, _c_int, _c_size_p],
-    'bsk_triples_mfma': [_c_int, _c_int, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _c_int, _int_p,
-                         _c_int, _vp, _c_double, _vp, _vp, _vp],
+    'bsk_pairs_create': [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _int_p],
+    'bsk_pairs_upload': [_vp],
+    'bsk_pairs_destroy': [_vp],
+    'bsk_pairs_info': [_vp, _int_p, _int_p, _int_p, _int_p, _int_p],
+    'bsk_pairs_slots': [_vp, _int_p],
+    'bsk_pairs_tables': [_vp, _int_p, _int_p],
+    'bsk_triples_mfma_workspace': [_vp, _c_int, _c_size_p],
+    'bsk_triples_mfma': [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _c_int, _c_double, _vp,
+                         _vp, _vp],
     'bsk_cic_paint': [_c_int, _c_double, _c_ll, _vp, _vp, _c_int, _vp],
     'bsk_direct_triangles': [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp,
                              _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp],

@@ -475,157 +475,346 @@ int bsk_triples_tiled(int dtype, const void* fields, long long shell_stride, int
   return bsk_reduce_tiled_launch(part, ntri, nsum, nchunk, scale, out_dev, s);
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------ pair plans
+struct bsk_pairs {
+  int mode = 0;
+  int nshell = 0;
+  int sets = 1;
+  int ntri = 0;
+  int npair = 0, ntile = 0, nitem = 0;
+  int nkb = 0;
+  int ni = 1;          // items per wave and launch (compiled count)
+  int npass = 1;
+  int item_total = 0;  // npass * waves * ni
+  std::vector<int> slot;
+  std::vector<int> h_items, h_pairs;
+  int* d_items = nullptr;  // [npass][waves * ni]
+  int* d_pairs = nullptr;  // [ntile + 1][16]
+  int* d_slot = nullptr;   // [ntri]
+};
+
 namespace {
 
-// Tile index of (i, jb, kb) in a workgroup's block tiles (see mfma.hip).
-int block_tile(int nblk, int mode, int i, int jb, int kb) {
-  const int ib = i / 16, ii = i % 16;
-  int before = 0;
-  for (int b = 0; b < ib; ++b)
-    before += mode == 0 ? (b + 1) * (b + 2) / 2 : (mode == 1 ? (b + 1) * nblk : nblk * nblk);
-  const int nti = mode == 0 ? (ib + 1) * (ib + 2) / 2 : (mode == 1 ? (ib + 1) * nblk : nblk * nblk);
-  const int tin = mode == 0 ? jb * (jb + 1) / 2 + kb : jb * nblk + kb;
-  return 16 * before + ii * nti + tin;
+struct Role {
+  int a, b, c;  // pair (a in set 0, b in set 1), third side c in set 2
+};
+
+struct PairTiles {
+  std::vector<std::pair<long long, unsigned>> pairs;  // (a << 32 | b, kb mask) in tile order
+  std::vector<int> tile_of_pair_start;                // first pair index of each tile
+  std::vector<unsigned> tile_mask;
+  int nitem = 0;
+};
+
+long long pkey(int a, int b) { return ((long long)a << 32) | (unsigned)b; }
+
+// Distinct pairs grouped by the set of third-side blocks they need, packed
+// into 16-row tiles (pairs of one group sorted by (a, b)).
+PairTiles build_tiles(const std::vector<Role>& roles) {
+  std::vector<std::pair<long long, unsigned>> pm;
+  pm.reserve(roles.size());
+  for (const Role& r : roles) pm.push_back({pkey(r.a, r.b), 1u << (r.c / 16)});
+  std::sort(pm.begin(), pm.end());
+  std::vector<std::pair<long long, unsigned>> uniq;
+  for (const auto& e : pm) {
+    if (!uniq.empty() && uniq.back().first == e.first)
+      uniq.back().second |= e.second;
+    else
+      uniq.push_back(e);
+  }
+  std::stable_sort(uniq.begin(), uniq.end(),
+                   [](const std::pair<long long, unsigned>& x,
+                      const std::pair<long long, unsigned>& y) { return x.second < y.second; });
+  PairTiles t;
+  t.pairs = uniq;
+  for (size_t i = 0; i < uniq.size();) {
+    size_t j = i;
+    while (j < uniq.size() && uniq[j].second == uniq[i].second && j - i < 16) ++j;
+    t.tile_of_pair_start.push_back((int)i);
+    t.tile_mask.push_back(uniq[i].second);
+    t.nitem += __builtin_popcount(uniq[i].second);
+    i = j;
+  }
+  return t;
 }
 
-// Canonical side order of a triangle for the mode's symmetry.
-void canonical(int mode, int& i, int& j, int& k) {
-  if (mode == 0) {
-    int v[3] = {i, j, k};
-    std::sort(v, v + 3);
-    i = v[2]; j = v[1]; k = v[0];
-  } else if (mode == 1 && j > i) {
-    std::swap(i, j);
+std::vector<Role> assign_roles(int mode, int option, int ntri, const int* tri) {
+  std::vector<Role> r(ntri);
+  for (int t = 0; t < ntri; ++t) {
+    int i = tri[3 * t], j = tri[3 * t + 1], k = tri[3 * t + 2];
+    if (mode == BSK_MODE_AUTO) {
+      int v[3] = {i, j, k};
+      std::sort(v, v + 3);  // v[0] <= v[1] <= v[2]
+      if (option == 0) r[t] = {v[2], v[1], v[0]};       // pair = two largest
+      else if (option == 1) r[t] = {v[1], v[0], v[2]};  // pair = two smallest
+      else r[t] = {v[2], v[0], v[1]};                   // pair = largest and smallest
+    } else if (mode == BSK_MODE_AAB) {
+      r[t] = {std::max(i, j), std::min(i, j), k};
+    } else {
+      r[t] = {i, j, k};
+    }
   }
+  return r;
+}
+
+int upload(int** dst, const std::vector<int>& v) {
+  BSK_HIP_CHECK(hipMalloc(dst, sizeof(int) * std::max<size_t>(v.size(), 1)));
+  if (!v.empty())
+    BSK_HIP_CHECK(hipMemcpy(*dst, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
+  return BSK_OK;
 }
 
 }  // namespace
 
-int bsk_mfma_layout(int nshell, int mode, int ntri, const int* tri, int* slot, int* layout,
-                    int* tiles_per_wg) {
-  if (nshell < 1 || mode < 0 || mode > 2 || ntri < 0 || (ntri && (!tri || !slot)) || !layout ||
-      !tiles_per_wg) {
-    bsk_set_error("bsk_mfma_layout: invalid argument");
+extern "C" {
+
+int bsk_pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri) {
+  if (!out || nshell < 1 || mode < 0 || mode > 2 || ntri < 0 || (ntri && !tri)) {
+    bsk_set_error("bsk_pairs_create: invalid argument");
     return BSK_EINVAL;
   }
+  *out = nullptr;
   for (int t = 0; t < ntri; ++t)
     for (int e = 0; e < 3; ++e)
       if (tri[3 * t + e] < 0 || tri[3 * t + e] >= nshell) {
-        bsk_set_error("bsk_mfma_layout: triangle %d has a shell index out of range", t);
+        bsk_set_error("bsk_pairs_create: triangle %d has a shell index out of range", t);
         return BSK_EINVAL;
       }
-  for (int e = 0; e < BSK_MFMA_LAYOUT_INTS; ++e) layout[e] = 0;
-  // auto spectra whose last few shells only appear as the largest side:
-  // full blocks for the rest plus a handful of extra tiles
-  int nblk = (nshell + 15) / 16;
-  int nextra = 0;
-  int xt[kMfmaMaxExtra][3];
-  if (mode == 0 && nshell > 16 && nshell % 16 != 0 && nshell / 16 <= 2) {
-    const int nb = nshell / 16, lim = 16 * nb;
-    bool ok = true;
-    for (int t = 0; t < ntri && ok; ++t) {
-      int i = tri[3 * t], j = tri[3 * t + 1], k = tri[3 * t + 2];
-      canonical(mode, i, j, k);
-      if (i < lim) continue;
-      if (j >= lim) { ok = false; break; }
-      int f = -1;
-      for (int x = 0; x < nextra; ++x)
-        if (xt[x][0] == i && xt[x][1] == j / 16 && xt[x][2] == k / 16) f = x;
-      if (f < 0) {
-        if (nextra == kMfmaMaxExtra) { ok = false; break; }
-        xt[nextra][0] = i; xt[nextra][1] = j / 16; xt[nextra][2] = k / 16;
-        ++nextra;
-      }
-    }
-    if (ok) {
-      nblk = nb;
-    } else {
-      nextra = 0;
-    }
-  }
-  if (!nextra && !bsk_mfma_supported(nblk, mode)) {
-    bsk_set_error("mfma: %d shells in mode %d exceed the kernel (48 shells, 32 for <ABC>)",
-                  nshell, mode);
+  const int sets = mode == BSK_MODE_AUTO ? 1 : (mode == BSK_MODE_AAB ? 2 : 3);
+  const int nkb = (nshell + 15) / 16;
+  int nsh[3] = {nshell, nshell, nshell};
+  if (!bsk_pair_prefetch(nshell) || bsk_pair_lds_bytes(sets, nsh, nkb) > 160 * 1024 ||
+      nkb > 16) {
+    bsk_set_error("bsk_pairs_create: %d shells exceed the MFMA kernel's LDS staging "
+                  "(128 shells, 96 for <ABC>)", nshell);
     return BSK_EUNSUPPORTED;
   }
-  const int tiles = bsk_mfma_tiles(nblk, mode);
-  for (int t = 0; t < ntri; ++t) {
-    int i = tri[3 * t], j = tri[3 * t + 1], k = tri[3 * t + 2];
-    canonical(mode, i, j, k);
-    int g;
-    if (i >= 16 * nblk) {
-      g = -1;
-      for (int x = 0; x < nextra; ++x)
-        if (xt[x][0] == i && xt[x][1] == j / 16 && xt[x][2] == k / 16) g = tiles + x;
-    } else {
-      g = block_tile(nblk, mode, i, j / 16, k / 16);
+  // role assignment: the fixed roles of <AAB>/<ABC>; for auto spectra the
+  // choice of the pair sides that needs the fewest items
+  std::vector<Role> roles;
+  PairTiles tiles;
+  const int nopt = mode == BSK_MODE_AUTO ? 3 : 1;
+  for (int o = 0; o < nopt; ++o) {
+    std::vector<Role> r = assign_roles(mode, o, ntri, tri);
+    PairTiles t = build_tiles(r);
+    if (o == 0 || t.nitem < tiles.nitem) {
+      roles.swap(r);
+      tiles = t;
     }
-    slot[t] = g * 256 + (j % 16) * 16 + (k % 16);
   }
-  layout[0] = nblk;
-  layout[1] = nextra;
-  for (int x = 0; x < nextra; ++x)
-    for (int e = 0; e < 3; ++e) layout[2 + 3 * x + e] = xt[x][e];
-  *tiles_per_wg = tiles + nextra;
+  const int W = kPairWaves, cap = kPairMaxItems;
+  const int ntile = (int)tiles.tile_mask.size();
+  if (ntile >= 0xffff) {
+    bsk_set_error("bsk_pairs_create: %d pair tiles exceed 65534; split the triangle list",
+                  ntile);
+    return BSK_EUNSUPPORTED;
+  }
+  // tiles to (pass, wave) bins, longest first, each to the least loaded bin
+  std::vector<int> order(ntile);
+  for (int i = 0; i < ntile; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return __builtin_popcount(tiles.tile_mask[x]) > __builtin_popcount(tiles.tile_mask[y]);
+  });
+  int npass = std::max(1, (tiles.nitem + W * cap - 1) / (W * cap));
+  std::vector<std::vector<int>> bins;
+  for (;; ++npass) {
+    bins.assign((size_t)npass * W, {});
+    std::vector<int> load((size_t)npass * W, 0);
+    bool ok = true;
+    for (int ti : order) {
+      const int n = __builtin_popcount(tiles.tile_mask[ti]);
+      const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      if (load[b] + n > cap) {
+        ok = false;
+        break;
+      }
+      load[b] += n;
+      bins[b].push_back(ti);
+    }
+    if (ok) break;
+  }
+  int maxload = 1;
+  for (const auto& b : bins) {
+    int l = 0;
+    for (int ti : b) l += __builtin_popcount(tiles.tile_mask[ti]);
+    maxload = std::max(maxload, l);
+  }
+  const int ni = bsk_pair_items_round(maxload);
+  // item descriptors: tile | kb << 16 | (first item of a tile) << 31
+  std::vector<int> items((size_t)npass * W * ni);
+  std::vector<int> item_of((size_t)ntile * nkb, -1);  // global item of (tile, kb)
+  for (size_t b = 0; b < bins.size(); ++b) {
+    int s = 0;
+    for (int ti : bins[b]) {
+      bool first = true;
+      for (int kb = 0; kb < nkb; ++kb) {
+        if (!(tiles.tile_mask[ti] >> kb & 1u)) continue;
+        const int g = (int)b * ni + s;
+        items[g] = ti | (kb << 16) | (first ? (int)0x80000000u : 0);
+        item_of[(size_t)ti * nkb + kb] = g;
+        first = false;
+        ++s;
+      }
+    }
+    for (; s < ni; ++s)  // padding: the zero tile against block 0
+      items[(int)b * ni + s] = ntile | (s == 0 ? (int)0x80000000u : 0);
+  }
+  // pair rows (padding rows point at the zero row of each set)
+  std::vector<int> prs((size_t)(ntile + 1) * 16, nshell | (nshell << 16));
+  std::vector<std::pair<int, int>> where(tiles.pairs.size());  // (tile, row)
+  for (int ti = 0; ti < ntile; ++ti) {
+    const int p0 = tiles.tile_of_pair_start[ti];
+    const int p1 = ti + 1 < ntile ? tiles.tile_of_pair_start[ti + 1] : (int)tiles.pairs.size();
+    for (int p = p0; p < p1; ++p) {
+      const long long k = tiles.pairs[p].first;
+      prs[(size_t)ti * 16 + (p - p0)] = (int)(k >> 32) | ((int)(k & 0xffffffff) << 16);
+      where[p] = {ti, p - p0};
+    }
+  }
+  // triangle slots
+  std::vector<long long> keys(tiles.pairs.size());
+  for (size_t p = 0; p < keys.size(); ++p) keys[p] = tiles.pairs[p].first;
+  std::vector<size_t> kidx(keys.size());
+  for (size_t p = 0; p < kidx.size(); ++p) kidx[p] = p;
+  std::sort(kidx.begin(), kidx.end(), [&](size_t x, size_t y) { return keys[x] < keys[y]; });
+  bsk_pairs* q = new bsk_pairs();
+  q->mode = mode;
+  q->nshell = nshell;
+  q->sets = sets;
+  q->ntri = ntri;
+  q->npair = (int)tiles.pairs.size();
+  q->ntile = ntile;
+  q->nitem = tiles.nitem;
+  q->nkb = nkb;
+  q->ni = ni;
+  q->npass = npass;
+  q->item_total = npass * W * ni;
+  q->slot.resize(ntri);
+  for (int t = 0; t < ntri; ++t) {
+    const long long k = pkey(roles[t].a, roles[t].b);
+    auto it = std::lower_bound(kidx.begin(), kidx.end(), k,
+                               [&](size_t x, long long v) { return keys[x] < v; });
+    const int p = (int)*it;
+    const int ti = where[p].first, row = where[p].second, c = roles[t].c;
+    const int g = item_of[(size_t)ti * nkb + c / 16];
+    q->slot[t] = g * 256 + row * 16 + (c % 16);
+  }
+  q->h_items.swap(items);
+  q->h_pairs.swap(prs);
+  *out = q;
   return BSK_OK;
 }
 
-int bsk_triples_mfma_workspace(int tiles_per_wg, int nwg, size_t* bytes) {
-  if (tiles_per_wg <= 0 || nwg <= 0 || !bytes) {
+int bsk_pairs_upload(bsk_pairs* q) {
+  if (!q) {
+    bsk_set_error("bsk_pairs_upload: null plan");
+    return BSK_EINVAL;
+  }
+  if (q->d_items) return BSK_OK;
+  int rc = upload(&q->d_items, q->h_items);
+  if (!rc) rc = upload(&q->d_pairs, q->h_pairs);
+  if (!rc) rc = upload(&q->d_slot, q->slot);
+  if (rc) {
+    (void)hipFree(q->d_items);
+    (void)hipFree(q->d_pairs);
+    (void)hipFree(q->d_slot);
+    q->d_items = q->d_pairs = q->d_slot = nullptr;
+  }
+  return rc;
+}
+
+int bsk_pairs_destroy(bsk_pairs* q) {
+  if (!q) return BSK_OK;
+  (void)hipFree(q->d_items);
+  (void)hipFree(q->d_pairs);
+  (void)hipFree(q->d_slot);
+  delete q;
+  return BSK_OK;
+}
+
+int bsk_pairs_info(const bsk_pairs* q, int* npair, int* ntile, int* nitem, int* items_per_wave,
+                   int* npass) {
+  if (!q) {
+    bsk_set_error("bsk_pairs_info: null plan");
+    return BSK_EINVAL;
+  }
+  if (npair) *npair = q->npair;
+  if (ntile) *ntile = q->ntile;
+  if (nitem) *nitem = q->nitem;
+  if (items_per_wave) *items_per_wave = q->ni;
+  if (npass) *npass = q->npass;
+  return BSK_OK;
+}
+
+int bsk_pairs_slots(const bsk_pairs* q, int* slot_host) {
+  if (!q || (q->ntri && !slot_host)) {
+    bsk_set_error("bsk_pairs_slots: invalid argument");
+    return BSK_EINVAL;
+  }
+  for (int t = 0; t < q->ntri; ++t) slot_host[t] = q->slot[t];
+  return BSK_OK;
+}
+
+int bsk_pairs_tables(const bsk_pairs* q, int* items, int* pairs) {
+  if (!q) {
+    bsk_set_error("bsk_pairs_tables: null plan");
+    return BSK_EINVAL;
+  }
+  if (items) std::copy(q->h_items.begin(), q->h_items.end(), items);
+  if (pairs) std::copy(q->h_pairs.begin(), q->h_pairs.end(), pairs);
+  return BSK_OK;
+}
+
+int bsk_triples_mfma_workspace(const bsk_pairs* q, int nwg, size_t* bytes) {
+  if (!q || nwg <= 0 || !bytes) {
     bsk_set_error("bsk_triples_mfma_workspace: invalid argument");
     return BSK_EINVAL;
   }
-  *bytes = sizeof(float) * 256 * (size_t)tiles_per_wg * nwg;
+  *bytes = sizeof(float) * 256 * (size_t)q->item_total * nwg;
   return BSK_OK;
 }
 
-int bsk_triples_mfma(int mode, int nshell, const uint16_t* hi_i, const uint16_t* lo_i,
+int bsk_triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
                      const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
                      const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
-                     const int* layout, int ntri, const int* slot_dev, double scale,
-                     double* out_dev, void* work, void* stream) {
-  if (!layout || !hi_j || !lo_j || !slot_dev || !out_dev || !work || nwg <= 0 ||
-      (mode >= 1 && (!hi_k || !lo_k)) || (mode == 2 && (!hi_i || !lo_i)) ||
-      shell_stride < ncell || mode < 0 || mode > 2) {
+                     double scale, double* out_dev, void* work, void* stream) {
+  if (!q || !hi_j || !lo_j || !out_dev || !work || nwg <= 0 || shell_stride < ncell ||
+      (q->mode >= 1 && (!hi_k || !lo_k)) || (q->mode == 2 && (!hi_i || !lo_i))) {
     bsk_set_error("bsk_triples_mfma: invalid argument");
     return BSK_EINVAL;
   }
-  const int nblk = layout[0], nextra = layout[1];
-  if (nblk < 1 || nextra < 0 || nextra > kMfmaMaxExtra || (nextra && mode != 0) ||
-      16 * nblk + (nextra ? 16 : 0) < nshell || (!nextra && 16 * nblk < nshell)) {
-    bsk_set_error("bsk_triples_mfma: layout does not match %d shells", nshell);
+  if (q->ntri == 0) return BSK_OK;
+  if (!q->d_items) {
+    bsk_set_error("bsk_triples_mfma: pair plan not uploaded (bsk_pairs_upload)");
     return BSK_EINVAL;
   }
-  MfmaArgs a;
+  PairArgs a;
+  // roles: set 0 / 1 = pair sides, set 2 = third side
+  a.hi[0] = q->mode == 2 ? hi_i : hi_j;
+  a.lo[0] = q->mode == 2 ? lo_i : lo_j;
   a.hi[1] = hi_j;
   a.lo[1] = lo_j;
-  a.hi[0] = mode == 2 ? hi_i : hi_j;
-  a.lo[0] = mode == 2 ? lo_i : lo_j;
-  a.hi[2] = mode >= 1 ? hi_k : hi_j;
-  a.lo[2] = mode >= 1 ? lo_k : lo_j;
-  for (int s = 0; s < 3; ++s) {
-    a.stride[s] = shell_stride;
-    a.nsh[s] = nshell;
-  }
+  a.hi[2] = q->mode >= 1 ? hi_k : hi_j;
+  a.lo[2] = q->mode >= 1 ? lo_k : lo_j;
+  a.stride = shell_stride;
+  for (int r = 0; r < 3; ++r) a.nsh[r] = q->nshell;
+  a.nkb = q->nkb;
   a.ncell = ncell;
-  a.mode = mode;
-  a.nblk = nblk;
   a.nwg = nwg;
+  a.pairs = q->d_pairs;
   a.partial = reinterpret_cast<float*>(work);
-  a.nextra = nextra;
-  for (int x = 0; x < 16; ++x)
-    for (int e = 0; e < 3; ++e) a.xt[x][e] = x < nextra ? layout[2 + 3 * x + e] : 0;
-  for (int x = 0; x < nextra; ++x)
-    if (a.xt[x][0] < 16 * nblk || a.xt[x][0] >= nshell || a.xt[x][1] < 0 ||
-        a.xt[x][1] >= nblk || a.xt[x][2] < 0 || a.xt[x][2] >= nblk) {
-      bsk_set_error("bsk_triples_mfma: extra tile %d out of range", x);
-      return BSK_EINVAL;
-    }
+  a.item_total = q->item_total;
   hipStream_t s = as_stream(stream);
-  int rc = bsk_triple_mfma_launch(a, s);
-  if (rc) return rc;
-  return bsk_reduce_mfma_launch(a.partial, nwg, bsk_mfma_tiles(nblk, mode) + nextra, ntri,
-                                slot_dev, scale, out_dev, s);
+  for (int p = 0; p < q->npass; ++p) {
+    a.item0 = p * kPairWaves * q->ni;
+    a.items = q->d_items + a.item0;
+    int rc = bsk_triple_pairs_launch(a, q->sets, q->ni, s);
+    if (rc) return rc;
+  }
+  return bsk_reduce_mfma_launch(a.partial, nwg, q->item_total, q->ntri, q->d_slot, scale,
+                                out_dev, s);
 }
 
 }  // extern "C"

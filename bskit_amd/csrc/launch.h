@@ -52,23 +52,27 @@ int bsk_triple_tiled_launch(const T* f, long long stride, int nfield, long long 
 int bsk_reduce_tiled_launch(const double* partial, int ntri, int nsum, int nchunk, double scale,
                             double* out, hipStream_t s);
 
-// Split-BF16 MFMA path.
-struct MfmaArgs {
-  const uint16_t* hi[3];   // i-side, j-side, k-side shell sets (hi planes)
+// Split-BF16 MFMA path over pair lists.
+constexpr int kPairWaves = 8;    // waves per workgroup of k_triple_pairs
+constexpr int kPairChunk = 128;  // cells per staged chunk
+constexpr int kPairMaxItems = 8; // items per wave and launch
+struct PairArgs {
+  const uint16_t* hi[3];  // role sets: 0 = first pair side, 1 = second, 2 = third side
   const uint16_t* lo[3];
-  long long stride[3];     // elements between consecutive shells of a set
-  int nsh[3];              // real shells per set
-  long long ncell;         // cells per shell (multiple of 128)
-  int mode;                // 0 auto (i>=j>=k), 1 <AAB> (j<=i, all k), 2 <ABC> (all)
-  int nblk;                // 16-shell blocks (1..3)
-  int nwg;                 // workgroups (cell ranges)
-  float* partial;          // [nwg][tiles + nextra][256]
-  int nextra;              // extra tiles (auto mode): rows i >= 16*nblk
-  int xt[16][3];           // extra tile (i, jb, kb)
+  long long stride;       // elements between consecutive shells of a set
+  int nsh[3];             // shells per role set
+  int nkb;                // 16-shell blocks of the third side staged in LDS
+  long long ncell;        // cells per shell (multiple of kPairChunk)
+  int nwg;                // workgroups (cell ranges)
+  const int* items;       // [kPairWaves * NI] item descriptors of this launch
+  const int* pairs;       // [tiles + 1][16]: (row in set 0) | (row in set 1) << 16
+  float* partial;         // [nwg][item_total][256]
+  int item0;              // first item of this launch in the partial layout
+  int item_total;         // items per workgroup in the partial layout
 };
-int bsk_mfma_tiles(int nblk, int mode);  // tile slots per workgroup
-int bsk_mfma_supported(int nblk, int mode);
-constexpr int kMfmaMaxExtra = 8;  // extra tiles per launch (one slot per wave)
-int bsk_triple_mfma_launch(const MfmaArgs& a, hipStream_t s);
-int bsk_reduce_mfma_launch(const float* partial, int nwg, int tiles_per_wg, int ntri,
+int bsk_pair_items_round(int ni);  // compiled items-per-wave count >= ni (0: too many)
+int bsk_pair_prefetch(int nshell);  // prefetch segments per thread (0: too many shells)
+size_t bsk_pair_lds_bytes(int sets, const int nsh[3], int nkb);
+int bsk_triple_pairs_launch(const PairArgs& a, int sets, int ni, hipStream_t s);
+int bsk_reduce_mfma_launch(const float* partial, int nwg, int items_per_wg, int ntri,
                            const int* slot_dev, double scale, double* out, hipStream_t s);

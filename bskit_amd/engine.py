@@ -98,6 +98,44 @@ class ShellSet:
         return f(self.data) + f(self.lo)
 
 
+class PairPlan:
+    """Owner of a bsk_pairs plan: the MFMA kernel's tiles for one triangle list."""
+
+    def __init__(self, tri, nshell, mode, upload=True):
+        tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
+        h = ctypes.c_void_p()
+        L.call('bsk_pairs_create', ctypes.byref(h), int(mode), int(nshell), len(tri),
+               tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        self.handle = h
+        if upload:
+            L.call('bsk_pairs_upload', h)
+        self.ntri = len(tri)
+        v = [ctypes.c_int() for _ in range(5)]
+        L.call('bsk_pairs_info', h, *[ctypes.byref(x) for x in v])
+        self.npair, self.ntile, self.nitem, self.items_per_wave, self.npass = [x.value for x in v]
+
+    def tables(self):
+        """(items, pair rows) host tables of the launch."""
+        items = np.zeros(self.npass * 8 * self.items_per_wave, dtype=np.int32)
+        pairs = np.zeros((self.ntile + 1) * 16, dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.call('bsk_pairs_tables', self.handle, items.ctypes.data_as(ip), pairs.ctypes.data_as(ip))
+        return items, pairs
+
+    def slots(self):
+        out = np.zeros(self.ntri, dtype=np.int32)
+        L.call('bsk_pairs_slots', self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        return out
+
+    def __del__(self):
+        try:
+            if self.handle:
+                L.lib().bsk_pairs_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+
 class GridEngine:
     """Plan + kernels for one (Nmesh, BoxSize, bins) key on one device.
 
@@ -237,40 +275,26 @@ class GridEngine:
             self._tri_cache[key] = t
         return t
 
-    def mfma_layout(self, tri, nshell, mode):
-        """(layout int array, device slots, tiles per workgroup) of an MFMA launch."""
+    def pair_plan(self, tri, nshell, mode):
+        """Cached MFMA pair plan of a triangle list (host tables uploaded once)."""
         tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
-        key = (tri.tobytes(), 'layout', nshell, mode)
+        key = (tri.tobytes(), 'pairs', nshell, mode)
         r = self._tri_cache.get(key)
         if r is None:
-            slots = np.zeros(len(tri), dtype=np.int32)
-            layout = (ctypes.c_int * L.MFMA_LAYOUT_INTS)()
-            tiles = ctypes.c_int()
-            L.call('bsk_mfma_layout', nshell, mode, len(tri),
-                   tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                   slots.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), layout,
-                   ctypes.byref(tiles))
-            r = (layout, torch.from_numpy(slots).to(self.device), tiles.value)
+            with torch.cuda.device(self.device):
+                r = PairPlan(tri, nshell, mode)
             self._tri_cache[key] = r
         return r
 
     def mfma_nwg(self, ncell=None):
-        """Workgroups of the MFMA kernel: ~4 per CU at large sizes, >= 8 chunks each."""
+        """Workgroups of the MFMA kernel: up to 1024, >= 8 chunks each."""
         nchunk = (self.ncell if ncell is None else ncell) // 128
         return int(max(1, min(1024, nchunk // 8)))
 
     @staticmethod
     def mfma_supported(nshell, nfield, tri):
-        """Whether the MFMA kernel covers this shell count / field roles / triangles."""
-        tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
-        slots = np.zeros(len(tri), dtype=np.int32)
-        layout = (ctypes.c_int * L.MFMA_LAYOUT_INTS)()
-        tiles = ctypes.c_int()
-        rc = L.lib().bsk_mfma_layout(int(nshell), _MODE[nfield], len(tri),
-                                     tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                     slots.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), layout,
-                                     ctypes.byref(tiles))
-        return rc == L.OK
+        """Whether the MFMA kernel covers this shell count / field roles."""
+        return nshell <= (96 if nfield == 3 else 128)
 
     def plain_nchunk(self, ncell=None):
         ncell = self.ncell if ncell is None else ncell
@@ -308,16 +332,13 @@ class GridEngine:
                     raise ValueError('the MFMA path needs BF16X2 shells')
             mode = _MODE[nf]
             nwg = self.mfma_nwg(ncell) if nwg is None else int(nwg)
-            layout, slots, tiles = self.mfma_layout(tri, sa.nb, mode)
+            plan = self.pair_plan(tri, sa.nb, mode)
             nbytes = L.size_out()
-            L.call('bsk_triples_mfma_workspace', tiles, nwg, ctypes.byref(nbytes))
+            L.call('bsk_triples_mfma_workspace', plan.handle, nwg, ctypes.byref(nbytes))
             w = (work or self.work).get('mfma', nbytes.value)
-            hi_i, lo_i = sa.data, sa.lo
-            hi_j, lo_j = sb.data, sb.lo
-            hi_k, lo_k = sc.data, sc.lo
-            L.call('bsk_triples_mfma', mode, sa.nb, L.ptr(hi_i), L.ptr(lo_i), L.ptr(hi_j),
-                   L.ptr(lo_j), L.ptr(hi_k), L.ptr(lo_k), ncell, ncell, nwg, layout, T,
-                   L.ptr(slots), float(scale), L.ptr(out), L.ptr(w), s)
+            L.call('bsk_triples_mfma', plan.handle, L.ptr(sa.data), L.ptr(sa.lo), L.ptr(sb.data),
+                   L.ptr(sb.lo), L.ptr(sc.data), L.ptr(sc.lo), ncell, ncell, nwg, float(scale),
+                   L.ptr(out), L.ptr(w), s)
             return out
         if path == PATH_PLAIN:
             dt = {L.FMT_F32: L.DT_F32, L.FMT_F64: L.DT_F64}.get(sa.fmt)

@@ -109,7 +109,7 @@ class Server:
         job.path = PATH_MFMA if GridEngine.mfma_supported(eng.nb, job.nfield, tri) else PATH_PLAIN
         if job.path == PATH_MFMA and len(tri):
             mode = {1: L.MODE_AUTO, 2: L.MODE_AAB}[job.nfield]
-            eng.mfma_layout(tri, eng.nb, mode)
+            eng.pair_plan(tri, eng.nb, mode)
         eng.triangles_device(tri)
         job.footprint = self.footprint(eng, job.nfield)
         if job.footprint > self.budget:

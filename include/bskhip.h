@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define BSK_ABI_VERSION 2
+#define BSK_ABI_VERSION 3
 
 /* status codes */
 #define BSK_OK 0
@@ -155,29 +155,41 @@ int bsk_triples_tiled(int dtype, const void* fields, long long shell_stride, int
                       long long ncell, int ntri, const int* tri_dev, int nsum, int nchunk,
                       double scale, double* out_dev, void* work, void* stream);
 
-/* Split-BF16 MFMA triple sums: for each i, the GEMM (D o d_i) . D^T over
- * the cells with the pair products split into hi+lo bf16 (hi*hi + hi*lo +
- * lo*hi on v_mfma_f32_16x16x32_bf16, FP32 accumulation over short chains,
- * FP64 second level, FP64 final reduction).  Shells are BF16X2 planes.
- * mode BSK_MODE_AUTO uses (hi_j,lo_j) for all three sides; BSK_MODE_AAB adds
- * the k side; BSK_MODE_ABC uses all three sets.
- * bsk_mfma_layout (host) plans a launch for a triangle list: layout[0] =
- * 16-shell blocks, layout[1] = extra tiles (auto spectra whose last shells
- * only appear as the largest side, up to 8 tiles), layout[2+3t..] their
- * (i, jb, kb); it returns per-triangle partial slots and the tiles per
- * workgroup.  Supports up to 48 shells (32 for <ABC>, 32 + extra shells for
- * auto).  Replaces the per-triangle loop of measure_bispectrum_faster
- * (bskit/main.py:1871-1882) and, on unit shells, the N_tri sum (:2027-2031).
- * work: bsk_triples_mfma_workspace(tiles_per_wg, nwg) bytes. */
-#define BSK_MFMA_LAYOUT_INTS 26
-int bsk_mfma_layout(int nshell, int mode, int ntri, const int* tri_host, int* slot_host,
-                    int* layout, int* tiles_per_wg);
-int bsk_triples_mfma_workspace(int tiles_per_wg, int nwg, size_t* bytes);
-int bsk_triples_mfma(int mode, int nshell, const uint16_t* hi_i, const uint16_t* lo_i,
+/* Split-BF16 MFMA triple sums over pair lists.  A pair plan (host tables
+ * uploaded once) lists the distinct pairs (i, j) that close a triangle of
+ * the list, packed into 16-row tiles; per tile and 16-shell block kb of the
+ * third side the kernel computes the GEMM over the cells of the pair
+ * products d_i d_j (A operand) and the shells d_k (B operand), both split
+ * into bf16 hi + lo and multiplied as hi*hi + hi*lo + lo*hi on
+ * v_mfma_f32_16x16x32_bf16, FP32 accumulation over short chains, FP64
+ * second level and FP64 final reduction; out[t] = scale * sum for every
+ * triangle in the order given.  Shells are BF16X2 planes (hi, lo).
+ * Roles: BSK_MODE_AUTO takes all sides from (hi_j, lo_j) (any side order);
+ * BSK_MODE_AAB takes sides i, j from (hi_j, lo_j) and k from (hi_k, lo_k);
+ * BSK_MODE_ABC takes i, j, k from the three sets.  Up to 128 shells (96 for
+ * <ABC>).  Replaces the per-triangle loop of measure_bispectrum_faster
+ * (bskit/main.py:1871-1882) and, on unit shells, the N_tri sum
+ * (:2027-2031).  work: bsk_triples_mfma_workspace(pairs, nwg) bytes. */
+typedef struct bsk_pairs bsk_pairs;
+/* Host-only planning (no device calls); bsk_pairs_upload copies the tables
+ * to the current device once, before the first bsk_triples_mfma. */
+int bsk_pairs_create(bsk_pairs** pairs, int mode, int nshell, int ntri, const int* tri_host);
+int bsk_pairs_upload(bsk_pairs* pairs);
+int bsk_pairs_destroy(bsk_pairs* pairs);
+int bsk_pairs_info(const bsk_pairs* pairs, int* npair, int* ntile, int* nitem,
+                   int* items_per_wave, int* npass);
+/* Partial-buffer slot of each triangle (item * 256 + row * 16 + column). */
+int bsk_pairs_slots(const bsk_pairs* pairs, int* slot_host);
+/* Host copies of the launch tables, for inspection: items
+ * [npass * 8 * items_per_wave] (tile | kb << 16 | new-tile flag << 31) and
+ * pair rows [(ntile + 1) * 16] (row in set 0 | row in set 1 << 16; padding
+ * rows and the last, all-padding tile point at row nshell, a zero row). */
+int bsk_pairs_tables(const bsk_pairs* pairs, int* items, int* pair_rows);
+int bsk_triples_mfma_workspace(const bsk_pairs* pairs, int nwg, size_t* bytes);
+int bsk_triples_mfma(const bsk_pairs* pairs, const uint16_t* hi_i, const uint16_t* lo_i,
                      const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
                      const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
-                     const int* layout, int ntri, const int* slot_dev, double scale,
-                     double* out_dev, void* work, void* stream);
+                     double scale, double* out_dev, void* work, void* stream);
 
 /* Cloud-in-cell assignment of npart particles (pos: npart x 3 float, box
  * units, C order) to the periodic mesh grid (nmesh^3 float, overwritten).

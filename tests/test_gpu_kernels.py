@@ -244,9 +244,9 @@ def test_mfma_unit_shell_counts_exact(N, nb, width):
 
 
 @pytest.mark.parametrize('N,nb', [(32, 33), (64, 17)])
-def test_mfma_extra_tiles(N, nb):
-    """Shells past the last full 16-block that only appear as the largest side
-    ride on extra tiles (the bench's [k_f, k_f] squeezed shell)."""
+def test_mfma_tail_shells(N, nb):
+    """Shells past the last full 16-block (the bench's [k_f, k_f] squeezed
+    shell) in the pair tiles."""
     from bskit_amd import _lib as L
     from bskit_amd import bins
     box = 1000.0
@@ -260,8 +260,8 @@ def test_mfma_extra_tiles(N, nb):
     base = bins.closed_triangle_indices(np.stack([lo[:-1], hi[:-1]], 1), 1)
     extra = np.array([[nb - 1, j, j] for j in range(1, nb - 1)] + [[nb - 1, 5, 2]])
     tri = np.concatenate([base, extra])
-    layout, _, tiles = eng.mfma_layout(tri, nb, 0)
-    assert layout[1] > 0, 'expected the extra-tile layout'
+    plan = eng.pair_plan(tri, nb, 0)
+    assert 0 < plan.npair <= len(tri)
     f64 = _field(N, 21, np.float64)
     sh = est.shells(est.forward(f64), N, eng.n2lo, eng.n2hi, 'delta')
     ref = est.triple_sums(sh, sh, sh, tri) / N ** 3
@@ -270,5 +270,41 @@ def test_mfma_extra_tiles(N, nb):
                      fmt=L.FMT_BF16X2)
     got = eng.triples([sbf], tri, path='mfma', scale=1.0 / eng.ncell).cpu().numpy()
     ok, err = _bcheck(got, ref, absum, 5e-5)
-    print('extra tiles N=%d nb=%d layout=(%d,%d) err %.2e' % (N, nb, layout[0], layout[1], err))
+    print('tail shells N=%d nb=%d pairs %d tiles %d items %d err %.2e'
+          % (N, nb, plan.npair, plan.ntile, plan.nitem, err))
     assert ok
+
+
+@pytest.mark.parametrize('mode,nb,ntri', [(0, 40, 2000), (1, 24, 800), (2, 20, 900),
+                                          (0, 70, 3000)])
+def test_mfma_random_lists_match_fp64(mode, nb, ntri):
+    """Random triangle lists (any side order, repeats, up to 70 shells) in every
+    role mode: split-BF16 MFMA against the FP64 plain path on the same fields."""
+    from bskit_amd import _lib as L
+    N, box = 64, 1000.0
+    kf = 2 * np.pi / box
+    lo = kf * (1 + 0.4 * np.arange(nb))
+    hi = lo + 0.4 * kf
+    eng = _engine(N, box, lo, hi)
+    rng = np.random.default_rng(7 + mode)
+    tri = rng.integers(0, nb, size=(ntri, 3)).astype(np.int32)
+    nf = mode + 1
+    fs = [_field(N, 30 + i, np.float64) for i in range(nf)]
+    bf = [eng.shells(eng.forward(torch.from_numpy(f.astype(np.float32)).cuda()),
+                     fmt=L.FMT_BF16X2) for f in fs]
+    s64 = [eng.shells(eng.forward(torch.from_numpy(f).cuda()), fmt=L.FMT_F64,
+                      dtype=torch.float64) for f in fs]
+    sc = 1.0 / eng.ncell
+    got = eng.triples(bf, tri, path='mfma', scale=sc).cpu().numpy()
+    ref = eng.triples(s64, tri, path='plain', scale=sc).cpu().numpy()
+    # |terms| scale from the FP64 shells
+    roles = {1: (0, 0, 0), 2: (0, 0, 1), 3: (0, 1, 2)}[nf]
+    a, b, c = (s64[r].data for r in roles)
+    absum = torch.stack([(a[i].abs() * b[j].abs() * c[k].abs()).sum() for i, j, k in tri])
+    absum = absum.cpu().numpy() * sc
+    err = np.abs(got - ref) / absum
+    plan = eng.pair_plan(tri, nb, mode)
+    print('mode %d nb %d T %d pairs %d tiles %d items %d ni %d passes %d err %.2e'
+          % (mode, nb, ntri, plan.npair, plan.ntile, plan.nitem, plan.items_per_wave,
+             plan.npass, err.max()))
+    assert err.max() < 5e-5

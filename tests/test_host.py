@@ -163,34 +163,66 @@ def test_library_loads_and_host_calls_work():
     L.call('bsk_shell_thresholds', 64, 100.0, 2, L.dbl_array([kf, 3 * kf]),
            L.dbl_array([3 * kf, 5 * kf]), lo, hi)
     assert list(lo) == [1, 9] and list(hi) == [9, 25]
-    tiles = ctypes.c_int()
-    layout = (ctypes.c_int * L.MFMA_LAYOUT_INTS)()
-    tri = np.array([[2, 1, 0], [0, 1, 2], [1, 1, 1]], dtype=np.int32)
-    slots = np.zeros(3, dtype=np.int32)
-    ip = ctypes.POINTER(ctypes.c_int)
-    L.call('bsk_mfma_layout', 3, 0, 3, tri.ctypes.data_as(ip), slots.ctypes.data_as(ip), layout,
-           ctypes.byref(tiles))
-    assert slots[0] == slots[1]  # symmetric spectra: any ordering maps to one slot
-    assert layout[0] == 1 and layout[1] == 0 and tiles.value == 16
-    # 33 shells where shell 32 only appears as the largest side: 2 blocks + extra tiles
-    tri = np.array([[32, j, j] for j in range(1, 32)] + [[20, 10, 3]], dtype=np.int32)
-    slots = np.zeros(len(tri), dtype=np.int32)
-    L.call('bsk_mfma_layout', 33, 0, len(tri), tri.ctypes.data_as(ip), slots.ctypes.data_as(ip),
-           layout, ctypes.byref(tiles))
-    assert layout[0] == 2 and layout[1] == 2
-    assert tiles.value == 16 * (1 + 3) + 2
-    assert sorted([tuple(layout[2 + 3 * t:5 + 3 * t]) for t in range(2)]) == [(32, 0, 0), (32, 1, 1)]
-    # a triangle with shell 32 as a middle side forces three full blocks
-    tri2 = np.concatenate([tri, [[32, 32, 1]]]).astype(np.int32)
-    slots2 = np.zeros(len(tri2), dtype=np.int32)
-    L.call('bsk_mfma_layout', 33, 0, len(tri2), tri2.ctypes.data_as(ip),
-           slots2.ctypes.data_as(ip), layout, ctypes.byref(tiles))
-    assert layout[0] == 3 and layout[1] == 0 and tiles.value == 16 * 10
     n = ctypes.c_size_t()
-    L.call('bsk_triples_mfma_workspace', 160, 10, ctypes.byref(n))
-    assert n.value == 10 * 160 * 256 * 4
+    from bskit_amd.engine import PairPlan
+    plan = PairPlan(np.array([[2, 1, 0], [0, 1, 2], [1, 1, 1]]), 3, L.MODE_AUTO, upload=False)
+    sl = plan.slots()
+    assert sl[0] == sl[1]  # symmetric spectra: any side order maps to one slot
+    L.call('bsk_triples_mfma_workspace', plan.handle, 10, ctypes.byref(n))
+    assert n.value == 10 * plan.npass * 8 * plan.items_per_wave * 256 * 4
+    for mode, nsh in ((L.MODE_AUTO, 129), (L.MODE_ABC, 97)):
+        with pytest.raises(L.BskError):
+            PairPlan(np.array([[0, 0, 0]]), nsh, mode, upload=False)
+    PairPlan(np.array([[127, 64, 63]]), 128, L.MODE_AUTO, upload=False)
     with pytest.raises(L.BskError):
-        L.call('bsk_mfma_layout', 60, 0, 0, None, None, layout, ctypes.byref(tiles))
+        PairPlan(np.array([[3, 0, 0]]), 3, L.MODE_AUTO, upload=False)
+
+
+def _emulate_pair_plan(plan, sets, tri, mode, nshell):
+    """FP64 numpy emulation of the MFMA kernel's tables: per item the 16x16
+    tile sum_x (d_a d_b)(x) d_k(x), read back through the triangle slots."""
+    items, prs = plan.tables()
+    f0, f1, f2 = sets
+    z = np.zeros_like(f0[0])
+    nb = nshell
+    part = np.zeros((len(items), 16, 16))
+    for g, d in enumerate(items):
+        tile = d & 0xffff
+        kb = (d >> 16) & 0x7fff
+        for row in range(16):
+            pr = prs[tile * 16 + row]
+            a, b = pr & 0xffff, pr >> 16
+            pa = (f0[a] if a < nb else z) * (f1[b] if b < nb else z)
+            for col in range(16):
+                k = 16 * kb + col
+                if k < nb:
+                    part[g, row, col] = np.sum(pa * f2[k])
+    flat = part.reshape(-1)
+    return flat[plan.slots()]
+
+
+@pytest.mark.parametrize('mode,nshell,ntri,seed', [(0, 7, 40, 1), (0, 33, 300, 2), (1, 20, 150, 3),
+                                                   (2, 18, 200, 4), (0, 40, 900, 5)])
+def test_pair_plan_tables_compute_every_triangle(mode, nshell, ntri, seed):
+    """The host plan's tiles, items and slots give every triangle its sum."""
+    from bskit_amd import _lib as L
+    from bskit_amd.engine import PairPlan
+    try:
+        L.lib()
+    except (ImportError, OSError) as e:
+        pytest.skip('library not loadable here: %s' % e)
+    rng = np.random.default_rng(seed)
+    ncell = 24
+    nset = {0: 1, 1: 2, 2: 3}[mode]
+    fields = [rng.standard_normal((nshell, ncell)) for _ in range(nset)]
+    roles = {0: (0, 0, 0), 1: (0, 0, 1), 2: (0, 1, 2)}[mode]
+    tri = rng.integers(0, nshell, size=(ntri, 3))
+    plan = PairPlan(tri, nshell, mode, upload=False)
+    assert plan.npair <= ntri and plan.items_per_wave <= 8
+    sets = [fields[r] for r in roles]
+    got = _emulate_pair_plan(plan, sets, tri, mode, nshell)
+    ref = np.array([np.sum(sets[0][i] * sets[1][j] * sets[2][k]) for i, j, k in tri])
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
 
 
 def test_shell_threshold_abi_matches_oracle():
