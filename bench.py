@@ -102,6 +102,15 @@ class StageTimer:
     def durations(self, a, b):
         return [x.elapsed_time(y) for x, y in zip(self.ev[a], self.ev[b])]
 
+    def pair(self, name):
+        """Two fresh events (start, end) recorded by the callee, kept under name."""
+        e = (self.torch.cuda.Event(enable_timing=True), self.torch.cuda.Event(enable_timing=True))
+        self.ev.setdefault(name, []).append(e)
+        return e
+
+    def pair_durations(self, name):
+        return [a.elapsed_time(b) for a, b in self.ev.get(name, [])]
+
 
 def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
     from bskit_amd import _lib as L
@@ -121,7 +130,8 @@ def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
         eng.shells(dk, fmt=fmt, dtype=dtype, out=shells)
         if t is not None:
             t.mark('shells')
-        eng.triples([shells], wl['tri'], path=path, scale=scale, out=out)
+        kev = t.pair('kernel') if (t is not None and path == 'mfma') else None
+        eng.triples([shells], wl['tri'], path=path, scale=scale, out=out, kernel_events=kev)
         if t is not None:
             t.mark('triples')
         b = out / ntri_dev
@@ -138,6 +148,21 @@ def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     return b, elapsed
+
+
+def profiled_traffic(kernel, nmesh, shell_mesh):
+    """HBM bytes per launch of kernel from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), or
+    None when no summary covers this workload."""
+    path = os.path.join(ROOT, 'profiles', 'hbm_traffic_%d_s%d.txt' % (nmesh, shell_mesh))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            w = line.split()
+            if w and not line.startswith('#') and w[0] == kernel:
+                return float(w[-1])
+    return None
 
 
 def run_serving(args, torch, np):
@@ -224,6 +249,8 @@ def main():
     ap.add_argument('--no-batch', action='store_true', help='skip the graph-batched cross run')
     ap.add_argument('--jobs', type=int, default=2000, help='jobs in the serving run')
     ap.add_argument('--load', type=float, default=0.8, help='arrival rate / measured capacity')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
+                    help='only run 5 device-to-device copies of BYTES (HBM counter calibration)')
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) != 1:
@@ -242,6 +269,15 @@ def main():
         print('bench.py needs a GPU', file=sys.stderr)
         sys.exit(1)
     torch.cuda.init()
+    if args.copy_calibration:
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes': 4 * n, 'copies': 5}), flush=True)
+        return
     log('[bench] building workload N=%d nbin=%d' % (args.nmesh, args.nbin))
     wl = build_workload(args, torch, np)
     T = len(wl['tri'])
@@ -295,30 +331,28 @@ def main():
             acc['fp32_max_rel_err'] = float(rel_p.max())
         log('[bench] accuracy vs FP64: %s' % acc)
 
-    # roofline of the dominant kernel (the MFMA triple sum over all shells)
+    # roofline of the dominant kernel: the split-BF16 MFMA pair kernel, timed
+    # by HIP events around its launch inside every timed step
     N = wl['N']
     ncell = float(eng.ncell)                     # shell-mesh cells S^3
     nsh = eng.nb
-    shell_bytes = nsh * 4 * ncell               # split-BF16 hi+lo, read once
-    flops = 2.0 * T * ncell
-    t_tri = stage_ms['triples_mfma'] * 1e-3
+    shell_bytes = nsh * 4 * ncell               # split-BF16 hi+lo, each value read once
+    flops = 2.0 * T * ncell                      # one multiply-add per triangle per cell
+    kern_ms = float(np.mean(timer.pair_durations('kernel')))
+    t_k = kern_ms * 1e-3
+    ai = flops / shell_bytes
+    bound = 'hbm' if ai * HBM_PEAK_GBS * 1e9 < BF16_PEAK_TFLOPS * 1e12 else 'mfma'
+    achieved = shell_bytes / t_k / 1e9
+    plan = eng.pair_plan(wl['tri'], nsh, 0)
+    roof = {'kernel': 'k_triple_pairs', 'bound': bound, 'achieved': achieved,
+            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+            'traffic': profiled_traffic('k_triple_pairs', N, eng.S),
+            'kernel_ms': kern_ms, 'algorithmic_bytes': shell_bytes,
+            'useful_tflops': flops / t_k / 1e12, 'arith_intensity_flop_per_byte': ai,
+            'pairs': plan.npair, 'tiles': plan.ntile, 'items': plan.nitem}
     H = eng.MX * eng.MY * eng.MZ                 # compact delta(k) cube
     t_sh = stage_ms['shells'] * 1e-3
     shells_alg = 8.0 * H + nsh * 4 * ncell       # delta(k) read + shells written
-    dominant = 'triples_mfma' if t_tri >= t_sh else 'shells'
-    if dominant == 'triples_mfma':
-        ai = flops / shell_bytes
-        bound = 'hbm' if ai * HBM_PEAK_GBS * 1e9 < BF16_PEAK_TFLOPS * 1e12 else 'mfma'
-        achieved = shell_bytes / t_tri / 1e9
-        roof = {'kernel': 'k_triple_mfma<3,0,8,2> + k_reduce_mfma', 'bound': bound,
-                'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                'useful_tflops': flops / t_tri / 1e12, 'arith_intensity_flop_per_byte': ai}
-    else:
-        achieved = shells_alg / t_sh / 1e9
-        roof = {'kernel': 'shell passes (k_inv_x_mask, k_inv_y, k_inv_z)', 'bound': 'hbm',
-                'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                'frac': achieved / HBM_PEAK_GBS, 'traffic': None}
     stage_gbs = {'shells_GBps': shells_alg / t_sh / 1e9,
                  'forward_GBps': (4.0 * N ** 3 + 8.0 * H) / (stage_ms['forward'] * 1e-3) / 1e9}
 

@@ -60,6 +60,7 @@ _SIGS = {
     'bsk_triples_mfma_workspace': [_vp, _c_int, _c_size_p],
     'bsk_triples_mfma': [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _c_int, _c_double, _vp,
                          _vp, _vp],
+    'bsk_triples_mfma_reduce': [_vp, _c_int, _c_double, _vp, _vp, _vp],
     'bsk_cic_paint': [_c_int, _c_double, _c_ll, _vp, _vp, _c_int, _vp],
     'bsk_direct_triangles': [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp,
                              _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp],

@@ -780,7 +780,7 @@ int bsk_triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* l
                      const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
                      const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
                      double scale, double* out_dev, void* work, void* stream) {
-  if (!q || !hi_j || !lo_j || !out_dev || !work || nwg <= 0 || shell_stride < ncell ||
+  if (!q || !hi_j || !lo_j || !work || nwg <= 0 || shell_stride < ncell ||
       (q->mode >= 1 && (!hi_k || !lo_k)) || (q->mode == 2 && (!hi_i || !lo_i))) {
     bsk_set_error("bsk_triples_mfma: invalid argument");
     return BSK_EINVAL;
@@ -813,8 +813,19 @@ int bsk_triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* l
     int rc = bsk_triple_pairs_launch(a, q->sets, q->ni, s);
     if (rc) return rc;
   }
+  if (!out_dev) return BSK_OK;  // partials only; bsk_triples_mfma_reduce finishes
   return bsk_reduce_mfma_launch(a.partial, nwg, q->item_total, q->ntri, q->d_slot, scale,
                                 out_dev, s);
+}
+
+int bsk_triples_mfma_reduce(const bsk_pairs* q, int nwg, double scale, double* out_dev,
+                            const void* work, void* stream) {
+  if (!q || nwg <= 0 || !out_dev || !work || !q->d_slot) {
+    bsk_set_error("bsk_triples_mfma_reduce: invalid argument (plan uploaded?)");
+    return BSK_EINVAL;
+  }
+  return bsk_reduce_mfma_launch(reinterpret_cast<const float*>(work), nwg, q->item_total,
+                                q->ntri, q->d_slot, scale, out_dev, as_stream(stream));
 }
 
 }  // extern "C"

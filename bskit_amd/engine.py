@@ -301,7 +301,7 @@ class GridEngine:
         return int(max(1, min(4096, ncell // 65536)))
 
     def triples(self, sets, tri, path=PATH_MFMA, scale=1.0, out=None, stream=None, nwg=None,
-                work=None):
+                work=None, kernel_events=None):
         """scale * sum_x A_i B_j C_k for every (i, j, k) row of tri.
 
         sets: [A] (auto), [A, B] (<AAB>: sides A, A, B) or [A, B, C].
@@ -336,9 +336,19 @@ class GridEngine:
             nbytes = L.size_out()
             L.call('bsk_triples_mfma_workspace', plan.handle, nwg, ctypes.byref(nbytes))
             w = (work or self.work).get('mfma', nbytes.value)
+            if kernel_events is None:
+                L.call('bsk_triples_mfma', plan.handle, L.ptr(sa.data), L.ptr(sa.lo),
+                       L.ptr(sb.data), L.ptr(sb.lo), L.ptr(sc.data), L.ptr(sc.lo), ncell, ncell,
+                       nwg, float(scale), L.ptr(out), L.ptr(w), s)
+                return out
+            # events around the MFMA kernel alone (the reduction follows)
+            kernel_events[0].record(stream)
             L.call('bsk_triples_mfma', plan.handle, L.ptr(sa.data), L.ptr(sa.lo), L.ptr(sb.data),
                    L.ptr(sb.lo), L.ptr(sc.data), L.ptr(sc.lo), ncell, ncell, nwg, float(scale),
-                   L.ptr(out), L.ptr(w), s)
+                   None, L.ptr(w), s)
+            kernel_events[1].record(stream)
+            L.call('bsk_triples_mfma_reduce', plan.handle, nwg, float(scale), L.ptr(out),
+                   L.ptr(w), s)
             return out
         if path == PATH_PLAIN:
             dt = {L.FMT_F32: L.DT_F32, L.FMT_F64: L.DT_F64}.get(sa.fmt)

@@ -190,6 +190,11 @@ int bsk_triples_mfma(const bsk_pairs* pairs, const uint16_t* hi_i, const uint16_
                      const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
                      const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
                      double scale, double* out_dev, void* work, void* stream);
+/* out_dev == NULL in bsk_triples_mfma leaves the per-workgroup partials in
+ * work; bsk_triples_mfma_reduce then sums them in FP64 into out_dev (the
+ * same result as one call with out_dev). */
+int bsk_triples_mfma_reduce(const bsk_pairs* pairs, int nwg, double scale, double* out_dev,
+                            const void* work, void* stream);
 
 /* Cloud-in-cell assignment of npart particles (pos: npart x 3 float, box
  * units, C order) to the periodic mesh grid (nmesh^3 float, overwritten).
