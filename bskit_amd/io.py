@@ -124,3 +124,16 @@ def resume_index(path):
     if last is None:
         return 0
     return int(last.split()[0]) + 1
+
+
+def truncate_partial(path):
+    """Drop an unterminated last line (a write cut short); returns bytes kept."""
+    try:
+        with open(path, 'rb+') as f:
+            data = f.read()
+            keep = data.rfind(b'\n') + 1
+            if keep != len(data):
+                f.truncate(keep)
+            return keep
+    except FileNotFoundError:
+        return 0

@@ -278,3 +278,22 @@ def test_constructor_errors_without_gpu():
     with pytest.raises(NotImplementedError):
         bskit_amd.FFTBispectrum(np.zeros((16, 16, 8), np.float32), BoxSize=100., kmin=0.1,
                                 kmax=1., for_grid_info_only=True)
+
+
+def test_truncate_partial_and_cli_errors(tmp_path):
+    from bskit_amd import measure
+    p = tmp_path / 'rows.dat'
+    p.write_text('0 1.0\n1 2.0\n2 3.0e+0')           # interrupted mid-line
+    assert bio.truncate_partial(str(p)) == len('0 1.0\n1 2.0\n')
+    assert p.read_text() == '0 1.0\n1 2.0\n'
+    assert bio.resume_index(str(p)) == 2
+    assert bio.truncate_partial(str(tmp_path / 'missing.dat')) == 0
+    g = tmp_path / 'g.npy'
+    np.save(g, np.zeros((16, 16, 16), dtype=np.float32))
+    base = ['npy_grid', str(g), '16', '16', '100.', '0', '10', str(tmp_path / 'o.dat')]
+    with pytest.raises(RuntimeError, match='k-bin file or kmin/kmax/dk'):
+        measure.main(base)
+    with pytest.raises(NotImplementedError):
+        measure.main(['gadget'] + base[1:] + ['--k_bin_info', '0.1', '0.5', '0.1'])
+    with pytest.raises(ValueError, match='isos_mult'):
+        measure.main(base + ['--k_bin_info', '0.1', '0.5', '0.1', '--triangle_type', 'isosceles'])
