@@ -247,6 +247,7 @@ def main():
     ap.add_argument('--no-plain', action='store_true', help='skip the FP32 plain path')
     ap.add_argument('--no-serve', action='store_true', help='skip the serving (latency) run')
     ap.add_argument('--no-batch', action='store_true', help='skip the graph-batched cross run')
+    ap.add_argument('--no-tune', action='store_true', help='skip the autotuner (default variants)')
     ap.add_argument('--jobs', type=int, default=2000, help='jobs in the serving run')
     ap.add_argument('--load', type=float, default=0.8, help='arrival rate / measured capacity')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
@@ -284,6 +285,12 @@ def main():
     eng = GridEngine(wl['N'], wl['L'], wl['k_low'], wl['k_high'], cic_window=wl['N'])
     log('[bench] T=%d shells=%d compact m=%d shell mesh S=%d' % (T, eng.nb, eng.m, eng.S))
 
+    tuned = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuned = eng.autotune(tri=wl['tri'], field=wl['field'], log=log)
+        tuned = {'variants': tuned['variants'], 'mfma_nwg': tuned.get('mfma_nwg'),
+                 'seconds': time.perf_counter() - t0}
     t0 = time.perf_counter()
     ntri, kbar = eng.grid_info(wl['tri'])
     torch.cuda.synchronize()
@@ -404,6 +411,7 @@ def main():
         'stage_ms': stage_ms,
         'stage_GBps': stage_gbs,
         'gridinfo_once_s': t_gridinfo,
+        'autotune': tuned,
         'accuracy_vs_fp64': acc,
         'latency': latency,
         'batched_cross': batched,

@@ -103,17 +103,19 @@ struct Dft<T, 5, INV> {
   }
 };
 
-// LDS index of element i of a pencil: one padding slot after every 8 complex
-// values, so the stride-R stores of a Stockham stage and the stride-LD
-// accesses across pencils fall in distinct banks.
-BSK_DEV int lpad(int i) { return i + (i >> 3); }
-// padded pencil stride for length L (odd, > lpad(L-1))
+// Pencil stride in LDS: the next odd length >= L (complex elements), so
+// consecutive pencils start two (FP32) or four (FP64) banks apart and the
+// lanes of a wave, which walk consecutive pencils, hit distinct banks.
 template <int L>
 constexpr int pencil_ld() {
-  return L + L / 8 + 1;
+  return L | 1;
 }
 
 // ---------------------------------------------------------------- one stage
+// Butterfly b of the stage: pencil p = b % P (fastest across lanes), column
+// j = b / P.  Loads read j + r*NB, stores write (j/NS)*NS*R + k + r*NS of the
+// pencil: one base address plus immediate offsets per butterfly, and lanes
+// with the same j share their twiddles (broadcast reads).
 template <typename T, int L, int R, int NS, bool INV, int P, int W, int LD, int TWN>
 BSK_DEV void stockham_stage(cplx<T>* buf, const cplx<T>* tw) {
   constexpr int NB = L / R;
@@ -125,10 +127,10 @@ BSK_DEV void stockham_stage(cplx<T>* buf, const cplx<T>* tw) {
   for (int q = 0; q < BPT; ++q) {
     const int b = tid + q * W;
     if ((TOT % W) == 0 || b < TOT) {
-      const int p = b / NB, j = b - p * NB;
-      const cplx<T>* src = buf + p * LD;
+      const int p = b % P, j = b / P;
+      const cplx<T>* src = buf + p * LD + j;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[q][r] = src[lpad(j + r * NB)];
+      for (int r = 0; r < R; ++r) v[q][r] = src[r * NB];
     }
   }
   __syncthreads();
@@ -136,21 +138,21 @@ BSK_DEV void stockham_stage(cplx<T>* buf, const cplx<T>* tw) {
   for (int q = 0; q < BPT; ++q) {
     const int b = tid + q * W;
     if ((TOT % W) == 0 || b < TOT) {
-      const int p = b / NB, j = b - p * NB;
+      const int p = b % P, j = b / P;
       const int k = j % NS;
       if (NS > 1) {
+        const int ks = k * (TWN / (NS * R));
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          cplx<T> w = tw[k * r * (TWN / (NS * R))];
+          cplx<T> w = tw[ks * r];
           if (INV) w.y = -w.y;
           v[q][r] = cmul(v[q][r], w);
         }
       }
       Dft<T, R, INV>::run(v[q]);
-      cplx<T>* dst = buf + p * LD;
-      const int o = (j / NS) * NS * R + k;
+      cplx<T>* dst = buf + p * LD + (j / NS) * NS * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[lpad(o + r * NS)] = v[q][r];
+      for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
     }
   }
   __syncthreads();

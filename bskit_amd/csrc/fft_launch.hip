@@ -20,11 +20,15 @@ template <typename T, int N>
 constexpr int pxy() {
   return (sizeof(T) == 8 || N > 640) ? 8 : 16;
 }
-// Rows per workgroup for the contiguous z passes: about 4096 complex values.
+// Rows per workgroup for the contiguous z passes: the power of two nearest
+// below about 4096 (FP64: 2048) complex values per workgroup, so the
+// butterfly -> (row, column) split of the LDS stages is a mask and a shift.
 template <typename T, int NH>
 constexpr int pz() {
   int p = (sizeof(T) == 8 ? 2048 : 4096) / NH;
-  return p < 1 ? 1 : (p > 64 ? 64 : p);
+  int q = 1;
+  while (2 * q <= p && q < 64) q *= 2;
+  return q;
 }
 
 inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }

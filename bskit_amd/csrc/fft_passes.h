@@ -45,7 +45,7 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
     const long long row = row0 + p;
     cplx<T> v = {(T)0, (T)0};
     if (row < nrows) v = src[row * NH + n];
-    buf[p * LD + lpad(n)] = v;
+    buf[p * LD + n] = v;
   }
   __syncthreads();
   fft_lds<T, NH, false, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
@@ -53,8 +53,8 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
     const int p = e / MZ, k = e - p * MZ;
     const long long row = row0 + p;
     if (row >= nrows) continue;
-    cplx<T> zk = buf[p * LD + lpad((k % NH))];
-    cplx<T> zc = cconj(buf[p * LD + lpad(((NH - k) % NH))]);
+    cplx<T> zk = buf[p * LD + (k % NH)];
+    cplx<T> zc = cconj(buf[p * LD + ((NH - k) % NH)]);
     cplx<T> xe = cscale(cadd(zk, zc), (T)0.5);
     cplx<T> d = csub(zk, zc);
     cplx<T> xo = {(T)0.5 * d.y, (T)-0.5 * d.x};  // (zk - zc) / (2i)
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
     const int y = e / P, p = e - y * P, kz = kz0 + p;
     cplx<T> v = {(T)0, (T)0};
     if (kz < g.MZ) v = in[(x * N + y) * g.MZ + kz];
-    buf[p * LD + lpad(y)] = v;
+    buf[p * LD + y] = v;
   }
   __syncthreads();
   fft_lds<T, N, false, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
@@ -86,7 +86,7 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
     const int r = e / P, p = e - r * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
     const int idx = (g.nlo + r + N) % N;
-    out[(x * g.MY + r) * g.MZ + kz] = buf[p * LD + lpad(idx)];
+    out[(x * g.MY + r) * g.MZ + kz] = buf[p * LD + idx];
   }
 }
 
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
     const int x = e / P, p = e - x * P, kz = kz0 + p;
     cplx<T> v = {(T)0, (T)0};
     if (kz < g.MZ) v = in[((long long)x * g.MY + ry) * g.MZ + kz];
-    buf[p * LD + lpad(x)] = v;
+    buf[p * LD + x] = v;
   }
   __syncthreads();
   fft_lds<T, N, false, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
     const int nx = g.nlo + r;
     T s = scale;
     if (comp) s *= comp[nx + N / 2] * comp[ny + N / 2] * comp[kz + N / 2];
-    out[(ry * g.MX + r) * g.MZ + kz] = cscale(buf[p * LD + lpad((nx + N) % N)], s);
+    out[(ry * g.MX + r) * g.MZ + kz] = cscale(buf[p * LD + (nx + N) % N], s);
   }
 }
 
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
           v.x = kf * sqrt((T)n2);
       }
     }
-    buf[p * LD + lpad(xi)] = v;
+    buf[p * LD + xi] = v;
   }
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
   for (int e = tid; e < P * N; e += W) {
     const int x = e / P, p = e - x * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
-    out[(obase + x) * g.MZ + kz] = buf[p * LD + lpad(x)];
+    out[(obase + x) * g.MZ + kz] = buf[p * LD + x];
   }
 }
 
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
     cplx<T> v = {(T)0, (T)0};
     if (kz < g.MZ && ry >= 0 && ry < g.MY)
       v = in[((bl * g.MY + ry) * N + x) * g.MZ + kz];
-    buf[p * LD + lpad(yi)] = v;
+    buf[p * LD + yi] = v;
   }
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
   for (int e = tid; e < P * N; e += W) {
     const int y = e / P, p = e - y * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
-    out[(obase + y) * g.MZ + kz] = buf[p * LD + lpad(y)];
+    out[(obase + y) * g.MZ + kz] = buf[p * LD + y];
   }
 }
 
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
       cplx<T> wd = cmul(w, d);
       z = {s.x - wd.y, s.y + wd.x};  // s + i * w * d
     }
-    buf[p * LD + lpad(k)] = z;
+    buf[p * LD + k] = z;
   }
   __syncthreads();
   fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
     const int p = e / NH, n = e - p * NH;
     const long long row = row0 + p;
     if (row >= nrows) continue;
-    const cplx<T> z = buf[p * LD + lpad(n)];
+    const cplx<T> z = buf[p * LD + n];
     const long long off = bl * shell_stride + cell0 + row * N + 2 * n;
     store_pair<T>(fmt, out, out_lo, off, z.x, z.y);
   }

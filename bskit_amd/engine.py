@@ -174,6 +174,7 @@ class GridEngine:
         self._set_shell_mesh(0 if shell_mesh is None else int(shell_mesh))
         self.work = Workspace(self.device)
         self._tri_cache = {}
+        self.tuned_nwg = None
         self.cic_window = 0
         if cic_window:
             self.set_compensation(cic_window)
@@ -206,6 +207,11 @@ class GridEngine:
     def set_compensation(self, nmesh_window):
         L.call('bsk_plan_set_compensation', self._plan, int(nmesh_window))
         self.cic_window = int(nmesh_window)
+
+    def autotune(self, tri=None, field=None, force=False, log=None):
+        """Tune radix variants and the MFMA workgroup count (cached per shape)."""
+        from . import tune
+        return tune.tune(self, tri=tri, field=field, force=force, log=log)
 
     def set_variant(self, fft_pass, variant):
         L.call('bsk_plan_set_variant', self._plan, int(fft_pass), int(variant))
@@ -287,7 +293,10 @@ class GridEngine:
         return r
 
     def mfma_nwg(self, ncell=None):
-        """Workgroups of the MFMA kernel: up to 1024, >= 8 chunks each."""
+        """Workgroups of the MFMA kernel: the tuned count, else up to 1024 with
+        >= 8 chunks each."""
+        if self.tuned_nwg and (ncell is None or ncell == self.ncell):
+            return self.tuned_nwg
         nchunk = (self.ncell if ncell is None else ncell) // 128
         return int(max(1, min(1024, nchunk // 8)))
 

@@ -308,3 +308,28 @@ def test_mfma_random_lists_match_fp64(mode, nb, ntri):
           % (mode, nb, ntri, plan.npair, plan.ntile, plan.nitem, plan.items_per_wave,
              plan.npass, err.max()))
     assert err.max() < 5e-5
+
+
+def test_autotune_picks_valid_config_and_keeps_results(tmp_path, monkeypatch):
+    """The tuner's choice is cached, re-applied, and changes only rounding."""
+    from bskit_amd import _lib as L
+    from bskit_amd import bins, tune
+    monkeypatch.setenv('BSK_TUNE_CACHE', str(tmp_path / 'tune.json'))
+    monkeypatch.setattr(tune, '_CACHE', None)
+    N, nb, box = 96, 8, 1000.0
+    lo, hi = _bins(nb, 1, box)
+    tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), 1)
+    f = torch.from_numpy(_field(N, 41, np.float32)).cuda()
+    eng = _engine(N, box, lo, hi)
+    ref = eng.triples([eng.shells(eng.forward(f), fmt=L.FMT_BF16X2)], tri, path='mfma',
+                      scale=1.0 / eng.ncell).cpu().numpy()
+    cfg = eng.autotune(tri=tri, field=f)
+    assert len(cfg['variants']) == 6 and set(cfg['variants']) <= {0, 1}
+    assert (tmp_path / 'tune.json').exists()
+    got = eng.triples([eng.shells(eng.forward(f), fmt=L.FMT_BF16X2)], tri, path='mfma',
+                      scale=1.0 / eng.ncell).cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
+    eng2 = _engine(N, box, lo, hi)
+    assert tune.lookup(eng2, len(tri)) == cfg
+    assert eng2.autotune(tri=tri) == cfg         # cached: no re-timing
+    assert eng2.mfma_nwg() == cfg.get('mfma_nwg', eng2.mfma_nwg())
