@@ -160,7 +160,7 @@ def profiled_traffic(kernel, nmesh, shell_mesh):
     with open(path) as f:
         for line in f:
             w = line.split()
-            if w and not line.startswith('#') and w[0] == kernel:
+            if w and not line.startswith('#') and w[0].split('<')[0] == kernel:
                 return float(w[-1])
     return None
 

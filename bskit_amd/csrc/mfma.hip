@@ -116,8 +116,15 @@ BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l) {
 // SETS: 1 = one field (pairs and third side from set 0), 2 = pairs from set
 // 0, third side from set 2 (<AAB>), 3 = pairs from sets 0 and 1, third side
 // from set 2 (<ABC>).
+// Waves per SIMD the register allocation must allow: 4 (two workgroups per
+// CU, at most 128 VGPRs) where the item and prefetch registers fit, else 2.
+template <int NI, int PV>
+constexpr int pair_occupancy() {
+  return (NI <= 4 && PV <= 2) ? 4 : 2;
+}
+
 template <int NI, int PV, int SETS>
-__global__ __launch_bounds__(kNT) void k_triple_pairs(PairArgs a) {
+__global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV>())) void k_triple_pairs(PairArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int n0 = a.nsh[0], n1 = a.nsh[1], n2 = a.nsh[2];
   float* fA = reinterpret_cast<float*>(smem);         // (n0 + 1) rows
@@ -142,15 +149,15 @@ __global__ __launch_bounds__(kNT) void k_triple_pairs(PairArgs a) {
   }
 
   // item descriptors of this wave and the LDS rows of this lane's pair
+  // (row offsets in floats, both < 2^16, packed in one register per item)
   int desc[NI];
-  int oi[NI], oj[NI];
+  int oij[NI];
 #pragma unroll
   for (int s = 0; s < NI; ++s) {
     desc[s] = __builtin_amdgcn_readfirstlane(a.items[wave * NI + s]);
     const int tile = desc[s] & 0xffff;
     const int pr = a.pairs[tile * 16 + col];
-    oi[s] = (pr & 0xffff) * kLDF;
-    oj[s] = (pr >> 16) * kLDF;
+    oij[s] = ((pr & 0xffff) * kLDF) | (((pr >> 16) * kLDF) << 16);
   }
 
   f32x4 acc[NI];
@@ -191,8 +198,8 @@ __global__ __launch_bounds__(kNT) void k_triple_pairs(PairArgs a) {
       static_for<0, NI>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         if (s == 0 || desc[s] < 0) {  // uniform: new tile -> new pair products
-          const float4* pi = reinterpret_cast<const float4*>(fA + oi[s] + c);
-          const float4* pj = reinterpret_cast<const float4*>(fB + oj[s] + c);
+          const float4* pi = reinterpret_cast<const float4*>(fA + (oij[s] & 0xffff) + c);
+          const float4* pj = reinterpret_cast<const float4*>(fB + ((unsigned)oij[s] >> 16) + c);
           const float4 x0 = pi[0], x1 = pi[1], y0 = pj[0], y1 = pj[1];
           uint32_t ah[4], al[4];
           split2(x0.x * y0.x, x0.y * y0.y, ah[0], al[0]);
