@@ -10,8 +10,10 @@ from .bins import (closed_triangle_indices, generate_bin_edge_list,  # noqa: F40
                    geometric_k_mean)
 from .main import (FFTBispectrum, bk_binned, bk_FFT_full, bk_FFT_grid_info,  # noqa: F401
                    bk_FFT_unnormalized_value, compute_bk_FFT_value, compute_k_means_on_grid,
-                   compute_Nbin, k_field, number_field, PRECISIONS)
+                   compute_Nbin, k_field, number_field, pk_FFT, power_spectrum, PRECISIONS)
 from .mesh import ArrayMesh, MeshSource, ParticleMesh, cast_source  # noqa: F401
+from .subbox import (field_subbox_pm, field_subbox_slice, measure_subboxes,  # noqa: F401
+                     subbox_index_to_multiindex, subbox_multiindex_to_index)
 from . import io  # noqa: F401
 
 __version__ = '0.1.0'

@@ -45,6 +45,8 @@ _SIGS = {
     'bsk_shells_workspace': [_vp, _c_int, _c_int, _c_int, _c_size_p],
     'bsk_shells': [_vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp,
                    _vp, _c_ll, _c_ll, _vp, _vp],
+    'bsk_power_spectrum_workspace': [_vp, _c_size_p],
+    'bsk_power_spectrum': [_vp, _c_int, _vp, _vp, _vp, _vp],
     'bsk_triples_plain_workspace': [_c_int, _c_int, _c_size_p],
     'bsk_triples_plain': [_c_int, _vp, _vp, _vp, _c_ll, _c_ll, _c_int, _vp, _c_int, _c_double, _vp,
                           _vp, _vp],

@@ -414,6 +414,33 @@ int bsk_shells(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int
 }
 
 // ---------------------------------------------------------------- triples
+static int spectrum_chunks(const bsk_plan* p) {
+  const long long total = (long long)p->g.MX * p->g.MY * p->g.MZ;
+  return (int)std::max(1LL, std::min(64LL, total / 4096));
+}
+
+int bsk_power_spectrum_workspace(const bsk_plan* p, size_t* bytes) {
+  if (int rc = check_plan(p)) return rc;
+  if (!bytes) {
+    bsk_set_error("bsk_power_spectrum_workspace: null output");
+    return BSK_EINVAL;
+  }
+  *bytes = sizeof(double) * 3 * (size_t)p->nbin * spectrum_chunks(p);
+  return BSK_OK;
+}
+
+int bsk_power_spectrum(bsk_plan* p, int dtype, const void* dk, double* out_dev, void* work,
+                       void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!dk || !out_dev || !work || (dtype != BSK_DT_F32 && dtype != BSK_DT_F64)) {
+    bsk_set_error("bsk_power_spectrum: invalid argument");
+    return BSK_EINVAL;
+  }
+  return bsk_spectrum_launch(dtype, dk, p->g, p->N, p->d_lo, p->d_hi, p->nbin,
+                             2.0 * M_PI / p->box, spectrum_chunks(p),
+                             reinterpret_cast<double*>(work), out_dev, as_stream(stream));
+}
+
 int bsk_triples_plain_workspace(int ntri, int nchunk, size_t* bytes) {
   *bytes = sizeof(double) * (size_t)std::max(ntri, 0) * std::max(nchunk, 1);
   return BSK_OK;

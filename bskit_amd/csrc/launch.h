@@ -76,3 +76,8 @@ size_t bsk_pair_lds_bytes(int sets, const int nsh[3], int nkb);
 int bsk_triple_pairs_launch(const PairArgs& a, int sets, int ni, hipStream_t s);
 int bsk_reduce_mfma_launch(const float* partial, int nwg, int items_per_wg, int ntri,
                            const int* slot_dev, double scale, double* out, hipStream_t s);
+
+// Power spectrum sums (spectrum.hip).
+int bsk_spectrum_launch(int dtype, const void* dk, Geom g, int N, const int* n2lo,
+                        const int* n2hi, int nbin, double kf, int nchunk, double* partial,
+                        double* out, hipStream_t s);

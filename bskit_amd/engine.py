@@ -444,6 +444,23 @@ class GridEngine:
         kbar[nz] = sums[nz, 1:4] / ntri[nz, None]
         return ntri, kbar
 
+    # -------------------------------------------------------- power spectrum
+    def power_spectrum(self, dk, stream=None, work=None):
+        """(P, N_modes, k_mean) per bin from the compact delta(k) cube (FP64)."""
+        dt = L.DT_F32 if dk.dtype == torch.complex64 else L.DT_F64
+        nbytes = L.size_out()
+        L.call('bsk_power_spectrum_workspace', self._plan, ctypes.byref(nbytes))
+        w = (work or self.work).get('pk', nbytes.value)
+        out = torch.empty(3 * self.nb, dtype=torch.float64, device=self.device)
+        L.call('bsk_power_spectrum', self._plan, dt, L.ptr(dk), L.ptr(out), L.ptr(w),
+               stream_handle(stream))
+        s = out.view(self.nb, 3).cpu().numpy()
+        nmodes = s[:, 1]
+        with np.errstate(divide='ignore', invalid='ignore'):
+            p = np.where(nmodes > 0, self.box ** 3 * s[:, 0] / nmodes, np.nan)
+            kmean = np.where(nmodes > 0, s[:, 2] / nmodes, np.nan)
+        return p, nmodes, kmean
+
     # ---------------------------------------------------------------- sizing
     def shell_bytes(self, fmt, nb=None):
         nb = self.nb if nb is None else nb

@@ -556,3 +556,33 @@ def bk_binned(mesh, bin0, bin1, bin2, verbose=0, periodic=False):
     """
     from .direct import direct_bispectrum
     return direct_bispectrum(mesh, bin0, bin1, bin2, periodic=periodic, verbose=verbose)
+
+
+# ---------------------------------------------------------------------------
+# Power spectrum (bskit/main.py:333-405).
+# ---------------------------------------------------------------------------
+def power_spectrum(mesh, k_edges, precision=None):
+    """Binned P(k), number of modes and mean |k| for each [kmin, kmax] row of
+    k_edges (closed bins), from one forward transform of the mesh.
+
+    The reference squares three shell fields per bin and sums over the cells
+    (pk_FFT, main.py:333-405); by Parseval the same sums are the mode sums of
+    |delta_k|^2, 1 and |k| over the bin, which one reduction over the compact
+    spectrum computes for every bin at once (bsk_power_spectrum).  Returns
+    float64 arrays (P, Nbin, k_mean).
+    """
+    m = cast_source(mesh)
+    N = int(m.attrs['Nmesh'][0])
+    box = float(m.attrs['BoxSize'][0])
+    kLU = np.asarray(k_edges, dtype=np.float64).reshape(-1, 2)
+    dev = require_device()
+    eng = get_engine(N, box, kLU, dev, int(m.compensation_window))
+    f = m.device_field(dev)
+    dtype = torch.float64 if precision == 'fp64' else (torch.float32 if precision else f.dtype)
+    return eng.power_spectrum(eng.forward(f.to(dtype)))
+
+
+def pk_FFT(mesh, kmin, kmax):
+    """Mean P(k), number of modes and mean k within one bin; main.py:333-405."""
+    p, n, k = power_spectrum(mesh, [[kmin, kmax]])
+    return float(p[0]), float(n[0]), float(k[0])

@@ -132,6 +132,18 @@ int bsk_shells(bsk_plan* plan, int dtype, int kind, const void* dk, int shell0, 
                int xslab, int stages, int fmt, void* out, void* out_lo, long long shell_stride,
                long long out_cell0, void* work, void* stream);
 
+/* Binned power spectrum sums of the plan's bins from the compact delta(k)
+ * cube (bsk_forward output): out_dev[3b + 0] = sum w |delta_k|^2,
+ * out_dev[3b + 1] = sum w (modes), out_dev[3b + 2] = sum w |k| over the modes
+ * of bin b, w counting both members of each conjugate pair.  By Parseval
+ * these are the cell sums / N^3 of the squared shell fields that pk_FFT
+ * forms (bskit/main.py:333-405): P = BoxSize^3 * out[3b] / out[3b+1],
+ * Nbin = out[3b+1], k_mean = out[3b+2] / out[3b+1].  FP64, fixed order.
+ * work: bsk_power_spectrum_workspace() bytes. */
+int bsk_power_spectrum_workspace(const bsk_plan* plan, size_t* bytes);
+int bsk_power_spectrum(bsk_plan* plan, int dtype, const void* dk, double* out_dev, void* work,
+                       void* stream);
+
 /* out[t] = scale * sum_x A[i_t](x) B[j_t](x) C[k_t](x) for tri_dev = (i,j,k)
  * per triangle (int32, device), one sweep over the cells per triangle in
  * the shells' own precision.  Replaces the per-triangle

@@ -149,3 +149,26 @@ def full_dk(field):
     """Full complex delta(k) (FFT index order, 1/N^3) for the direct enumerator."""
     field = np.asarray(field, dtype=np.float64)
     return np.fft.fftn(field) / field.size
+
+
+def pk_fft(field, box, kmin, kmax):
+    """(P, Nbin, k_mean) exactly as pk_FFT forms them (bskit/main.py:333-405):
+    Nbin = sum_x n^2 / N^3 with n the unnormalised c2r of the unit mask,
+    k_mean = sum_x kappa^2 / N^3 / Nbin with kappa the c2r of |k|^(1/2) on
+    the mask, P = sum_x d^2 L^3 / N^3 / Nbin with d the c2r of the masked
+    forward transform.  Literal real-space sums, FP64."""
+    field = np.asarray(field, dtype=np.float64)
+    N = field.shape[0]
+    kf = kf_of(box)
+    lo, hi = shell_limits([kmin], [kmax], kf)
+    n2 = n2_half(N)
+    m = (n2 >= lo[0]) & (n2 <= hi[0])
+    s = float(N) ** 3
+    n = np.fft.irfftn(m.astype(np.complex128), s=(N, N, N), axes=(0, 1, 2)) * s
+    nbin = np.sum(n * n) / s
+    kap = np.fft.irfftn(np.where(m, np.sqrt(kf * np.sqrt(n2.astype(np.float64))), 0.0)
+                        .astype(np.complex128), s=(N, N, N), axes=(0, 1, 2)) * s
+    kmean = np.sum(kap * kap) / s / nbin
+    d = np.fft.irfftn(np.where(m, forward(field), 0.0), s=(N, N, N), axes=(0, 1, 2)) * s
+    p = np.sum(d * d) * float(box) ** 3 / s / nbin
+    return p, nbin, kmean

@@ -297,3 +297,18 @@ def test_truncate_partial_and_cli_errors(tmp_path):
         measure.main(['gadget'] + base[1:] + ['--k_bin_info', '0.1', '0.5', '0.1'])
     with pytest.raises(ValueError, match='isos_mult'):
         measure.main(base + ['--k_bin_info', '0.1', '0.5', '0.1', '--triangle_type', 'isosceles'])
+
+
+def test_subbox_index_roundtrip_and_cuts():
+    from bskit_amd import subbox
+    for n in (1, 2, 3, 4):
+        for i in range(n ** 3):
+            m = subbox.subbox_index_to_multiindex(i, n)
+            assert m.dtype == np.float64
+            assert subbox.subbox_multiindex_to_index(m, n) == i
+    f = np.arange(8 ** 3, dtype=np.float64).reshape(8, 8, 8)
+    sm = subbox.field_subbox_pm(np.array([1., 0., 1.]), 2, f, BoxSize=100.)
+    assert np.array_equal(sm.array, f[4:8, 0:4, 4:8])
+    assert np.allclose(sm.attrs['BoxSize'], 50.) and list(sm.attrs['Nmesh']) == [4, 4, 4]
+    sl = subbox.field_subbox_slice(f, [(50., 100.), (0., 50.), (50., 100.)], BoxSize=100.)
+    assert np.array_equal(sl.array, sm.array)

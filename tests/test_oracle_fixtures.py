@@ -201,3 +201,22 @@ def test_shell_mesh_identity():
     sh = est.shells_on_mesh(dk, N, 16, lo, hi)
     got = est.triple_sums(sh, sh, sh, idx) / 16 ** 3
     assert not np.allclose(got, ref, rtol=1e-6)
+
+
+def test_pk_fft_oracle_closed_forms():
+    """Single cosine mode: P = L^3 (A/2)^2 * 2 / Nbin; Nbin = modes of the
+    shell (direct count); k_mean = mean |k| over those modes."""
+    N, box = 24, 300.
+    kf = est.kf_of(box)
+    x = np.arange(N) * box / N
+    X, Y, Z = np.meshgrid(x, x, x, indexing='ij')
+    A = 1.7
+    f = A * np.cos(kf * (3 * X + 4 * Y))            # |n| = 5
+    kmin, kmax = 4.5 * kf, 5.5 * kf
+    p, nbin, kmean = est.pk_fft(f, box, kmin, kmax)
+    g = np.fft.fftfreq(N, 1.0 / N)
+    nn = np.sqrt(g[:, None, None] ** 2 + g[None, :, None] ** 2 + g[None, None, :] ** 2)
+    sel = (nn * kf >= kmin) & (nn * kf <= kmax)
+    assert abs(nbin - sel.sum()) < 1e-9
+    assert np.isclose(kmean, np.mean(nn[sel]) * kf, rtol=1e-12)
+    assert np.isclose(p, box ** 3 * 2 * (A / 2) ** 2 / sel.sum(), rtol=1e-12)
