@@ -207,35 +207,48 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
 
 // --------------------------------------------------------- S3: z c2r + store
 // grid (ceil(xs*N/P), nshell).  Rows are (xl, y) of the slab.
-BSK_DEV uint32_t bf16_bits_rne(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
+//
+// c2r of length N = 2 NH by one NH-point complex transform: with X the half
+// spectrum (kz < MZ stored, the rest zero), s = X_k + conj(X_{NH-k}),
+// d = X_k - conj(X_{NH-k}) and w = exp(+2 pi i k / N),
+//   z_k = s + i w d,   z_{NH-k} = conj(s) + i conj(w d),
+// so one thread builds both from the same two loads; the transform of z gives
+// x_{2n} = Re z_n, x_{2n+1} = Im z_n.
+typedef float bsk_f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bsk_bf16x2 __attribute__((ext_vector_type(2)));
+
+// hi = rne_bf16(v), lo = rne_bf16(v - hi) of two values, packed per word.
+BSK_DEV void bf16_split2(float a, float b, uint32_t& hi, uint32_t& lo) {
+  const bsk_f32x2 v = {a, b};
+  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bsk_bf16x2));
+  const bsk_f32x2 r = {a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u)};
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bsk_bf16x2));
 }
 
+// Four consecutive reals at element offset off (a multiple of 4).
 template <typename T>
-BSK_DEV void store_pair(int fmt, void* out, void* out_lo, long long off, T a, T b);
+BSK_DEV void store_quad(int fmt, void* out, void* out_lo, long long off, cplx<T> u, cplx<T> v);
 
 template <>
-BSK_DEV void store_pair<float>(int fmt, void* out, void* out_lo, long long off, float a, float b) {
+BSK_DEV void store_quad<float>(int fmt, void* out, void* out_lo, long long off, cplx<float> u,
+                               cplx<float> v) {
   if (fmt == BSK_FMT_BF16X2) {
-    // hi = rne_bf16(v), lo = rne_bf16(v - hi) with v_cvt_pk_bf16_f32
-    typedef float f32x2 __attribute__((ext_vector_type(2)));
-    typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-    const f32x2 v = {a, b};
-    const uint32_t hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
-    const f32x2 r = {a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u)};
-    const uint32_t lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
-    reinterpret_cast<uint32_t*>(out)[off >> 1] = hi;
-    reinterpret_cast<uint32_t*>(out_lo)[off >> 1] = lo;
+    uint32_t h0, l0, h1, l1;
+    bf16_split2(u.x, u.y, h0, l0);
+    bf16_split2(v.x, v.y, h1, l1);
+    reinterpret_cast<uint2*>(out)[off >> 2] = make_uint2(h0, h1);
+    reinterpret_cast<uint2*>(out_lo)[off >> 2] = make_uint2(l0, l1);
   } else {
-    reinterpret_cast<float2*>(out)[off >> 1] = make_float2(a, b);
+    reinterpret_cast<float4*>(out)[off >> 2] = make_float4(u.x, u.y, v.x, v.y);
   }
 }
 
 template <>
-BSK_DEV void store_pair<double>(int fmt, void* out, void*, long long off, double a, double b) {
-  reinterpret_cast<double2*>(out)[off >> 1] = make_double2(a, b);
+BSK_DEV void store_quad<double>(int fmt, void* out, void*, long long off, cplx<double> u,
+                                cplx<double> v) {
+  double2* o = reinterpret_cast<double2*>(out) + (off >> 1);
+  o[0] = make_double2(u.x, u.y);
+  o[1] = make_double2(v.x, v.y);
 }
 
 template <typename T, int NH, int V, int P, int W>
@@ -245,6 +258,8 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
                                              long long shell_stride, int fmt) {
   constexpr int N = 2 * NH;
   constexpr int LD = pencil_ld<NH>();
+  constexpr int KH = NH / 2 + 1;  // k = 0..NH/2 (each with its mirror NH - k)
+  static_assert(NH % 2 == 0, "z c2r needs N divisible by 4");
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
@@ -252,33 +267,32 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   __syncthreads();
   const long long row0 = (long long)blockIdx.x * P;
   const long long bl = blockIdx.y;
-  const cplx<T>* src = in + bl * nrows * MZ;
-  for (int e = tid; e < P * NH; e += W) {
-    const int p = e / NH, k = e - p * NH;
-    const long long row = row0 + p;
-    cplx<T> z = {(T)0, (T)0};
-    if (row < nrows) {
-      const int j = NH - k;  // 1..NH
+  const cplx<T>* src = in + (bl * nrows + row0) * MZ;
+  for (int e = tid; e < P * KH; e += W) {
+    const int p = e / KH, k = e - p * KH;
+    const int j = NH - k;  // NH/2..NH
+    cplx<T> s = {(T)0, (T)0}, wd = {(T)0, (T)0};
+    if (row0 + p < nrows) {
+      const cplx<T>* r = src + (long long)p * MZ;
       cplx<T> xk = {(T)0, (T)0}, xj = {(T)0, (T)0};
-      if (k < MZ) xk = src[row * MZ + k];
-      if (j < MZ) xj = src[row * MZ + j];
-      cplx<T> xjc = cconj(xj);
-      cplx<T> s = cadd(xk, xjc);
-      cplx<T> d = csub(xk, xjc);
-      cplx<T> w = cconj(tw[k]);  // exp(+2 pi i k / N)
-      cplx<T> wd = cmul(w, d);
-      z = {s.x - wd.y, s.y + wd.x};  // s + i * w * d
+      if (k < MZ) xk = r[k];
+      if (j < MZ) xj = r[j];
+      const cplx<T> xjc = cconj(xj);
+      s = cadd(xk, xjc);
+      wd = cmul(cconj(tw[k]), csub(xk, xjc));  // exp(+2 pi i k / N) * d
     }
-    buf[p * LD + k] = z;
+    cplx<T>* b = buf + p * LD;
+    if (k < NH) b[k] = {s.x - wd.y, s.y + wd.x};      // s + i w d
+    if (j != k && j < NH) b[j] = {s.x + wd.y, wd.x - s.y};  // conj(s) + i conj(w d)
   }
   __syncthreads();
   fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
-  for (int e = tid; e < P * NH; e += W) {
-    const int p = e / NH, n = e - p * NH;
-    const long long row = row0 + p;
-    if (row >= nrows) continue;
-    const cplx<T> z = buf[p * LD + n];
-    const long long off = bl * shell_stride + cell0 + row * N + 2 * n;
-    store_pair<T>(fmt, out, out_lo, off, z.x, z.y);
+  // two complex values (four reals) per thread and store
+  const long long obase = bl * shell_stride + cell0 + row0 * N;
+  for (int e = tid; e < P * (NH / 2); e += W) {
+    const int p = e / (NH / 2), q = e - p * (NH / 2);
+    if (row0 + p >= nrows) continue;
+    const cplx<T>* b = buf + p * LD + 2 * q;
+    store_quad<T>(fmt, out, out_lo, obase + (long long)p * N + 4 * q, b[0], b[1]);
   }
 }
