@@ -30,6 +30,21 @@ BSK_DEV cplx<T> mul_mi(cplx<T> a) {
   return {a.y, -a.x};
 }
 
+// Packed bf16 pairs for v_dot2_f32_bf16 (element 0 in the low half).  The
+// bits reach the instruction through a register: an empty asm hides them from
+// constant folding, so no packed-operand inline constant (whose bf16
+// interpretation is not the 32-bit pattern) is ever emitted; being
+// side-effect free, the move is hoisted out of loops.
+typedef __bf16 bsk_bf16x2_t __attribute__((ext_vector_type(2)));
+BSK_DEV bsk_bf16x2_t bf16_pair_bits(uint32_t bits) {
+  asm("" : "+v"(bits));
+  return __builtin_bit_cast(bsk_bf16x2_t, bits);
+}
+constexpr uint32_t kBf16NegOneLo = 0x0000bf80u;  // (-1, 0)
+constexpr uint32_t kBf16NegOneHi = 0xbf800000u;  // (0, -1)
+constexpr uint32_t kBf16OneLo = 0x00003f80u;     // (1, 0)
+constexpr uint32_t kBf16OneHi = 0x3f800000u;     // (0, 1)
+
 // Status codes, dtypes, formats and kinds come from the public header.
 #include "../../include/bskhip.h"
 

@@ -83,7 +83,7 @@ namespace {
 template <typename T, int N, int V>
 void fwd_y_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, hipStream_t s) {
   constexpr int P = pxy<T, N>();
-  hipLaunchKernelGGL((k_fwd_y<T, N, V, P, kW>), dim3(N, cdiv(g.MZ, P)), dim3(kW), 0, s, in, out,
+  hipLaunchKernelGGL((k_fwd_y<T, N, V, P, kW>), dim3(cdiv(g.MZ, P), N), dim3(kW), 0, s, in, out,
                      tw, g);
 }
 }  // namespace
@@ -106,7 +106,7 @@ template <typename T, int N, int V>
 void fwd_x_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, T scale, const T* comp,
              hipStream_t s) {
   constexpr int P = pxy<T, N>();
-  hipLaunchKernelGGL((k_fwd_x<T, N, V, P, kW>), dim3(g.MY, cdiv(g.MZ, P)), dim3(kW), 0, s, in,
+  hipLaunchKernelGGL((k_fwd_x<T, N, V, P, kW>), dim3(cdiv(g.MZ, P), g.MY), dim3(kW), 0, s, in,
                      out, tw, g, scale, comp);
 }
 }  // namespace
@@ -130,7 +130,7 @@ template <typename T, int N, int V>
 void inv_x_n(const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g, const int* lo,
              const int* hi, int shell0, int nsh, int kind, T kf, hipStream_t s) {
   constexpr int P = pxy<T, N>();
-  hipLaunchKernelGGL((k_inv_x_mask<T, N, V, P, kW>), dim3(g.MY, cdiv(g.MZ, P), nsh), dim3(kW),
+  hipLaunchKernelGGL((k_inv_x_mask<T, N, V, P, kW>), dim3(cdiv(g.MZ, P), g.MY, nsh), dim3(kW),
                      0, s, dk, out, tw, g, lo, hi, shell0, kind, kf);
 }
 }  // namespace
@@ -154,7 +154,7 @@ template <typename T, int N, int V>
 void inv_y_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0, int xs, int nsh,
              hipStream_t s) {
   constexpr int P = pxy<T, N>();
-  hipLaunchKernelGGL((k_inv_y<T, N, V, P, kW>), dim3(xs, cdiv(g.MZ, P), nsh), dim3(kW), 0, s, in,
+  hipLaunchKernelGGL((k_inv_y<T, N, V, P, kW>), dim3(cdiv(g.MZ, P), xs, nsh), dim3(kW), 0, s, in,
                      out, tw, g, x0, xs);
 }
 }  // namespace

@@ -11,7 +11,9 @@
 // Retained frequencies: along x and y the signed frequencies nlo..nlo+MX-1
 // (nlo = -m, MX = 2m+1, or the whole axis when 2m+1 >= N); along z 0..MZ-1.
 // Every pass keeps the kz index innermost, so the strided x and y passes read
-// and write P consecutive kz values (P*8 or P*16 bytes) per grid point.
+// and write P consecutive kz values (P*8 or P*16 bytes) per grid point; the
+// kz block is the fastest grid dimension, so workgroups running together
+// cover whole kz rows.
 #pragma once
 #include "fft.h"
 
@@ -72,8 +74,8 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += W) tw[i] = twg[i];
-  const long long x = blockIdx.x;
-  const int kz0 = blockIdx.y * P;
+  const long long x = blockIdx.y;
+  const int kz0 = blockIdx.x * P;
   for (int e = tid; e < P * N; e += W) {
     const int y = e / P, p = e - y * P, kz = kz0 + p;
     cplx<T> v = {(T)0, (T)0};
@@ -103,8 +105,8 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += W) tw[i] = twg[i];
-  const long long ry = blockIdx.x;
-  const int kz0 = blockIdx.y * P;
+  const long long ry = blockIdx.y;
+  const int kz0 = blockIdx.x * P;
   for (int e = tid; e < P * N; e += W) {
     const int x = e / P, p = e - x * P, kz = kz0 + p;
     cplx<T> v = {(T)0, (T)0};
@@ -138,8 +140,8 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += W) tw[i] = twg[i];
-  const long long ry = blockIdx.x;
-  const int kz0 = blockIdx.y * P;
+  const long long ry = blockIdx.y;
+  const int kz0 = blockIdx.x * P;
   const int bl = blockIdx.z;
   const int lo = n2lo[shell0 + bl], hi = n2hi[shell0 + bl];
   const int ny = g.nlo + (int)ry;
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
 }
 
 // -------------------------------------------------------- S2: inverse y FFT
-// grid (xs, ceil(MZ/P), nshell); x = x0 + blockIdx.x.
+// grid (ceil(MZ/P), xs, nshell); x = x0 + blockIdx.y.
 template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
                                              cplx<T>* __restrict__ out,
@@ -183,9 +185,9 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += W) tw[i] = twg[i];
-  const int xl = blockIdx.x;
+  const int xl = blockIdx.y;
   const long long x = x0 + xl;
-  const int kz0 = blockIdx.y * P;
+  const int kz0 = blockIdx.x * P;
   const long long bl = blockIdx.z;
   for (int e = tid; e < P * N; e += W) {
     const int yi = e / P, p = e - yi * P, kz = kz0 + p;
@@ -218,10 +220,13 @@ typedef float bsk_f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bsk_bf16x2 __attribute__((ext_vector_type(2)));
 
 // hi = rne_bf16(v), lo = rne_bf16(v - hi) of two values, packed per word.
+// (v - hi exactly by v_dot2_f32_bf16: hi.x * -1 + hi.y * 0 + v.)
 BSK_DEV void bf16_split2(float a, float b, uint32_t& hi, uint32_t& lo) {
   const bsk_f32x2 v = {a, b};
-  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bsk_bf16x2));
-  const bsk_f32x2 r = {a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u)};
+  const bsk_bf16x2 h = __builtin_convertvector(v, bsk_bf16x2);
+  const bsk_f32x2 r = {__builtin_amdgcn_fdot2_f32_bf16(h, bf16_pair_bits(kBf16NegOneLo), a, false),
+                       __builtin_amdgcn_fdot2_f32_bf16(h, bf16_pair_bits(kBf16NegOneHi), b, false)};
+  hi = __builtin_bit_cast(uint32_t, h);
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bsk_bf16x2));
 }
 

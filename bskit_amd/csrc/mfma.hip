@@ -47,15 +47,29 @@ BSK_DEV void static_for(F&& f) {
 }
 
 // hi = rne_bf16(v), lo = rne_bf16(v - hi), packed two values per word.
+// v - hi comes from v_dot2_f32_bf16 (hi.x * -1 + hi.y * 0 + v): one op per
+// value instead of an unpack and a subtraction, and exact (v - hi is
+// representable in FP32).
 BSK_DEV void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
   const f32x2 v = {a, b};
-  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
-  const f32x2 r = {a - __uint_as_float(hi << 16), b - __uint_as_float(hi & 0xffff0000u)};
+  const bf16x2 h = __builtin_convertvector(v, bf16x2);
+  const f32x2 r = {__builtin_amdgcn_fdot2_f32_bf16(h, bf16_pair_bits(kBf16NegOneLo), a, false),
+                   __builtin_amdgcn_fdot2_f32_bf16(h, bf16_pair_bits(kBf16NegOneHi), b, false)};
+  hi = __builtin_bit_cast(uint32_t, h);
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
 }
 
-BSK_DEV float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
-BSK_DEV float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+// hi + lo of the low / high bf16 of a pair of words, as one dot2 each.
+BSK_DEV float bf_sum_lo(uint32_t h, uint32_t l) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h),
+                                         bf16_pair_bits(kBf16OneLo), __uint_as_float(l << 16),
+                                         false);
+}
+BSK_DEV float bf_sum_hi(uint32_t h, uint32_t l) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h),
+                                         bf16_pair_bits(kBf16OneHi),
+                                         __uint_as_float(l & 0xffff0000u), false);
+}
 
 constexpr int kW = kPairWaves;   // waves per workgroup
 constexpr int kNT = 64 * kW;
@@ -100,10 +114,10 @@ BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l) {
       const uint4 hv = p.h[k], lv = p.l[k];
       if (f) {
         float4* dst = reinterpret_cast<float4*>(f + row * kLDF + sg * 8);
-        dst[0] = make_float4(bf_lo(hv.x) + bf_lo(lv.x), bf_hi(hv.x) + bf_hi(lv.x),
-                             bf_lo(hv.y) + bf_lo(lv.y), bf_hi(hv.y) + bf_hi(lv.y));
-        dst[1] = make_float4(bf_lo(hv.z) + bf_lo(lv.z), bf_hi(hv.z) + bf_hi(lv.z),
-                             bf_lo(hv.w) + bf_lo(lv.w), bf_hi(hv.w) + bf_hi(lv.w));
+        dst[0] = make_float4(bf_sum_lo(hv.x, lv.x), bf_sum_hi(hv.x, lv.x),
+                             bf_sum_lo(hv.y, lv.y), bf_sum_hi(hv.y, lv.y));
+        dst[1] = make_float4(bf_sum_lo(hv.z, lv.z), bf_sum_hi(hv.z, lv.z),
+                             bf_sum_lo(hv.w, lv.w), bf_sum_hi(hv.w, lv.w));
       }
       if (h) {
         *reinterpret_cast<uint4*>(h + row * kLDH + sg * 8) = hv;
