@@ -27,6 +27,36 @@ struct Geom {
 
 BSK_DEV int signed_freq(int idx, int N) { return idx < N / 2 ? idx : idx - N; }
 
+// Global <-> LDS staging with the accesses of a batch in flight together.
+// Thread t handles the elements t, t + W, ...; a batch issues up to 16
+// loads (or LDS reads) into registers before the first is consumed, so a
+// workgroup waits for the memory latency once per batch, not per element.
+template <typename T, int TOTAL, int W, typename LoadF, typename StoreF>
+BSK_DEV void stage_in(LoadF&& load, StoreF&& store) {
+  constexpr int CNT = (TOTAL + W - 1) / W;
+  constexpr int B = CNT < 16 ? CNT : 16;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q0 = 0; q0 < CNT; q0 += B) {
+    cplx<T> v[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int e = tid + (q0 + q) * W;
+      v[q] = (q0 + q < CNT && (TOTAL % W == 0 || e < TOTAL)) ? load(e) : cplx<T>{(T)0, (T)0};
+    }
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int e = tid + (q0 + q) * W;
+      if (q0 + q < CNT && (TOTAL % W == 0 || e < TOTAL)) store(e, v[q]);
+    }
+  }
+}
+
+template <typename T, int N, int W>
+BSK_DEV void load_twiddles(cplx<T>* tw, const cplx<T>* __restrict__ twg) {
+  stage_in<T, N, W>([&](int i) { return twg[i]; }, [&](int i, cplx<T> v) { tw[i] = v; });
+}
+
 // ------------------------------------------------------------------ F1: z r2c
 // One workgroup = P rows (x,y) of N reals, viewed as NH = N/2 complex values.
 template <typename T, int NH, int V, int P, int W>
@@ -39,16 +69,18 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += W) tw[i] = twg[i];
+  load_twiddles<T, N, W>(tw, twg);
   const long long row0 = (long long)blockIdx.x * P;
-  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in);
-  for (int e = tid; e < P * NH; e += W) {
-    const int p = e / NH, n = e - p * NH;
-    const long long row = row0 + p;
-    cplx<T> v = {(T)0, (T)0};
-    if (row < nrows) v = src[row * NH + n];
-    buf[p * LD + n] = v;
-  }
+  const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in) + row0 * NH;
+  const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
+  stage_in<T, P * NH, W>(
+      [&](int e) {
+        return e < nr * NH ? src[e] : cplx<T>{(T)0, (T)0};
+      },
+      [&](int e, cplx<T> v) {
+        const int p = e / NH;
+        buf[p * LD + (e - p * NH)] = v;
+      });
   __syncthreads();
   fft_lds<T, NH, false, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
   for (int e = tid; e < P * MZ; e += W) {
@@ -72,19 +104,23 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += W) tw[i] = twg[i];
+  load_twiddles<T, N, W>(tw, twg);
   const long long x = blockIdx.y;
   const int kz0 = blockIdx.x * P;
-  for (int e = tid; e < P * N; e += W) {
-    const int y = e / P, p = e - y * P, kz = kz0 + p;
-    cplx<T> v = {(T)0, (T)0};
-    if (kz < g.MZ) v = in[(x * N + y) * g.MZ + kz];
-    buf[p * LD + y] = v;
-  }
+  const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;  // valid pencils
+  const cplx<T>* src = in + x * N * g.MZ + kz0;
+  stage_in<T, P * N, W>(
+      [&](int e) {
+        const int y = e / P, p = e - y * P;
+        return p < np ? src[(long long)y * g.MZ + p] : cplx<T>{(T)0, (T)0};
+      },
+      [&](int e, cplx<T> v) {
+        const int y = e / P, p = e - y * P;
+        buf[p * LD + y] = v;
+      });
   __syncthreads();
   fft_lds<T, N, false, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
-  for (int e = tid; e < P * g.MY; e += W) {
+  for (int e = threadIdx.x; e < P * g.MY; e += W) {
     const int r = e / P, p = e - r * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
     const int idx = (g.nlo + r + N) % N;
@@ -103,20 +139,25 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += W) tw[i] = twg[i];
+  load_twiddles<T, N, W>(tw, twg);
   const long long ry = blockIdx.y;
   const int kz0 = blockIdx.x * P;
-  for (int e = tid; e < P * N; e += W) {
-    const int x = e / P, p = e - x * P, kz = kz0 + p;
-    cplx<T> v = {(T)0, (T)0};
-    if (kz < g.MZ) v = in[((long long)x * g.MY + ry) * g.MZ + kz];
-    buf[p * LD + x] = v;
-  }
+  const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
+  const cplx<T>* src = in + ry * g.MZ + kz0;
+  const long long xstride = (long long)g.MY * g.MZ;
+  stage_in<T, P * N, W>(
+      [&](int e) {
+        const int x = e / P, p = e - x * P;
+        return p < np ? src[x * xstride + p] : cplx<T>{(T)0, (T)0};
+      },
+      [&](int e, cplx<T> v) {
+        const int x = e / P, p = e - x * P;
+        buf[p * LD + x] = v;
+      });
   __syncthreads();
   fft_lds<T, N, false, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
   const int ny = g.nlo + (int)ry;
-  for (int e = tid; e < P * g.MX; e += W) {
+  for (int e = threadIdx.x; e < P * g.MX; e += W) {
     const int r = e / P, p = e - r * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
     const int nx = g.nlo + r;
@@ -127,7 +168,7 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
 }
 
 // ------------------------------------------------- S1: masked inverse x FFT
-// grid (MY, ceil(MZ/P), nshell).  kind: delta / unit / |k|.
+// grid (ceil(MZ/P), MY, nshell).  kind: delta / unit / |k|.
 template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk,
                                                   cplx<T>* __restrict__ out,
@@ -138,35 +179,40 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += W) tw[i] = twg[i];
+  load_twiddles<T, N, W>(tw, twg);
   const long long ry = blockIdx.y;
   const int kz0 = blockIdx.x * P;
   const int bl = blockIdx.z;
   const int lo = n2lo[shell0 + bl], hi = n2hi[shell0 + bl];
   const int ny = g.nlo + (int)ry;
-  for (int e = tid; e < P * N; e += W) {
-    const int xi = e / P, p = e - xi * P, kz = kz0 + p;
-    const int nx = signed_freq(xi, N);
-    const int rx = nx - g.nlo;
-    cplx<T> v = {(T)0, (T)0};
-    if (kz < g.MZ && rx >= 0 && rx < g.MX) {
-      const int n2 = nx * nx + ny * ny + kz * kz;
-      if (n2 >= lo && n2 <= hi) {
-        if (kind == BSK_KIND_DELTA)
-          v = dk[(ry * g.MX + rx) * g.MZ + kz];
-        else if (kind == BSK_KIND_UNIT)
-          v.x = (T)1;
-        else
-          v.x = kf * sqrt((T)n2);
-      }
-    }
-    buf[p * LD + xi] = v;
-  }
+  const cplx<T>* src = dk + ry * g.MX * g.MZ;
+  stage_in<T, P * N, W>(
+      [&](int e) {
+        const int xi = e / P, p = e - xi * P, kz = kz0 + p;
+        const int nx = signed_freq(xi, N);
+        const int rx = nx - g.nlo;
+        cplx<T> v = {(T)0, (T)0};
+        if (kz < g.MZ && rx >= 0 && rx < g.MX) {
+          const int n2 = nx * nx + ny * ny + kz * kz;
+          if (n2 >= lo && n2 <= hi) {
+            if (kind == BSK_KIND_DELTA)
+              v = src[rx * g.MZ + kz];
+            else if (kind == BSK_KIND_UNIT)
+              v.x = (T)1;
+            else
+              v.x = kf * sqrt((T)n2);
+          }
+        }
+        return v;
+      },
+      [&](int e, cplx<T> v) {
+        const int xi = e / P, p = e - xi * P;
+        buf[p * LD + xi] = v;
+      });
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
   const long long obase = ((long long)bl * g.MY + ry) * N;
-  for (int e = tid; e < P * N; e += W) {
+  for (int e = threadIdx.x; e < P * N; e += W) {
     const int x = e / P, p = e - x * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
     out[(obase + x) * g.MZ + kz] = buf[p * LD + x];
@@ -183,24 +229,29 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
-  const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += W) tw[i] = twg[i];
+  load_twiddles<T, N, W>(tw, twg);
   const int xl = blockIdx.y;
   const long long x = x0 + xl;
   const int kz0 = blockIdx.x * P;
   const long long bl = blockIdx.z;
-  for (int e = tid; e < P * N; e += W) {
-    const int yi = e / P, p = e - yi * P, kz = kz0 + p;
-    const int ry = signed_freq(yi, N) - g.nlo;
-    cplx<T> v = {(T)0, (T)0};
-    if (kz < g.MZ && ry >= 0 && ry < g.MY)
-      v = in[((bl * g.MY + ry) * N + x) * g.MZ + kz];
-    buf[p * LD + yi] = v;
-  }
+  const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
+  const cplx<T>* src = in + (bl * g.MY * N + x) * g.MZ + kz0;
+  const long long rstride = (long long)N * g.MZ;
+  stage_in<T, P * N, W>(
+      [&](int e) {
+        const int yi = e / P, p = e - yi * P;
+        const int ry = signed_freq(yi, N) - g.nlo;
+        return (p < np && ry >= 0 && ry < g.MY) ? src[ry * rstride + p]
+                                                : cplx<T>{(T)0, (T)0};
+      },
+      [&](int e, cplx<T> v) {
+        const int yi = e / P, p = e - yi * P;
+        buf[p * LD + yi] = v;
+      });
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
   const long long obase = (bl * xs + xl) * N;
-  for (int e = tid; e < P * N; e += W) {
+  for (int e = threadIdx.x; e < P * N; e += W) {
     const int y = e / P, p = e - y * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
     out[(obase + y) * g.MZ + kz] = buf[p * LD + y];
@@ -268,27 +319,40 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += W) tw[i] = twg[i];
+  load_twiddles<T, N, W>(tw, twg);
   __syncthreads();
   const long long row0 = (long long)blockIdx.x * P;
   const long long bl = blockIdx.y;
   const cplx<T>* src = in + (bl * nrows + row0) * MZ;
-  for (int e = tid; e < P * KH; e += W) {
-    const int p = e / KH, k = e - p * KH;
-    const int j = NH - k;  // NH/2..NH
-    cplx<T> s = {(T)0, (T)0}, wd = {(T)0, (T)0};
-    if (row0 + p < nrows) {
+  const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
+  // (k, NH - k) pairs, 8 pairs (16 loads) in flight per batch
+  constexpr int TOT = P * KH;
+  constexpr int CNT = (TOT + W - 1) / W;
+  constexpr int B = CNT < 8 ? CNT : 8;
+#pragma unroll
+  for (int q0 = 0; q0 < CNT; q0 += B) {
+    cplx<T> xk[B], xj[B];
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int e = tid + (q0 + q) * W;
+      const int p = e / KH, k = e - p * KH, j = NH - k;
+      const bool ok = q0 + q < CNT && e < TOT && p < nr;
       const cplx<T>* r = src + (long long)p * MZ;
-      cplx<T> xk = {(T)0, (T)0}, xj = {(T)0, (T)0};
-      if (k < MZ) xk = r[k];
-      if (j < MZ) xj = r[j];
-      const cplx<T> xjc = cconj(xj);
-      s = cadd(xk, xjc);
-      wd = cmul(cconj(tw[k]), csub(xk, xjc));  // exp(+2 pi i k / N) * d
+      xk[q] = (ok && k < MZ) ? r[k] : cplx<T>{(T)0, (T)0};
+      xj[q] = (ok && j < MZ) ? r[j] : cplx<T>{(T)0, (T)0};
     }
-    cplx<T>* b = buf + p * LD;
-    if (k < NH) b[k] = {s.x - wd.y, s.y + wd.x};      // s + i w d
-    if (j != k && j < NH) b[j] = {s.x + wd.y, wd.x - s.y};  // conj(s) + i conj(w d)
+#pragma unroll
+    for (int q = 0; q < B; ++q) {
+      const int e = tid + (q0 + q) * W;
+      if (q0 + q >= CNT || e >= TOT) continue;
+      const int p = e / KH, k = e - p * KH, j = NH - k;
+      const cplx<T> xjc = cconj(xj[q]);
+      const cplx<T> s = cadd(xk[q], xjc);
+      const cplx<T> wd = cmul(cconj(tw[k]), csub(xk[q], xjc));  // exp(+2 pi i k / N) * d
+      cplx<T>* b = buf + p * LD;
+      if (k < NH) b[k] = {s.x - wd.y, s.y + wd.x};                // s + i w d
+      if (j != k && j < NH) b[j] = {s.x + wd.y, wd.x - s.y};      // conj(s) + i conj(w d)
+    }
   }
   __syncthreads();
   fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
