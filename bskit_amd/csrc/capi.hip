@@ -631,53 +631,77 @@ int bsk_pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int*
       tiles = t;
     }
   }
-  const int W = kPairWaves, cap = kPairMaxItems;
+  const int W = kPairWaves, cap = bsk_pair_item_cap(sets, nshell);
   const int ntile = (int)tiles.tile_mask.size();
   if (ntile >= 0xffff) {
     bsk_set_error("bsk_pairs_create: %d pair tiles exceed 65534; split the triangle list",
                   ntile);
     return BSK_EUNSUPPORTED;
   }
-  // tiles to (pass, wave) bins, longest first, each to the least loaded bin
-  std::vector<int> order(ntile);
-  for (int i = 0; i < ntile; ++i) order[i] = i;
+  // units: a tile with its kb blocks, split into groups of at most `cap`
+  // blocks (each group forms the tile's pair products once)
+  std::vector<int> unit_tile;
+  std::vector<unsigned> unit_mask;
+  for (int ti = 0; ti < ntile; ++ti) {
+    unsigned m = 0;
+    int n = 0;
+    for (int kb = 0; kb < nkb; ++kb) {
+      if (!(tiles.tile_mask[ti] >> kb & 1u)) continue;
+      m |= 1u << kb;
+      if (++n == cap) {
+        unit_tile.push_back(ti);
+        unit_mask.push_back(m);
+        m = 0;
+        n = 0;
+      }
+    }
+    if (m) {
+      unit_tile.push_back(ti);
+      unit_mask.push_back(m);
+    }
+  }
+  const int nunit = (int)unit_tile.size();
+  // units to (pass, wave) bins, longest first, each to the least loaded bin
+  std::vector<int> order(nunit);
+  for (int i = 0; i < nunit; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    return __builtin_popcount(tiles.tile_mask[x]) > __builtin_popcount(tiles.tile_mask[y]);
+    return __builtin_popcount(unit_mask[x]) > __builtin_popcount(unit_mask[y]);
   });
   int npass = std::max(1, (tiles.nitem + W * cap - 1) / (W * cap));
   std::vector<std::vector<int>> bins;
-  for (;; ++npass) {
+  for (;; ++npass) {  // ends: every unit fits an empty bin (popcount <= cap)
     bins.assign((size_t)npass * W, {});
     std::vector<int> load((size_t)npass * W, 0);
     bool ok = true;
-    for (int ti : order) {
-      const int n = __builtin_popcount(tiles.tile_mask[ti]);
+    for (int u : order) {
+      const int n = __builtin_popcount(unit_mask[u]);
       const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
       if (load[b] + n > cap) {
         ok = false;
         break;
       }
       load[b] += n;
-      bins[b].push_back(ti);
+      bins[b].push_back(u);
     }
     if (ok) break;
   }
   int maxload = 1;
   for (const auto& b : bins) {
     int l = 0;
-    for (int ti : b) l += __builtin_popcount(tiles.tile_mask[ti]);
+    for (int u : b) l += __builtin_popcount(unit_mask[u]);
     maxload = std::max(maxload, l);
   }
   const int ni = bsk_pair_items_round(maxload);
-  // item descriptors: tile | kb << 16 | (first item of a tile) << 31
+  // item descriptors: tile | kb << 16 | (first item of a unit) << 31
   std::vector<int> items((size_t)npass * W * ni);
   std::vector<int> item_of((size_t)ntile * nkb, -1);  // global item of (tile, kb)
   for (size_t b = 0; b < bins.size(); ++b) {
     int s = 0;
-    for (int ti : bins[b]) {
+    for (int u : bins[b]) {
+      const int ti = unit_tile[u];
       bool first = true;
       for (int kb = 0; kb < nkb; ++kb) {
-        if (!(tiles.tile_mask[ti] >> kb & 1u)) continue;
+        if (!(unit_mask[u] >> kb & 1u)) continue;
         const int g = (int)b * ni + s;
         items[g] = ti | (kb << 16) | (first ? (int)0x80000000u : 0);
         item_of[(size_t)ti * nkb + kb] = g;

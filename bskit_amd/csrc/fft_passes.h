@@ -12,8 +12,8 @@
 // (nlo = -m, MX = 2m+1, or the whole axis when 2m+1 >= N); along z 0..MZ-1.
 // Every pass keeps the kz index innermost, so the strided x and y passes read
 // and write P consecutive kz values (P*8 or P*16 bytes) per grid point; the
-// kz block is the fastest grid dimension, so workgroups running together
-// cover whole kz rows.
+// kz block is the fastest logical grid dimension and the XCD remap below keeps
+// the blocks of one kz row on one XCD, so they run together in one L2.
 #pragma once
 #include "fft.h"
 
@@ -26,6 +26,33 @@ struct Geom {
 };
 
 BSK_DEV int signed_freq(int idx, int N) { return idx < N / 2 ? idx : idx - N; }
+
+// The dispatcher deals workgroups round-robin to the 8 XCDs, each with its own
+// L2.  Neighbouring kz blocks share the cache lines at their edges (a kz row
+// is MZ*8 bytes, rarely a multiple of 128), so the linear block id is
+// remapped to give each XCD one contiguous range of logical blocks, walked in
+// dispatch order: the blocks of one row then meet in one L2.  The map is a
+// bijection of [0, total): XCDs below `tall` take `per` blocks, the rest one
+// fewer.
+BSK_DEV int xcd_linear(int pid, int total) {
+  const int per = (total + 7) >> 3;
+  const int tall = total - (per - 1) * 8;
+  const int xcd = pid & 7, local = pid >> 3;
+  return xcd < tall ? xcd * per + local : tall * per + (xcd - tall) * (per - 1) + local;
+}
+
+struct Blk {
+  int x, y, z;
+};
+
+// Logical (x, y, z) block of this workgroup under the XCD remap.
+BSK_DEV Blk xcd_block() {
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int pid = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int l = xcd_linear(pid, gx * gy * gridDim.z);
+  const int r = l / gx;
+  return Blk{l - r * gx, r % gy, r / gy};
+}
 
 // Global <-> LDS staging with the accesses of a batch in flight together.
 // Thread t handles the elements t, t + W, ...; a batch issues up to 16
@@ -105,8 +132,9 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   load_twiddles<T, N, W>(tw, twg);
-  const long long x = blockIdx.y;
-  const int kz0 = blockIdx.x * P;
+  const Blk bk = xcd_block();
+  const long long x = bk.y;
+  const int kz0 = bk.x * P;
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;  // valid pencils
   const cplx<T>* src = in + x * N * g.MZ + kz0;
   stage_in<T, P * N, W>(
@@ -140,8 +168,9 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   load_twiddles<T, N, W>(tw, twg);
-  const long long ry = blockIdx.y;
-  const int kz0 = blockIdx.x * P;
+  const Blk bk = xcd_block();
+  const long long ry = bk.y;
+  const int kz0 = bk.x * P;
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
   const cplx<T>* src = in + ry * g.MZ + kz0;
   const long long xstride = (long long)g.MY * g.MZ;
@@ -180,9 +209,10 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   load_twiddles<T, N, W>(tw, twg);
-  const long long ry = blockIdx.y;
-  const int kz0 = blockIdx.x * P;
-  const int bl = blockIdx.z;
+  const Blk bk = xcd_block();
+  const long long ry = bk.y;
+  const int kz0 = bk.x * P;
+  const int bl = bk.z;
   const int lo = n2lo[shell0 + bl], hi = n2hi[shell0 + bl];
   const int ny = g.nlo + (int)ry;
   const cplx<T>* src = dk + ry * g.MX * g.MZ;
@@ -220,7 +250,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
 }
 
 // -------------------------------------------------------- S2: inverse y FFT
-// grid (ceil(MZ/P), xs, nshell); x = x0 + blockIdx.y.
+// grid (ceil(MZ/P), xs, nshell); x = x0 + logical block y.
 template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
                                              cplx<T>* __restrict__ out,
@@ -230,10 +260,11 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   load_twiddles<T, N, W>(tw, twg);
-  const int xl = blockIdx.y;
+  const Blk bk = xcd_block();
+  const int xl = bk.y;
   const long long x = x0 + xl;
-  const int kz0 = blockIdx.x * P;
-  const long long bl = blockIdx.z;
+  const int kz0 = bk.x * P;
+  const long long bl = bk.z;
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
   const cplx<T>* src = in + (bl * g.MY * N + x) * g.MZ + kz0;
   const long long rstride = (long long)N * g.MZ;

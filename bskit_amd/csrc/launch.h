@@ -71,6 +71,7 @@ struct PairArgs {
   int item_total;         // items per workgroup in the partial layout
 };
 int bsk_pair_items_round(int ni);  // compiled items-per-wave count >= ni (0: too many)
+int bsk_pair_item_cap(int sets, int nshell);  // items per wave that compile without spills
 int bsk_pair_prefetch(int nshell);  // prefetch segments per thread (0: too many shells)
 size_t bsk_pair_lds_bytes(int sets, const int nsh[3], int nkb);
 int bsk_triple_pairs_launch(const PairArgs& a, int sets, int ni, hipStream_t s);

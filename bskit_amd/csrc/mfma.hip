@@ -131,14 +131,16 @@ BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l) {
 // 0, third side from set 2 (<AAB>), 3 = pairs from sets 0 and 1, third side
 // from set 2 (<ABC>).
 // Waves per SIMD the register allocation must allow: 4 (two workgroups per
-// CU, at most 128 VGPRs) where the item and prefetch registers fit, else 2.
-template <int NI, int PV>
+// CU, at most 128 VGPRs) where the item, prefetch and role-set registers fit
+// without spilling, else 2.
+template <int NI, int PV, int SETS>
 constexpr int pair_occupancy() {
-  return (NI <= 4 && PV <= 2) ? 4 : 2;
+  constexpr int ni_max = SETS == 1 ? 4 : (SETS == 2 ? 3 : 1);
+  return (NI <= ni_max && PV <= 2) ? 4 : 2;
 }
 
 template <int NI, int PV, int SETS>
-__global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV>())) void k_triple_pairs(PairArgs a) {
+__global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_triple_pairs(PairArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int n0 = a.nsh[0], n1 = a.nsh[1], n2 = a.nsh[2];
   float* fA = reinterpret_cast<float*>(smem);         // (n0 + 1) rows
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV>())) void k_triple_pair
       if (SETS == 3) fetch(p1, a.hi[1], a.lo[1], a.stride, n1, cb);
       if (SETS >= 2) fetch(p2, a.hi[2], a.lo[2], a.stride, n2, cb);
     }
-#pragma unroll 1
+#pragma unroll  // constant LDS offsets per K step
     for (int ks = 0; ks < kKSTEPS; ++ks) {
       const int c = ks * 32 + grp * 8;
       bf16x8 Ah, Al;
@@ -315,6 +317,13 @@ int bsk_pair_items_round(int ni) {
   if (ni <= 6) return 6;
   if (ni <= 8) return 8;
   return 0;
+}
+
+int bsk_pair_item_cap(int sets, int nshell) {
+  // with 4 prefetch segments per set, 8 (<AAB>) or 6 and 8 (<ABC>) items per
+  // wave exceed the register file at two waves per SIMD
+  if (bsk_pair_prefetch(nshell) == 4 && sets >= 2) return sets == 2 ? 6 : 4;
+  return kPairMaxItems;
 }
 
 int bsk_pair_prefetch(int nshell) {

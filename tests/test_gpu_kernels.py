@@ -276,18 +276,25 @@ def test_mfma_tail_shells(N, nb):
 
 
 @pytest.mark.parametrize('mode,nb,ntri', [(0, 40, 2000), (1, 24, 800), (2, 20, 900),
-                                          (0, 70, 3000)])
+                                          (0, 70, 3000), (2, 70, 0), (1, 100, 0)])
 def test_mfma_random_lists_match_fp64(mode, nb, ntri):
     """Random triangle lists (any side order, repeats, up to 70 shells) in every
-    role mode: split-BF16 MFMA against the FP64 plain path on the same fields."""
+    role mode: split-BF16 MFMA against the FP64 plain path on the same fields.
+    ntri = 0: a dense list whose pair tiles need more kb blocks than a wave
+    may hold at these shell counts, so tiles are split over waves."""
     from bskit_amd import _lib as L
     N, box = 64, 1000.0
     kf = 2 * np.pi / box
-    lo = kf * (1 + 0.4 * np.arange(nb))
-    hi = lo + 0.4 * kf
+    step, width = (0.4, 0.4) if nb <= 70 else (0.25, 0.5)  # overlapping: no empty shell
+    lo = kf * (1 + step * np.arange(nb))
+    hi = lo + width * kf
     eng = _engine(N, box, lo, hi)
     rng = np.random.default_rng(7 + mode)
-    tri = rng.integers(0, nb, size=(ntri, 3)).astype(np.int32)
+    if ntri:
+        tri = rng.integers(0, nb, size=(ntri, 3)).astype(np.int32)
+    else:
+        tri = np.array([(i, j, k) for i in range(5) for j in range(4) for k in range(0, nb, 3)],
+                       dtype=np.int32)
     nf = mode + 1
     fs = [_field(N, 30 + i, np.float64) for i in range(nf)]
     bf = [eng.shells(eng.forward(torch.from_numpy(f.astype(np.float32)).cuda()),
@@ -302,6 +309,7 @@ def test_mfma_random_lists_match_fp64(mode, nb, ntri):
     a, b, c = (s64[r].data for r in roles)
     absum = torch.stack([(a[i].abs() * b[j].abs() * c[k].abs()).sum() for i, j, k in tri])
     absum = absum.cpu().numpy() * sc
+    assert absum.min() > 0
     err = np.abs(got - ref) / absum
     plan = eng.pair_plan(tri, nb, mode)
     print('mode %d nb %d T %d pairs %d tiles %d items %d ni %d passes %d err %.2e'

@@ -182,6 +182,15 @@ def _emulate_pair_plan(plan, sets, tri, mode, nshell):
     """FP64 numpy emulation of the MFMA kernel's tables: per item the 16x16
     tile sum_x (d_a d_b)(x) d_k(x), read back through the triangle slots."""
     items, prs = plan.tables()
+    # the kernel re-forms pair products only at flagged items: an unflagged
+    # item continues the previous item's tile in the same wave's list
+    ni = plan.items_per_wave
+    for w in range(len(items) // ni):
+        lst = items[w * ni:(w + 1) * ni]
+        assert lst[0] < 0
+        for s in range(1, ni):
+            if lst[s] >= 0 and (lst[s] & 0xffff) != plan.ntile:  # padding items are unread
+                assert (lst[s] & 0xffff) == (lst[s - 1] & 0xffff)
     f0, f1, f2 = sets
     z = np.zeros_like(f0[0])
     nb = nshell
@@ -219,6 +228,36 @@ def test_pair_plan_tables_compute_every_triangle(mode, nshell, ntri, seed):
     tri = rng.integers(0, nshell, size=(ntri, 3))
     plan = PairPlan(tri, nshell, mode, upload=False)
     assert plan.npair <= ntri and plan.items_per_wave <= 8
+    sets = [fields[r] for r in roles]
+    got = _emulate_pair_plan(plan, sets, tri, mode, nshell)
+    ref = np.array([np.sum(sets[0][i] * sets[1][j] * sets[2][k]) for i, j, k in tri])
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize('mode,nshell', [(2, 70), (1, 100)])
+def test_pair_plan_splits_wide_tiles_over_waves(mode, nshell):
+    """Where the item count per wave is capped below a tile's kb blocks
+    (four prefetch segments per role set), the tile's items are split over
+    waves, each part forming the pair products again."""
+    from bskit_amd import _lib as L
+    from bskit_amd.engine import PairPlan
+    try:
+        L.lib()
+    except (ImportError, OSError) as e:
+        pytest.skip('library not loadable here: %s' % e)
+    tri = np.array([(i, j, k) for i in range(5) for j in range(4) for k in range(0, nshell, 3)])
+    plan = PairPlan(tri, nshell, mode, upload=False)
+    items, _ = plan.tables()
+    ni = plan.items_per_wave
+    waves = {}
+    for g, d in enumerate(items):
+        if (d & 0xffff) != plan.ntile:
+            waves.setdefault(d & 0xffff, set()).add(g // ni)
+    assert any(len(w) > 1 for w in waves.values())
+    rng = np.random.default_rng(8)
+    nset = {1: 2, 2: 3}[mode]
+    fields = [rng.standard_normal((nshell, 16)) for _ in range(nset)]
+    roles = {1: (0, 0, 1), 2: (0, 1, 2)}[mode]
     sets = [fields[r] for r in roles]
     got = _emulate_pair_plan(plan, sets, tri, mode, nshell)
     ref = np.array([np.sum(sets[0][i] * sets[1][j] * sets[2][k]) for i, j, k in tri])
