@@ -5,7 +5,7 @@
 //   E1           [x][y][kz]           after the z r2c, kz < MZ kept
 //   E2           [x][ry][kz]          after the y FFT, ry < MY kept
 //   D0 (delta_k) [ry][rx][kz]         after the x FFT, compact cube, scaled 1/N^3
-//   D1           [b][ry][x][kz]       after the masked inverse x FFT
+//   D1           [b][x][ry][kz]       after the masked inverse x FFT
 //   D2           [b][xl][y][kz]       after the inverse y FFT (x slab local)
 //   shells       [b][x][y][z]         after the z c2r, one N^3 array per bin
 // Retained frequencies: along x and y the signed frequencies nlo..nlo+MX-1
@@ -241,11 +241,12 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
       });
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
-  const long long obase = ((long long)bl * g.MY + ry) * N;
+  cplx<T>* dst = out + ((long long)bl * N * g.MY + ry) * g.MZ;
+  const long long xstride = (long long)g.MY * g.MZ;
   for (int e = threadIdx.x; e < P * N; e += W) {
     const int x = e / P, p = e - x * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
-    out[(obase + x) * g.MZ + kz] = buf[p * LD + x];
+    dst[x * xstride + kz] = buf[p * LD + x];
   }
 }
 
@@ -266,13 +267,12 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
   const int kz0 = bk.x * P;
   const long long bl = bk.z;
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
-  const cplx<T>* src = in + (bl * g.MY * N + x) * g.MZ + kz0;
-  const long long rstride = (long long)N * g.MZ;
+  const cplx<T>* src = in + (bl * N + x) * g.MY * g.MZ + kz0;
   stage_in<T, P * N, W>(
       [&](int e) {
         const int yi = e / P, p = e - yi * P;
         const int ry = signed_freq(yi, N) - g.nlo;
-        return (p < np && ry >= 0 && ry < g.MY) ? src[ry * rstride + p]
+        return (p < np && ry >= 0 && ry < g.MY) ? src[ry * g.MZ + p]
                                                 : cplx<T>{(T)0, (T)0};
       },
       [&](int e, cplx<T> v) {
@@ -310,6 +310,19 @@ BSK_DEV void bf16_split2(float a, float b, uint32_t& hi, uint32_t& lo) {
                        __builtin_amdgcn_fdot2_f32_bf16(h, bf16_pair_bits(kBf16NegOneHi), b, false)};
   hi = __builtin_bit_cast(uint32_t, h);
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bsk_bf16x2));
+}
+
+// The two z values of the mirrored pair (k, NH - k) of one row from
+// X_k = xk and X_{NH-k} = xj (zero where not retained); tw[t] = exp(-2 pi i t / N).
+template <typename T>
+BSK_DEV void c2r_pair(cplx<T>* row, int k, int NH, cplx<T> xk, cplx<T> xj,
+                      const cplx<T>* tw) {
+  const int j = NH - k;
+  const cplx<T> xjc = cconj(xj);
+  const cplx<T> s = cadd(xk, xjc);
+  const cplx<T> wd = cmul(cconj(tw[k]), csub(xk, xjc));  // exp(+2 pi i k / N) * d
+  if (k < NH) row[k] = {s.x - wd.y, s.y + wd.x};        // s + i w d
+  if (j != k && j < NH) row[j] = {s.x + wd.y, wd.x - s.y};  // conj(s) + i conj(w d)
 }
 
 // Four consecutive reals at element offset off (a multiple of 4).
@@ -376,13 +389,8 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
     for (int q = 0; q < B; ++q) {
       const int e = tid + (q0 + q) * W;
       if (q0 + q >= CNT || e >= TOT) continue;
-      const int p = e / KH, k = e - p * KH, j = NH - k;
-      const cplx<T> xjc = cconj(xj[q]);
-      const cplx<T> s = cadd(xk[q], xjc);
-      const cplx<T> wd = cmul(cconj(tw[k]), csub(xk[q], xjc));  // exp(+2 pi i k / N) * d
-      cplx<T>* b = buf + p * LD;
-      if (k < NH) b[k] = {s.x - wd.y, s.y + wd.x};                // s + i w d
-      if (j != k && j < NH) b[j] = {s.x + wd.y, wd.x - s.y};      // conj(s) + i conj(w d)
+      const int p = e / KH, k = e - p * KH;
+      c2r_pair<T>(buf + p * LD, k, NH, xk[q], xj[q], tw);
     }
   }
   __syncthreads();

@@ -63,7 +63,8 @@ def test_forward_full_cube(N):
 
 
 @pytest.mark.parametrize('N,nb,width,S', [(32, 5, 2, 32), (48, 6, 3, 48), (64, 8, 2, 64),
-                                          (64, 4, 2, 32), (96, 5, 2, 40), (64, 4, 2, 64)])
+                                          (64, 4, 2, 32), (96, 5, 2, 40), (64, 4, 2, 64),
+                                          (128, 12, 2, 80), (256, 4, 16, 200)])
 def test_delta_shells(N, nb, width, S):
     """Shell fields on the automatic shell mesh S (and on N when pinned)."""
     from bskit_amd import _lib as L
