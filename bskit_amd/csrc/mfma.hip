@@ -75,8 +75,16 @@ constexpr int kW = kPairWaves;   // waves per workgroup
 constexpr int kNT = 64 * kW;
 constexpr int kCK = kPairChunk;  // cells per staged chunk
 constexpr int kF = 8;            // K steps per FP32 accumulation chain
-constexpr int kLDF = kCK + 4;    // FP32 row stride (floats): +4 banks per row
-constexpr int kLDH = kCK + 8;    // bf16 row stride (elements): +4 banks per row
+// LDS layout (strides chosen by measured ds_read_b128 throughput on gfx950):
+//  * FP32 pair-side rows of 136 floats (544 B), cells in natural order; in a
+//    K step lane group g reads cells 4g..4g+3 and 16+4g..16+4g+3, the pattern
+//    that serves 16 arbitrary rows with the fewest bank conflicts;
+//  * bf16 third-side rows of 144 elements (288 B), each 32-cell K step stored
+//    permuted so that lane group g finds the same eight cells contiguous at
+//    8g: cell 4g+e at 8g+e and cell 16+4g+e at 8g+4+e (e < 4).  The MFMA K
+//    order is then the same permutation on both operands.
+constexpr int kLDF = kCK + 8;    // FP32 row stride (floats)
+constexpr int kLDH = kCK + 16;   // bf16 row stride (elements)
 constexpr int kSEGS = kCK / 8;   // 16-byte segments per row and plane
 constexpr int kKSTEPS = kCK / 32;
 
@@ -120,8 +128,13 @@ BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l) {
                              bf_sum_lo(hv.w, lv.w), bf_sum_hi(hv.w, lv.w));
       }
       if (h) {
-        *reinterpret_cast<uint4*>(h + row * kLDH + sg * 8) = hv;
-        *reinterpret_cast<uint4*>(l + row * kLDH + sg * 8) = lv;
+        // cells 8s+4q..8s+4q+3 of the K step (s = sg % 4, q = 0, 1) go to
+        // 16(s & 1) + 8q + 4(s >> 1)
+        const int base = row * kLDH + (sg & ~3) * 8 + 16 * (sg & 1) + 4 * ((sg >> 1) & 1);
+        *reinterpret_cast<uint2*>(h + base) = make_uint2(hv.x, hv.y);
+        *reinterpret_cast<uint2*>(h + base + 8) = make_uint2(hv.z, hv.w);
+        *reinterpret_cast<uint2*>(l + base) = make_uint2(lv.x, lv.y);
+        *reinterpret_cast<uint2*>(l + base + 8) = make_uint2(lv.z, lv.w);
       }
     }
   }
@@ -209,14 +222,15 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
     }
 #pragma unroll  // constant LDS offsets per K step
     for (int ks = 0; ks < kKSTEPS; ++ks) {
-      const int c = ks * 32 + grp * 8;
+      const int c = ks * 32 + grp * 4;   // FP32 rows: cells c..c+3, c+16..c+19
+      const int cb = ks * 32 + grp * 8;  // bf16 rows (permuted): eight at cb
       bf16x8 Ah, Al;
       static_for<0, NI>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         if (s == 0 || desc[s] < 0) {  // uniform: new tile -> new pair products
           const float4* pi = reinterpret_cast<const float4*>(fA + (oij[s] & 0xffff) + c);
           const float4* pj = reinterpret_cast<const float4*>(fB + ((unsigned)oij[s] >> 16) + c);
-          const float4 x0 = pi[0], x1 = pi[1], y0 = pj[0], y1 = pj[1];
+          const float4 x0 = pi[0], x1 = pi[4], y0 = pj[0], y1 = pj[4];
           uint32_t ah[4], al[4];
           split2(x0.x * y0.x, x0.y * y0.y, ah[0], al[0]);
           split2(x0.z * y0.z, x0.w * y0.w, ah[1], al[1]);
@@ -226,7 +240,7 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
           Al = __builtin_bit_cast(bf16x8, al);
         }
         const int kb = (desc[s] >> 16) & 0x7fff;
-        const int ko = (16 * kb + col) * kLDH + c;
+        const int ko = (16 * kb + col) * kLDH + cb;
         const bf16x8 Bh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(hK + ko));
         const bf16x8 Bl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(lK + ko));
         f32x4 t = acc[s];
