@@ -46,28 +46,38 @@ BSK_DEV void static_for(F&& f) {
   }
 }
 
+// The bf16 pairs (-1, 0), (0, -1), (1, 0), (0, 1) as dot2 operands, made once
+// per thread and kept in registers (as inline constants the packed pairs are
+// not encodable).
+struct PairConsts {
+  bf16x2 neg_lo, neg_hi, one_lo, one_hi;
+};
+
+BSK_DEV PairConsts pair_consts() {
+  return PairConsts{bf16_pair_bits(kBf16NegOneLo), bf16_pair_bits(kBf16NegOneHi),
+                    bf16_pair_bits(kBf16OneLo), bf16_pair_bits(kBf16OneHi)};
+}
+
 // hi = rne_bf16(v), lo = rne_bf16(v - hi), packed two values per word.
 // v - hi comes from v_dot2_f32_bf16 (hi.x * -1 + hi.y * 0 + v): one op per
 // value instead of an unpack and a subtraction, and exact (v - hi is
 // representable in FP32).
-BSK_DEV void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
+BSK_DEV void split2(float a, float b, uint32_t& hi, uint32_t& lo, const PairConsts& k) {
   const f32x2 v = {a, b};
   const bf16x2 h = __builtin_convertvector(v, bf16x2);
-  const f32x2 r = {__builtin_amdgcn_fdot2_f32_bf16(h, bf16_pair_bits(kBf16NegOneLo), a, false),
-                   __builtin_amdgcn_fdot2_f32_bf16(h, bf16_pair_bits(kBf16NegOneHi), b, false)};
+  const f32x2 r = {__builtin_amdgcn_fdot2_f32_bf16(h, k.neg_lo, a, false),
+                   __builtin_amdgcn_fdot2_f32_bf16(h, k.neg_hi, b, false)};
   hi = __builtin_bit_cast(uint32_t, h);
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
 }
 
 // hi + lo of the low / high bf16 of a pair of words, as one dot2 each.
-BSK_DEV float bf_sum_lo(uint32_t h, uint32_t l) {
-  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h),
-                                         bf16_pair_bits(kBf16OneLo), __uint_as_float(l << 16),
-                                         false);
+BSK_DEV float bf_sum_lo(uint32_t h, uint32_t l, const PairConsts& k) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h), k.one_lo,
+                                         __uint_as_float(l << 16), false);
 }
-BSK_DEV float bf_sum_hi(uint32_t h, uint32_t l) {
-  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h),
-                                         bf16_pair_bits(kBf16OneHi),
+BSK_DEV float bf_sum_hi(uint32_t h, uint32_t l, const PairConsts& k) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h), k.one_hi,
                                          __uint_as_float(l & 0xffff0000u), false);
 }
 
@@ -113,7 +123,8 @@ BSK_DEV void fetch(Pref<PV>& p, const uint16_t* __restrict__ gh, const uint16_t*
 // FP32 values d = hi + lo to f (if non-null) and raw planes to h/l (if
 // non-null) for rows < n.
 template <int PV>
-BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l) {
+BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l,
+                 const PairConsts& kc) {
 #pragma unroll
   for (int k = 0; k < PV; ++k) {
     const int v = threadIdx.x + k * kNT;
@@ -122,10 +133,10 @@ BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l) {
       const uint4 hv = p.h[k], lv = p.l[k];
       if (f) {
         float4* dst = reinterpret_cast<float4*>(f + row * kLDF + sg * 8);
-        dst[0] = make_float4(bf_sum_lo(hv.x, lv.x), bf_sum_hi(hv.x, lv.x),
-                             bf_sum_lo(hv.y, lv.y), bf_sum_hi(hv.y, lv.y));
-        dst[1] = make_float4(bf_sum_lo(hv.z, lv.z), bf_sum_hi(hv.z, lv.z),
-                             bf_sum_lo(hv.w, lv.w), bf_sum_hi(hv.w, lv.w));
+        dst[0] = make_float4(bf_sum_lo(hv.x, lv.x, kc), bf_sum_hi(hv.x, lv.x, kc),
+                             bf_sum_lo(hv.y, lv.y, kc), bf_sum_hi(hv.y, lv.y, kc));
+        dst[1] = make_float4(bf_sum_lo(hv.z, lv.z, kc), bf_sum_hi(hv.z, lv.z, kc),
+                             bf_sum_lo(hv.w, lv.w, kc), bf_sum_hi(hv.w, lv.w, kc));
       }
       if (h) {
         // cells 8s+4q..8s+4q+3 of the K step (s = sg % 4, q = 0, 1) go to
@@ -163,6 +174,7 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
   const int krows = 16 * a.nkb;
   uint16_t* lK = hK + krows * kLDH;
 
+  const PairConsts kc = pair_consts();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
@@ -210,9 +222,9 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
   }
   for (long long ch = ch0; ch < ch1; ++ch) {
     __syncthreads();  // every wave is done with the previous chunk
-    put(p0, n0, fA, SETS == 1 ? hK : nullptr, SETS == 1 ? lK : nullptr);
-    if (SETS == 3) put(p1, n1, fB, nullptr, nullptr);
-    if (SETS >= 2) put(p2, n2, nullptr, hK, lK);
+    put(p0, n0, fA, SETS == 1 ? hK : nullptr, SETS == 1 ? lK : nullptr, kc);
+    if (SETS == 3) put(p1, n1, fB, nullptr, nullptr, kc);
+    if (SETS >= 2) put(p2, n2, nullptr, hK, lK, kc);
     __syncthreads();
     if (ch + 1 < ch1) {  // next chunk's HBM loads overlap this chunk's math
       const long long cb = (ch + 1) * kCK;
@@ -232,10 +244,10 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
           const float4* pj = reinterpret_cast<const float4*>(fB + ((unsigned)oij[s] >> 16) + c);
           const float4 x0 = pi[0], x1 = pi[4], y0 = pj[0], y1 = pj[4];
           uint32_t ah[4], al[4];
-          split2(x0.x * y0.x, x0.y * y0.y, ah[0], al[0]);
-          split2(x0.z * y0.z, x0.w * y0.w, ah[1], al[1]);
-          split2(x1.x * y1.x, x1.y * y1.y, ah[2], al[2]);
-          split2(x1.z * y1.z, x1.w * y1.w, ah[3], al[3]);
+          split2(x0.x * y0.x, x0.y * y0.y, ah[0], al[0], kc);
+          split2(x0.z * y0.z, x0.w * y0.w, ah[1], al[1], kc);
+          split2(x1.x * y1.x, x1.y * y1.y, ah[2], al[2], kc);
+          split2(x1.z * y1.z, x1.w * y1.w, ah[3], al[3], kc);
           Ah = __builtin_bit_cast(bf16x8, ah);
           Al = __builtin_bit_cast(bf16x8, al);
         }
