@@ -8,6 +8,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
+#include <climits>
+#include <queue>
 #include <vector>
 
 #include "../../include/bskhip.h"
@@ -537,9 +540,12 @@ struct PairTiles {
 
 long long pkey(int a, int b) { return ((long long)a << 32) | (unsigned)b; }
 
-// Distinct pairs grouped by the set of third-side blocks they need, packed
-// into 16-row tiles (pairs of one group sorted by (a, b)).
-PairTiles build_tiles(const std::vector<Role>& roles) {
+// Distinct pairs sorted by the set of third-side blocks they need (then by
+// (a, b)) and packed into 16-row tiles in that order; a tile's items are the
+// union of its rows' blocks.  exact: a new tile at every change of the block
+// set (no MFMA on a block a row does not need); otherwise every tile is
+// filled (fewer tiles, so fewer pair products and pair-side LDS reads).
+PairTiles build_tiles(const std::vector<Role>& roles, bool exact) {
   std::vector<std::pair<long long, unsigned>> pm;
   pm.reserve(roles.size());
   for (const Role& r : roles) pm.push_back({pkey(r.a, r.b), 1u << (r.c / 16)});
@@ -558,14 +564,22 @@ PairTiles build_tiles(const std::vector<Role>& roles) {
   t.pairs = uniq;
   for (size_t i = 0; i < uniq.size();) {
     size_t j = i;
-    while (j < uniq.size() && uniq[j].second == uniq[i].second && j - i < 16) ++j;
+    unsigned m = 0;
+    while (j < uniq.size() && j - i < 16 && (!exact || uniq[j].second == uniq[i].second))
+      m |= uniq[j++].second;
     t.tile_of_pair_start.push_back((int)i);
-    t.tile_mask.push_back(uniq[i].second);
-    t.nitem += __builtin_popcount(uniq[i].second);
+    t.tile_mask.push_back(m);
+    t.nitem += __builtin_popcount(m);
     i = j;
   }
   return t;
 }
+
+// Relative kernel cost of a plan: a tile forms 16 pair products per cell
+// (VALU) and reads both pair sides from LDS; an item reads one third-side
+// block and issues three MFMAs.  Measured on gfx950 a tile costs about
+// three items.
+int plan_cost(const PairTiles& t) { return 3 * (int)t.tile_mask.size() + t.nitem; }
 
 std::vector<Role> assign_roles(int mode, int option, int ntri, const int* tri) {
   std::vector<Role> r(ntri);
@@ -582,6 +596,75 @@ std::vector<Role> assign_roles(int mode, int option, int ntri, const int* tri) {
     } else {
       r[t] = {i, j, k};
     }
+  }
+  return r;
+}
+
+// Auto spectra, pair chosen per triangle: a greedy cover of the triangles by
+// pairs (each step takes the pair that closes the most triangles not yet
+// covered, ties to the smaller key; lazy re-counting), then every triangle
+// takes the earliest chosen pair among those that close it.  Fewer distinct
+// pairs than any single side pairing for typical bin lists (the 512^3 bench
+// list: 233 against 334).
+std::vector<Role> assign_roles_cover(int ntri, const int* tri) {
+  std::vector<std::array<Role, 3>> cand(ntri);
+  std::vector<int> ncand(ntri, 0);
+  std::vector<std::pair<long long, int>> pt;  // (pair key, triangle)
+  pt.reserve((size_t)3 * ntri);
+  for (int t = 0; t < ntri; ++t) {
+    int v[3] = {tri[3 * t], tri[3 * t + 1], tri[3 * t + 2]};
+    std::sort(v, v + 3);
+    const Role opts[3] = {{v[2], v[1], v[0]}, {v[1], v[0], v[2]}, {v[2], v[0], v[1]}};
+    for (const Role& o : opts) {
+      bool dup = false;
+      for (int q = 0; q < ncand[t]; ++q)
+        dup |= cand[t][q].a == o.a && cand[t][q].b == o.b && cand[t][q].c == o.c;
+      if (dup) continue;
+      cand[t][ncand[t]++] = o;
+      pt.push_back({pkey(o.a, o.b), t});
+    }
+  }
+  std::sort(pt.begin(), pt.end());
+  std::vector<long long> keys;
+  std::vector<size_t> start;  // triangles of key i: pt[start[i] .. start[i+1])
+  for (size_t e = 0; e < pt.size(); ++e)
+    if (e == 0 || pt[e].first != pt[e - 1].first) {
+      keys.push_back(pt[e].first);
+      start.push_back(e);
+    }
+  start.push_back(pt.size());
+  std::vector<char> covered(ntri, 0);
+  std::vector<int> rank(keys.size(), -1);
+  std::priority_queue<std::pair<int, long long>> heap;  // (count, -key index)
+  for (size_t i = 0; i < keys.size(); ++i)
+    heap.push({(int)(start[i + 1] - start[i]), -(long long)i});
+  int nrank = 0;
+  while (!heap.empty()) {
+    const auto top = heap.top();
+    heap.pop();
+    const size_t i = (size_t)(-top.second);
+    int cnt = 0;
+    for (size_t e = start[i]; e < start[i + 1]; ++e) cnt += !covered[pt[e].second];
+    if (cnt == 0) continue;
+    if (cnt < top.first) {
+      heap.push({cnt, top.second});
+      continue;
+    }
+    rank[i] = nrank++;
+    for (size_t e = start[i]; e < start[i + 1]; ++e) covered[pt[e].second] = 1;
+  }
+  std::vector<Role> r(ntri);
+  for (int t = 0; t < ntri; ++t) {
+    int best = -1, br = INT_MAX;
+    for (int q = 0; q < ncand[t]; ++q) {
+      const long long k = pkey(cand[t][q].a, cand[t][q].b);
+      const size_t i = (size_t)(std::lower_bound(keys.begin(), keys.end(), k) - keys.begin());
+      if (rank[i] >= 0 && rank[i] < br) {
+        br = rank[i];
+        best = q;
+      }
+    }
+    r[t] = cand[t][best];
   }
   return r;
 }
@@ -622,13 +705,17 @@ int bsk_pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int*
   // choice of the pair sides that needs the fewest items
   std::vector<Role> roles;
   PairTiles tiles;
-  const int nopt = mode == BSK_MODE_AUTO ? 3 : 1;
+  const int nopt = mode == BSK_MODE_AUTO ? 4 : 1;
+  bool first = true;
   for (int o = 0; o < nopt; ++o) {
-    std::vector<Role> r = assign_roles(mode, o, ntri, tri);
-    PairTiles t = build_tiles(r);
-    if (o == 0 || t.nitem < tiles.nitem) {
-      roles.swap(r);
-      tiles = t;
+    std::vector<Role> r = o == 3 ? assign_roles_cover(ntri, tri) : assign_roles(mode, o, ntri, tri);
+    for (int exact = 1; exact >= 0; --exact) {
+      PairTiles t = build_tiles(r, exact != 0);
+      if (first || plan_cost(t) < plan_cost(tiles)) {
+        roles = r;
+        tiles = t;
+        first = false;
+      }
     }
   }
   const int W = kPairWaves, cap = bsk_pair_item_cap(sets, nshell);
