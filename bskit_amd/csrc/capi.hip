@@ -748,26 +748,29 @@ int bsk_pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int*
     }
   }
   const int nunit = (int)unit_tile.size();
-  // units to (pass, wave) bins, longest first, each to the least loaded bin
+  // units to (pass, wave) bins, costliest first (forming a tile's products
+  // weighs three items), each to the cheapest bin with room for its items
+  auto ucost = [&](int u) { return 3 + __builtin_popcount(unit_mask[u]); };
   std::vector<int> order(nunit);
   for (int i = 0; i < nunit; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-    return __builtin_popcount(unit_mask[x]) > __builtin_popcount(unit_mask[y]);
-  });
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ucost(x) > ucost(y); });
   int npass = std::max(1, (tiles.nitem + W * cap - 1) / (W * cap));
   std::vector<std::vector<int>> bins;
   for (;; ++npass) {  // ends: every unit fits an empty bin (popcount <= cap)
     bins.assign((size_t)npass * W, {});
-    std::vector<int> load((size_t)npass * W, 0);
+    std::vector<int> load((size_t)npass * W, 0), cost((size_t)npass * W, 0);
     bool ok = true;
     for (int u : order) {
       const int n = __builtin_popcount(unit_mask[u]);
-      const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-      if (load[b] + n > cap) {
+      int b = -1;
+      for (int q = 0; q < (int)bins.size(); ++q)
+        if (load[q] + n <= cap && (b < 0 || cost[q] < cost[b])) b = q;
+      if (b < 0) {
         ok = false;
         break;
       }
       load[b] += n;
+      cost[b] += ucost(u);
       bins[b].push_back(u);
     }
     if (ok) break;
@@ -779,7 +782,8 @@ int bsk_pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int*
     maxload = std::max(maxload, l);
   }
   const int ni = bsk_pair_items_round(maxload);
-  // item descriptors: tile | kb << 16 | (first item of a unit) << 31
+  // item descriptors: tile | kb << 16 | padding << 29 | (first item of a
+  // unit: new pair products) << 31
   std::vector<int> items((size_t)npass * W * ni);
   std::vector<int> item_of((size_t)ntile * nkb, -1);  // global item of (tile, kb)
   for (size_t b = 0; b < bins.size(); ++b) {
@@ -796,8 +800,8 @@ int bsk_pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int*
         ++s;
       }
     }
-    for (; s < ni; ++s)  // padding: the zero tile against block 0
-      items[(int)b * ni + s] = ntile | (s == 0 ? (int)0x80000000u : 0);
+    for (; s < ni; ++s)  // padding: skipped by the kernel (the zero tile, block 0)
+      items[(int)b * ni + s] = ntile | (1 << 29) | (s == 0 ? (int)0x80000000u : 0);
   }
   // pair rows (padding rows point at the zero row of each set)
   std::vector<int> prs((size_t)(ntile + 1) * 16, nshell | (nshell << 16));

@@ -19,9 +19,10 @@
 //    side); one zero FP32 row per pair set serves padding rows;
 //  * wave w runs a fixed list of NI items (descriptors in SGPRs): lane
 //    (col = lane%16, grp = lane/16) forms the products of pair row col over
-//    cells 8*grp..8*grp+7 of each 32-cell K step (skipped when the item
-//    reuses the previous item's tile), loads its B column 16*kb + col, and
-//    issues three MFMAs;
+//    its eight cells of each 32-cell K step (skipped when the item reuses
+//    the previous item's tile), loads its B column 16*kb + col, and issues
+//    three MFMAs; padding items do nothing (keeping the third-side fragment
+//    live across items for reuse costs more in registers than it saves);
 //  * FP32 MFMA chains run for at most F K steps and are then added into an
 //    FP64 accumulator (long FP32 chains lose accuracy: per-region partial
 //    sums are spatially coherent and large against the cancelling total).
@@ -46,16 +47,15 @@ BSK_DEV void static_for(F&& f) {
   }
 }
 
-// The bf16 pairs (-1, 0), (0, -1), (1, 0), (0, 1) as dot2 operands, made once
-// per thread and kept in registers (as inline constants the packed pairs are
-// not encodable).
+// The bf16 pairs (-1, 0), (0, -1) as dot2 operands for the product splits,
+// made once per thread and kept in registers (as inline constants the packed
+// pairs are not encodable).
 struct PairConsts {
-  bf16x2 neg_lo, neg_hi, one_lo, one_hi;
+  bf16x2 neg_lo, neg_hi;
 };
 
 BSK_DEV PairConsts pair_consts() {
-  return PairConsts{bf16_pair_bits(kBf16NegOneLo), bf16_pair_bits(kBf16NegOneHi),
-                    bf16_pair_bits(kBf16OneLo), bf16_pair_bits(kBf16OneHi)};
+  return PairConsts{bf16_pair_bits(kBf16NegOneLo), bf16_pair_bits(kBf16NegOneHi)};
 }
 
 // hi = rne_bf16(v), lo = rne_bf16(v - hi), packed two values per word.
@@ -71,13 +71,14 @@ BSK_DEV void split2(float a, float b, uint32_t& hi, uint32_t& lo, const PairCons
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
 }
 
-// hi + lo of the low / high bf16 of a pair of words, as one dot2 each.
-BSK_DEV float bf_sum_lo(uint32_t h, uint32_t l, const PairConsts& k) {
-  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h), k.one_lo,
+// hi + lo of the low / high bf16 of a pair of words, as one dot2 each (the
+// (1, 0) / (0, 1) operands are made per staged chunk, not kept live).
+BSK_DEV float bf_sum_lo(uint32_t h, uint32_t l, bf16x2 one_lo) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h), one_lo,
                                          __uint_as_float(l << 16), false);
 }
-BSK_DEV float bf_sum_hi(uint32_t h, uint32_t l, const PairConsts& k) {
-  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h), k.one_hi,
+BSK_DEV float bf_sum_hi(uint32_t h, uint32_t l, bf16x2 one_hi) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, h), one_hi,
                                          __uint_as_float(l & 0xffff0000u), false);
 }
 
@@ -123,8 +124,8 @@ BSK_DEV void fetch(Pref<PV>& p, const uint16_t* __restrict__ gh, const uint16_t*
 // FP32 values d = hi + lo to f (if non-null) and raw planes to h/l (if
 // non-null) for rows < n.
 template <int PV>
-BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l,
-                 const PairConsts& kc) {
+BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l) {
+  const bf16x2 olo = bf16_pair_bits(kBf16OneLo), ohi = bf16_pair_bits(kBf16OneHi);
 #pragma unroll
   for (int k = 0; k < PV; ++k) {
     const int v = threadIdx.x + k * kNT;
@@ -133,10 +134,10 @@ BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l,
       const uint4 hv = p.h[k], lv = p.l[k];
       if (f) {
         float4* dst = reinterpret_cast<float4*>(f + row * kLDF + sg * 8);
-        dst[0] = make_float4(bf_sum_lo(hv.x, lv.x, kc), bf_sum_hi(hv.x, lv.x, kc),
-                             bf_sum_lo(hv.y, lv.y, kc), bf_sum_hi(hv.y, lv.y, kc));
-        dst[1] = make_float4(bf_sum_lo(hv.z, lv.z, kc), bf_sum_hi(hv.z, lv.z, kc),
-                             bf_sum_lo(hv.w, lv.w, kc), bf_sum_hi(hv.w, lv.w, kc));
+        dst[0] = make_float4(bf_sum_lo(hv.x, lv.x, olo), bf_sum_hi(hv.x, lv.x, ohi),
+                             bf_sum_lo(hv.y, lv.y, olo), bf_sum_hi(hv.y, lv.y, ohi));
+        dst[1] = make_float4(bf_sum_lo(hv.z, lv.z, olo), bf_sum_hi(hv.z, lv.z, ohi),
+                             bf_sum_lo(hv.w, lv.w, olo), bf_sum_hi(hv.w, lv.w, ohi));
       }
       if (h) {
         // cells 8s+4q..8s+4q+3 of the K step (s = sg % 4, q = 0, 1) go to
@@ -222,9 +223,9 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
   }
   for (long long ch = ch0; ch < ch1; ++ch) {
     __syncthreads();  // every wave is done with the previous chunk
-    put(p0, n0, fA, SETS == 1 ? hK : nullptr, SETS == 1 ? lK : nullptr, kc);
-    if (SETS == 3) put(p1, n1, fB, nullptr, nullptr, kc);
-    if (SETS >= 2) put(p2, n2, nullptr, hK, lK, kc);
+    put(p0, n0, fA, SETS == 1 ? hK : nullptr, SETS == 1 ? lK : nullptr);
+    if (SETS == 3) put(p1, n1, fB, nullptr, nullptr);
+    if (SETS >= 2) put(p2, n2, nullptr, hK, lK);
     __syncthreads();
     if (ch + 1 < ch1) {  // next chunk's HBM loads overlap this chunk's math
       const long long cb = (ch + 1) * kCK;
@@ -239,7 +240,8 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
       bf16x8 Ah, Al;
       static_for<0, NI>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
-        if (s == 0 || desc[s] < 0) {  // uniform: new tile -> new pair products
+        if (desc[s] & (1 << 29)) return;  // uniform: padding item
+        if (s == 0 || desc[s] < 0) {      // uniform: new tile -> new pair products
           const float4* pi = reinterpret_cast<const float4*>(fA + (oij[s] & 0xffff) + c);
           const float4* pj = reinterpret_cast<const float4*>(fB + ((unsigned)oij[s] >> 16) + c);
           const float4 x0 = pi[0], x1 = pi[4], y0 = pj[0], y1 = pj[4];
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
           Ah = __builtin_bit_cast(bf16x8, ah);
           Al = __builtin_bit_cast(bf16x8, al);
         }
-        const int kb = (desc[s] >> 16) & 0x7fff;
+        const int kb = (desc[s] >> 16) & 0xff;
         const int ko = (16 * kb + col) * kLDH + cb;
         const bf16x8 Bh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(hK + ko));
         const bf16x8 Bl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(lK + ko));

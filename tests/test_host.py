@@ -184,20 +184,26 @@ def _emulate_pair_plan(plan, sets, tri, mode, nshell):
     items, prs = plan.tables()
     # the kernel re-forms pair products only at flagged items: an unflagged
     # item continues the previous item's tile in the same wave's list
+    # (flag 29: padding)
     ni = plan.items_per_wave
     for w in range(len(items) // ni):
         lst = items[w * ni:(w + 1) * ni]
         assert lst[0] < 0
         for s in range(1, ni):
-            if lst[s] >= 0 and (lst[s] & 0xffff) != plan.ntile:  # padding items are unread
+            if (lst[s] >> 29) & 1:
+                assert all((x >> 29) & 1 for x in lst[s:])  # padding closes the list
+                break
+            if lst[s] >= 0:
                 assert (lst[s] & 0xffff) == (lst[s - 1] & 0xffff)
     f0, f1, f2 = sets
     z = np.zeros_like(f0[0])
     nb = nshell
     part = np.zeros((len(items), 16, 16))
     for g, d in enumerate(items):
+        if (d >> 29) & 1:
+            continue  # padding
         tile = d & 0xffff
-        kb = (d >> 16) & 0x7fff
+        kb = (d >> 16) & 0xff
         for row in range(16):
             pr = prs[tile * 16 + row]
             a, b = pr & 0xffff, pr >> 16
