@@ -117,14 +117,14 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
     if (rc) return rc;
   }
   if (!(stages & BSK_STAGE_YZ)) return BSK_OK;
-  rc = bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, nsh, s);
+  rc = bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, nsh, p->d_hi, shell0, s);
   if (rc) return rc;
   // shells [shell0, shell0+nsh) of the output arrays, cells of the slab
   const size_t esz = (fmt == BSK_FMT_F64) ? 8 : (fmt == BSK_FMT_F32 ? 4 : 2);
   char* o = reinterpret_cast<char*>(out) + (size_t)shell0 * stride * esz;
   char* ol = out_lo ? reinterpret_cast<char*>(out_lo) + (size_t)shell0 * stride * esz : nullptr;
   return bsk_inv_z<T>(N, p->variant[5], d2, o, ol, tw, g.MZ, (long long)xs * N, cell0, stride, fmt,
-                      nsh, s);
+                      nsh, p->d_hi, shell0, s);
 }
 
 const cplx<float>* shell_tw(const bsk_plan* p, float) { return p->tws_f ? p->tws_f : p->tw_f; }

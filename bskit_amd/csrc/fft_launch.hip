@@ -152,47 +152,50 @@ template int bsk_inv_x<double>(int, int, const cplx<double>*, cplx<double>*, con
 namespace {
 template <typename T, int N, int V>
 void inv_y_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0, int xs, int nsh,
-             hipStream_t s) {
+             const int* n2hi, int shell0, hipStream_t s) {
   constexpr int P = pxy<T, N>();
   hipLaunchKernelGGL((k_inv_y<T, N, V, P, kW>), dim3(cdiv(g.MZ, P), xs, nsh), dim3(kW), 0, s, in,
-                     out, tw, g, x0, xs);
+                     out, tw, g, x0, xs, n2hi, shell0);
 }
 }  // namespace
 template <typename T>
 int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0,
-              int xs, int nsh, hipStream_t s) {
-#define C(n) BSK_VCASE(n, (inv_y_n<T, n, 0>(in, out, tw, g, x0, xs, nsh, s)), \
-                       (inv_y_n<T, n, 1>(in, out, tw, g, x0, xs, nsh, s)))
+              int xs, int nsh, const int* n2hi, int shell0, hipStream_t s) {
+#define C(n) BSK_VCASE(n, (inv_y_n<T, n, 0>(in, out, tw, g, x0, xs, nsh, n2hi, shell0, s)), \
+                       (inv_y_n<T, n, 1>(in, out, tw, g, x0, xs, nsh, n2hi, shell0, s)))
   BSK_SWITCH(T, N, V, C)
 #undef C
 }
 template int bsk_inv_y<float>(int, int, const cplx<float>*, cplx<float>*, const cplx<float>*, Geom,
-                              int, int, int, hipStream_t);
+                              int, int, int, const int*, int, hipStream_t);
 template int bsk_inv_y<double>(int, int, const cplx<double>*, cplx<double>*, const cplx<double>*,
-                               Geom, int, int, int, hipStream_t);
+                               Geom, int, int, int, const int*, int, hipStream_t);
 #endif
 
 #if BSK_PASS == 5
 namespace {
 template <typename T, int N, int V>
 void inv_z_n(const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
-             long long nrows, long long cell0, long long stride, int fmt, int nsh, hipStream_t s) {
+             long long nrows, long long cell0, long long stride, int fmt, int nsh,
+             const int* n2hi, int shell0, hipStream_t s) {
   constexpr int NH = N / 2, P = pz<T, NH>();
   hipLaunchKernelGGL((k_inv_z<T, NH, V, P, kW>), dim3(cdiv(nrows, P), nsh), dim3(kW), 0, s, in,
-                     out, out_lo, tw, MZ, nrows, cell0, stride, fmt);
+                     out, out_lo, tw, MZ, nrows, cell0, stride, fmt, n2hi, shell0);
 }
 }  // namespace
 template <typename T>
 int bsk_inv_z(int N, int V, const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
               long long nrows, long long cell0, long long stride, int fmt, int nsh,
-              hipStream_t s) {
-#define C(n) BSK_VCASE(n, (inv_z_n<T, n, 0>(in, out, out_lo, tw, MZ, nrows, cell0, stride, fmt, nsh, s)), \
-                       (inv_z_n<T, n, 1>(in, out, out_lo, tw, MZ, nrows, cell0, stride, fmt, nsh, s)))
+              const int* n2hi, int shell0, hipStream_t s) {
+#define C(n) BSK_VCASE(n, (inv_z_n<T, n, 0>(in, out, out_lo, tw, MZ, nrows, cell0, stride, fmt, nsh, n2hi, shell0, s)), \
+                       (inv_z_n<T, n, 1>(in, out, out_lo, tw, MZ, nrows, cell0, stride, fmt, nsh, n2hi, shell0, s)))
   BSK_SWITCH(T, N, V, C)
 #undef C
 }
 template int bsk_inv_z<float>(int, int, const cplx<float>*, void*, void*, const cplx<float>*, int,
-                              long long, long long, long long, int, int, hipStream_t);
+                              long long, long long, long long, int, int, const int*, int,
+                              hipStream_t);
 template int bsk_inv_z<double>(int, int, const cplx<double>*, void*, void*, const cplx<double>*,
-                               int, long long, long long, long long, int, int, hipStream_t);
+                               int, long long, long long, long long, int, int, const int*, int,
+                               hipStream_t);
 #endif

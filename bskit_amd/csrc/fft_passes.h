@@ -27,6 +27,27 @@ struct Geom {
 
 BSK_DEV int signed_freq(int idx, int N) { return idx < N / 2 ? idx : idx - N; }
 
+// Largest r with r*r <= v (-1 for v < 0).
+BSK_DEV int isqrt_floor(int v) {
+  if (v < 0) return -1;
+  int r = (int)sqrtf((float)v);
+  while (r * r > v) --r;
+  while ((r + 1) * (r + 1) <= v) ++r;
+  return r;
+}
+
+// Shell b holds only modes with |n|^2 <= n2hi[b], so after the inverse x
+// transform the pencil (ny, kz) of D1 is zero unless ny^2 + kz^2 <= n2hi[b],
+// and after the inverse y transform the kz values of D2 are zero from
+// mz_b = min(MZ, floor(sqrt(n2hi[b])) + 1) on.  The passes write and read
+// only those parts (readers take the rest as zero): the same bits, and for a
+// set of shells of radii up to m about a quarter of the D1 and half of the D2
+// traffic of the full rectangles.
+BSK_DEV int shell_mz(int MZ, int hi) {
+  const int r = isqrt_floor(hi) + 1;
+  return r < MZ ? r : MZ;
+}
+
 // The dispatcher deals workgroups round-robin to the 8 XCDs, each with its own
 // L2.  Neighbouring kz blocks share the cache lines at their edges (a kz row
 // is MZ*8 bytes, rarely a multiple of 128), so the linear block id is
@@ -197,7 +218,8 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
 }
 
 // ------------------------------------------------- S1: masked inverse x FFT
-// grid (ceil(MZ/P), MY, nshell).  kind: delta / unit / |k|.
+// grid (ceil(MZ/P), MY, nshell).  kind: delta / unit / |k|.  Workgroups whose
+// pencils all lie outside the shell's disk ny^2 + kz^2 <= n2hi write nothing.
 template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk,
                                                   cplx<T>* __restrict__ out,
@@ -208,13 +230,14 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
-  load_twiddles<T, N, W>(tw, twg);
   const Blk bk = xcd_block();
   const long long ry = bk.y;
   const int kz0 = bk.x * P;
   const int bl = bk.z;
   const int lo = n2lo[shell0 + bl], hi = n2hi[shell0 + bl];
   const int ny = g.nlo + (int)ry;
+  if (ny * ny + kz0 * kz0 > hi) return;  // uniform: every pencil is zero
+  load_twiddles<T, N, W>(tw, twg);
   const cplx<T>* src = dk + ry * g.MX * g.MZ;
   stage_in<T, P * N, W>(
       [&](int e) {
@@ -251,29 +274,34 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
 }
 
 // -------------------------------------------------------- S2: inverse y FFT
-// grid (ceil(MZ/P), xs, nshell); x = x0 + logical block y.
+// grid (ceil(MZ/P), xs, nshell); x = x0 + logical block y.  Reads only the
+// shell's disk of D1; kz blocks from mz_b on are all zero and not written.
 template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
                                              cplx<T>* __restrict__ out,
                                              const cplx<T>* __restrict__ twg, Geom g, int x0,
-                                             int xs) {
+                                             int xs, const int* __restrict__ n2hi,
+                                             int shell0) {
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
-  load_twiddles<T, N, W>(tw, twg);
   const Blk bk = xcd_block();
   const int xl = bk.y;
   const long long x = x0 + xl;
   const int kz0 = bk.x * P;
   const long long bl = bk.z;
+  const int hi = n2hi[shell0 + bl];
+  if (kz0 >= shell_mz(g.MZ, hi)) return;  // uniform: zero block, never read
+  load_twiddles<T, N, W>(tw, twg);
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
   const cplx<T>* src = in + (bl * N + x) * g.MY * g.MZ + kz0;
   stage_in<T, P * N, W>(
       [&](int e) {
         const int yi = e / P, p = e - yi * P;
-        const int ry = signed_freq(yi, N) - g.nlo;
-        return (p < np && ry >= 0 && ry < g.MY) ? src[ry * g.MZ + p]
-                                                : cplx<T>{(T)0, (T)0};
+        const int ny = signed_freq(yi, N), ry = ny - g.nlo, kz = kz0 + p;
+        return (p < np && ry >= 0 && ry < g.MY && ny * ny + kz * kz <= hi)
+                   ? src[ry * g.MZ + p]
+                   : cplx<T>{(T)0, (T)0};
       },
       [&](int e, cplx<T> v) {
         const int yi = e / P, p = e - yi * P;
@@ -355,7 +383,8 @@ template <typename T, int NH, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, void* out,
                                              void* out_lo, const cplx<T>* __restrict__ twg,
                                              int MZ, long long nrows, long long cell0,
-                                             long long shell_stride, int fmt) {
+                                             long long shell_stride, int fmt,
+                                             const int* __restrict__ n2hi, int shell0) {
   constexpr int N = 2 * NH;
   constexpr int LD = pencil_ld<NH>();
   constexpr int KH = NH / 2 + 1;  // k = 0..NH/2 (each with its mirror NH - k)
@@ -367,6 +396,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   __syncthreads();
   const long long row0 = (long long)blockIdx.x * P;
   const long long bl = blockIdx.y;
+  const int mzb = shell_mz(MZ, n2hi[shell0 + bl]);  // kz from mzb on are zero
   const cplx<T>* src = in + (bl * nrows + row0) * MZ;
   const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
   // (k, NH - k) pairs, 8 pairs (16 loads) in flight per batch
@@ -382,8 +412,8 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
       const int p = e / KH, k = e - p * KH, j = NH - k;
       const bool ok = q0 + q < CNT && e < TOT && p < nr;
       const cplx<T>* r = src + (long long)p * MZ;
-      xk[q] = (ok && k < MZ) ? r[k] : cplx<T>{(T)0, (T)0};
-      xj[q] = (ok && j < MZ) ? r[j] : cplx<T>{(T)0, (T)0};
+      xk[q] = (ok && k < mzb) ? r[k] : cplx<T>{(T)0, (T)0};
+      xj[q] = (ok && j < mzb) ? r[j] : cplx<T>{(T)0, (T)0};
     }
 #pragma unroll
     for (int q = 0; q < B; ++q) {

@@ -28,10 +28,11 @@ int bsk_inv_x(int N, int V, const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, 
               const int* lo, const int* hi, int shell0, int nsh, int kind, T kf, hipStream_t s);
 template <typename T>
 int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0,
-              int xs, int nsh, hipStream_t s);
+              int xs, int nsh, const int* n2hi, int shell0, hipStream_t s);
 template <typename T>
 int bsk_inv_z(int N, int V, const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
-              long long nrows, long long cell0, long long stride, int fmt, int nsh, hipStream_t s);
+              long long nrows, long long cell0, long long stride, int fmt, int nsh,
+              const int* n2hi, int shell0, hipStream_t s);
 
 // Triple products ------------------------------------------------------------
 // Reference-structured FP32/FP64 path: one sweep over all cells per triangle.
