@@ -93,8 +93,8 @@ int forward_t(bsk_plan* p, const T* field, cplx<T>* dk, void* work, hipStream_t 
   const int N = p->N;
   const Geom g = p->g;
   cplx<T>* e1 = reinterpret_cast<cplx<T>*>(work);
-  cplx<T>* e2 = e1 + (size_t)N * N * g.MZ;
-  int rc = bsk_fwd_z<T>(N, p->variant[0], field, e1, tw, (long long)N * N, g.MZ, s);
+  cplx<T>* e2 = e1 + (size_t)N * N * g.MZP;
+  int rc = bsk_fwd_z<T>(N, p->variant[0], field, e1, tw, (long long)N * N, g.MZ, g.MZP, s);
   if (rc) return rc;
   rc = bsk_fwd_y<T>(N, p->variant[1], e1, e2, tw, g, s);
   if (rc) return rc;
@@ -109,7 +109,7 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
   const int N = p->S;
   const Geom g = p->g;
   cplx<T>* d1 = reinterpret_cast<cplx<T>*>(work);
-  cplx<T>* d2 = d1 + (size_t)nsh * g.MY * N * g.MZ;
+  cplx<T>* d2 = d1 + (size_t)nsh * g.MY * N * g.MZP;
   const T kf = (T)(2.0 * M_PI / p->box);
   int rc = BSK_OK;
   if (stages & BSK_STAGE_X) {
@@ -123,7 +123,8 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
   const size_t esz = (fmt == BSK_FMT_F64) ? 8 : (fmt == BSK_FMT_F32 ? 4 : 2);
   char* o = reinterpret_cast<char*>(out) + (size_t)shell0 * stride * esz;
   char* ol = out_lo ? reinterpret_cast<char*>(out_lo) + (size_t)shell0 * stride * esz : nullptr;
-  return bsk_inv_z<T>(N, p->variant[5], d2, o, ol, tw, g.MZ, (long long)xs * N, cell0, stride, fmt,
+  return bsk_inv_z<T>(N, p->variant[5], d2, o, ol, tw, g.MZ, g.MZP, (long long)xs * N, cell0,
+                      stride, fmt,
                       nsh, p->d_hi, shell0, s);
 }
 
@@ -229,6 +230,7 @@ int bsk_plan_create(bsk_plan** out, int nmesh, double box_size, int nbin, const 
     g.nlo = -m;
   }
   g.MZ = std::min(m + 1, N / 2 + 1);
+  g.MZP = (g.MZ + 15) & ~15;
   p->g = g;
   p->S = N;
   for (int i = 0; i < 6; ++i) p->variant[i] = 0;
@@ -344,7 +346,7 @@ int bsk_forward_workspace(const bsk_plan* p, int dtype, size_t* bytes) {
   if (int rc = check_plan(p)) return rc;
   const size_t c = dtype == BSK_DT_F64 ? 16 : 8;
   const size_t N = p->N;
-  *bytes = c * (N * N * p->g.MZ + N * p->g.MY * p->g.MZ);
+  *bytes = c * (N * N * p->g.MZP + N * p->g.MY * p->g.MZP);
   return BSK_OK;
 }
 
@@ -368,7 +370,7 @@ int bsk_shells_workspace(const bsk_plan* p, int dtype, int nshell, int xslab, si
   if (int rc = check_plan(p)) return rc;
   const size_t c = dtype == BSK_DT_F64 ? 16 : 8;
   const size_t S = p->S;
-  *bytes = c * (size_t)nshell * p->g.MZ * ((size_t)p->g.MY * S + (size_t)xslab * S);
+  *bytes = c * (size_t)nshell * p->g.MZP * ((size_t)p->g.MY * S + (size_t)xslab * S);
   return BSK_OK;
 }
 

@@ -57,25 +57,25 @@ inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) /
 #if BSK_PASS == 0
 namespace {
 template <typename T, int N, int V>
-void fwd_z_n(const T* in, cplx<T>* out, const cplx<T>* tw, long long nrows, int MZ,
+void fwd_z_n(const T* in, cplx<T>* out, const cplx<T>* tw, long long nrows, int MZ, int MZP,
              hipStream_t s) {
   constexpr int NH = N / 2, P = pz<T, NH>();
   hipLaunchKernelGGL((k_fwd_z<T, NH, V, P, kW>), dim3(cdiv(nrows, P)), dim3(kW), 0, s, in, out,
-                     tw, nrows, MZ);
+                     tw, nrows, MZ, MZP);
 }
 }  // namespace
 template <typename T>
 int bsk_fwd_z(int N, int V, const T* in, cplx<T>* out, const cplx<T>* tw, long long nrows, int MZ,
-              hipStream_t s) {
-#define C(n) BSK_VCASE(n, (fwd_z_n<T, n, 0>(in, out, tw, nrows, MZ, s)), \
-                       (fwd_z_n<T, n, 1>(in, out, tw, nrows, MZ, s)))
+              int MZP, hipStream_t s) {
+#define C(n) BSK_VCASE(n, (fwd_z_n<T, n, 0>(in, out, tw, nrows, MZ, MZP, s)), \
+                       (fwd_z_n<T, n, 1>(in, out, tw, nrows, MZ, MZP, s)))
   BSK_SWITCH(T, N, V, C)
 #undef C
 }
 template int bsk_fwd_z<float>(int, int, const float*, cplx<float>*, const cplx<float>*, long long,
-                              int, hipStream_t);
+                              int, int, hipStream_t);
 template int bsk_fwd_z<double>(int, int, const double*, cplx<double>*, const cplx<double>*,
-                               long long, int, hipStream_t);
+                               long long, int, int, hipStream_t);
 #endif
 
 #if BSK_PASS == 1
@@ -175,27 +175,27 @@ template int bsk_inv_y<double>(int, int, const cplx<double>*, cplx<double>*, con
 #if BSK_PASS == 5
 namespace {
 template <typename T, int N, int V>
-void inv_z_n(const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
+void inv_z_n(const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ, int MZP,
              long long nrows, long long cell0, long long stride, int fmt, int nsh,
              const int* n2hi, int shell0, hipStream_t s) {
   constexpr int NH = N / 2, P = pz<T, NH>();
   hipLaunchKernelGGL((k_inv_z<T, NH, V, P, kW>), dim3(cdiv(nrows, P), nsh), dim3(kW), 0, s, in,
-                     out, out_lo, tw, MZ, nrows, cell0, stride, fmt, n2hi, shell0);
+                     out, out_lo, tw, MZ, MZP, nrows, cell0, stride, fmt, n2hi, shell0);
 }
 }  // namespace
 template <typename T>
 int bsk_inv_z(int N, int V, const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
-              long long nrows, long long cell0, long long stride, int fmt, int nsh,
+              int MZP, long long nrows, long long cell0, long long stride, int fmt, int nsh,
               const int* n2hi, int shell0, hipStream_t s) {
-#define C(n) BSK_VCASE(n, (inv_z_n<T, n, 0>(in, out, out_lo, tw, MZ, nrows, cell0, stride, fmt, nsh, n2hi, shell0, s)), \
-                       (inv_z_n<T, n, 1>(in, out, out_lo, tw, MZ, nrows, cell0, stride, fmt, nsh, n2hi, shell0, s)))
+#define C(n) BSK_VCASE(n, (inv_z_n<T, n, 0>(in, out, out_lo, tw, MZ, MZP, nrows, cell0, stride, fmt, nsh, n2hi, shell0, s)), \
+                       (inv_z_n<T, n, 1>(in, out, out_lo, tw, MZ, MZP, nrows, cell0, stride, fmt, nsh, n2hi, shell0, s)))
   BSK_SWITCH(T, N, V, C)
 #undef C
 }
 template int bsk_inv_z<float>(int, int, const cplx<float>*, void*, void*, const cplx<float>*, int,
-                              long long, long long, long long, int, int, const int*, int,
+                              int, long long, long long, long long, int, int, const int*, int,
                               hipStream_t);
 template int bsk_inv_z<double>(int, int, const cplx<double>*, void*, void*, const cplx<double>*,
-                               int, long long, long long, long long, int, int, const int*, int,
+                               int, int, long long, long long, long long, int, int, const int*, int,
                                hipStream_t);
 #endif

@@ -2,11 +2,11 @@
 //
 // HBM layouts (complex elements are interleaved re,im; all offsets 64-bit):
 //   real field   R [x][y][z]          N^3 reals
-//   E1           [x][y][kz]           after the z r2c, kz < MZ kept
-//   E2           [x][ry][kz]          after the y FFT, ry < MY kept
+//   E1           [x][y][kz]           after the z r2c, kz < MZ kept (stride MZP)
+//   E2           [x][ry][kz]          after the y FFT, ry < MY kept (stride MZP)
 //   D0 (delta_k) [ry][rx][kz]         after the x FFT, compact cube, scaled 1/N^3
-//   D1           [b][x][ry][kz]       after the masked inverse x FFT
-//   D2           [b][xl][y][kz]       after the inverse y FFT (x slab local)
+//   D1           [b][x][ry][kz]       after the masked inverse x FFT (stride MZP)
+//   D2           [b][xl][y][kz]       after the inverse y FFT, x slab local (MZP)
 //   shells       [b][x][y][z]         after the z c2r, one N^3 array per bin
 // Retained frequencies: along x and y the signed frequencies nlo..nlo+MX-1
 // (nlo = -m, MX = 2m+1, or the whole axis when 2m+1 >= N); along z 0..MZ-1.
@@ -23,6 +23,9 @@ struct Geom {
   int MY;   // retained y frequencies
   int MZ;   // retained z frequencies
   int nlo;  // first retained signed x/y frequency
+  int MZP;  // kz stride of the FFT intermediates E1, E2, D1, D2: MZ rounded
+            // up to 16, so the P = 16 (FP32) kz blocks of the strided passes
+            // are whole 128-B lines (the padding is never read or written)
 };
 
 BSK_DEV int signed_freq(int idx, int N) { return idx < N / 2 ? idx : idx - N; }
@@ -111,7 +114,7 @@ template <typename T, int NH, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
                                              cplx<T>* __restrict__ out,
                                              const cplx<T>* __restrict__ twg,
-                                             long long nrows, int MZ) {
+                                             long long nrows, int MZ, int MZP) {
   constexpr int N = 2 * NH;
   constexpr int LD = pencil_ld<NH>();
   __shared__ cplx<T> buf[P * LD];
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
     cplx<T> xe = cscale(cadd(zk, zc), (T)0.5);
     cplx<T> d = csub(zk, zc);
     cplx<T> xo = {(T)0.5 * d.y, (T)-0.5 * d.x};  // (zk - zc) / (2i)
-    out[row * MZ + k] = cadd(xe, cmul(tw[k], xo));
+    out[row * MZP + k] = cadd(xe, cmul(tw[k], xo));
   }
 }
 
@@ -157,11 +160,11 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
   const long long x = bk.y;
   const int kz0 = bk.x * P;
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;  // valid pencils
-  const cplx<T>* src = in + x * N * g.MZ + kz0;
+  const cplx<T>* src = in + x * N * g.MZP + kz0;
   stage_in<T, P * N, W>(
       [&](int e) {
         const int y = e / P, p = e - y * P;
-        return p < np ? src[(long long)y * g.MZ + p] : cplx<T>{(T)0, (T)0};
+        return p < np ? src[(long long)y * g.MZP + p] : cplx<T>{(T)0, (T)0};
       },
       [&](int e, cplx<T> v) {
         const int y = e / P, p = e - y * P;
@@ -173,7 +176,7 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
     const int r = e / P, p = e - r * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
     const int idx = (g.nlo + r + N) % N;
-    out[(x * g.MY + r) * g.MZ + kz] = buf[p * LD + idx];
+    out[(x * g.MY + r) * g.MZP + kz] = buf[p * LD + idx];
   }
 }
 
@@ -193,8 +196,8 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
   const long long ry = bk.y;
   const int kz0 = bk.x * P;
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
-  const cplx<T>* src = in + ry * g.MZ + kz0;
-  const long long xstride = (long long)g.MY * g.MZ;
+  const cplx<T>* src = in + ry * g.MZP + kz0;
+  const long long xstride = (long long)g.MY * g.MZP;
   stage_in<T, P * N, W>(
       [&](int e) {
         const int x = e / P, p = e - x * P;
@@ -264,8 +267,8 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
       });
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
-  cplx<T>* dst = out + ((long long)bl * N * g.MY + ry) * g.MZ;
-  const long long xstride = (long long)g.MY * g.MZ;
+  cplx<T>* dst = out + ((long long)bl * N * g.MY + ry) * g.MZP;
+  const long long xstride = (long long)g.MY * g.MZP;
   for (int e = threadIdx.x; e < P * N; e += W) {
     const int x = e / P, p = e - x * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
@@ -294,13 +297,13 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
   if (kz0 >= shell_mz(g.MZ, hi)) return;  // uniform: zero block, never read
   load_twiddles<T, N, W>(tw, twg);
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
-  const cplx<T>* src = in + (bl * N + x) * g.MY * g.MZ + kz0;
+  const cplx<T>* src = in + (bl * N + x) * g.MY * g.MZP + kz0;
   stage_in<T, P * N, W>(
       [&](int e) {
         const int yi = e / P, p = e - yi * P;
         const int ny = signed_freq(yi, N), ry = ny - g.nlo, kz = kz0 + p;
         return (p < np && ry >= 0 && ry < g.MY && ny * ny + kz * kz <= hi)
-                   ? src[ry * g.MZ + p]
+                   ? src[ry * g.MZP + p]
                    : cplx<T>{(T)0, (T)0};
       },
       [&](int e, cplx<T> v) {
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
   for (int e = threadIdx.x; e < P * N; e += W) {
     const int y = e / P, p = e - y * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
-    out[(obase + y) * g.MZ + kz] = buf[p * LD + y];
+    out[(obase + y) * g.MZP + kz] = buf[p * LD + y];
   }
 }
 
@@ -382,7 +385,7 @@ BSK_DEV void store_quad<double>(int fmt, void* out, void*, long long off, cplx<d
 template <typename T, int NH, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, void* out,
                                              void* out_lo, const cplx<T>* __restrict__ twg,
-                                             int MZ, long long nrows, long long cell0,
+                                             int MZ, int MZP, long long nrows, long long cell0,
                                              long long shell_stride, int fmt,
                                              const int* __restrict__ n2hi, int shell0) {
   constexpr int N = 2 * NH;
@@ -397,7 +400,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   const long long row0 = (long long)blockIdx.x * P;
   const long long bl = blockIdx.y;
   const int mzb = shell_mz(MZ, n2hi[shell0 + bl]);  // kz from mzb on are zero
-  const cplx<T>* src = in + (bl * nrows + row0) * MZ;
+  const cplx<T>* src = in + (bl * nrows + row0) * MZP;
   const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
   // (k, NH - k) pairs, 8 pairs (16 loads) in flight per batch
   constexpr int TOT = P * KH;
@@ -411,7 +414,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
       const int e = tid + (q0 + q) * W;
       const int p = e / KH, k = e - p * KH, j = NH - k;
       const bool ok = q0 + q < CNT && e < TOT && p < nr;
-      const cplx<T>* r = src + (long long)p * MZ;
+      const cplx<T>* r = src + (long long)p * MZP;
       xk[q] = (ok && k < mzb) ? r[k] : cplx<T>{(T)0, (T)0};
       xj[q] = (ok && j < mzb) ? r[j] : cplx<T>{(T)0, (T)0};
     }

@@ -16,7 +16,7 @@ int bsk_fft_size_above(int n);
 
 template <typename T>
 int bsk_fwd_z(int N, int V, const T* in, cplx<T>* out, const cplx<T>* tw, long long nrows, int MZ,
-              hipStream_t s);
+              int MZP, hipStream_t s);
 template <typename T>
 int bsk_fwd_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g,
               hipStream_t s);
@@ -31,7 +31,7 @@ int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, 
               int xs, int nsh, const int* n2hi, int shell0, hipStream_t s);
 template <typename T>
 int bsk_inv_z(int N, int V, const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
-              long long nrows, long long cell0, long long stride, int fmt, int nsh,
+              int MZP, long long nrows, long long cell0, long long stride, int fmt, int nsh,
               const int* n2hi, int shell0, hipStream_t s);
 
 // Triple products ------------------------------------------------------------
