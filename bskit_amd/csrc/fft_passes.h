@@ -333,40 +333,49 @@ typedef float bsk_f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bsk_bf16x2 __attribute__((ext_vector_type(2)));
 
 // hi = rne_bf16(v), lo = rne_bf16(v - hi) of two values, packed per word.
-// (v - hi exactly by v_dot2_f32_bf16: hi.x * -1 + hi.y * 0 + v.)
-BSK_DEV void bf16_split2(float a, float b, uint32_t& hi, uint32_t& lo) {
+// (v - hi exactly by v_dot2_f32_bf16: hi.x * -1 + hi.y * 0 + v; the bf16
+// pairs (-1, 0) and (0, -1) come in registers, made once per thread.)
+struct SplitConsts {
+  bsk_bf16x2 neg_lo, neg_hi;
+};
+
+BSK_DEV SplitConsts split_consts() {
+  return SplitConsts{bf16_pair_bits(kBf16NegOneLo), bf16_pair_bits(kBf16NegOneHi)};
+}
+
+BSK_DEV void bf16_split2(float a, float b, uint32_t& hi, uint32_t& lo, const SplitConsts& c) {
   const bsk_f32x2 v = {a, b};
   const bsk_bf16x2 h = __builtin_convertvector(v, bsk_bf16x2);
-  const bsk_f32x2 r = {__builtin_amdgcn_fdot2_f32_bf16(h, bf16_pair_bits(kBf16NegOneLo), a, false),
-                       __builtin_amdgcn_fdot2_f32_bf16(h, bf16_pair_bits(kBf16NegOneHi), b, false)};
+  const bsk_f32x2 r = {__builtin_amdgcn_fdot2_f32_bf16(h, c.neg_lo, a, false),
+                       __builtin_amdgcn_fdot2_f32_bf16(h, c.neg_hi, b, false)};
   hi = __builtin_bit_cast(uint32_t, h);
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bsk_bf16x2));
 }
 
 // The two z values of the mirrored pair (k, NH - k) of one row from
-// X_k = xk and X_{NH-k} = xj (zero where not retained); tw[t] = exp(-2 pi i t / N).
+// X_k = xk and X_{NH-k} = xj (zero where not retained); wk = exp(+2 pi i k / N).
 template <typename T>
-BSK_DEV void c2r_pair(cplx<T>* row, int k, int NH, cplx<T> xk, cplx<T> xj,
-                      const cplx<T>* tw) {
+BSK_DEV void c2r_pair(cplx<T>* row, int k, int NH, cplx<T> xk, cplx<T> xj, cplx<T> wk) {
   const int j = NH - k;
   const cplx<T> xjc = cconj(xj);
   const cplx<T> s = cadd(xk, xjc);
-  const cplx<T> wd = cmul(cconj(tw[k]), csub(xk, xjc));  // exp(+2 pi i k / N) * d
-  if (k < NH) row[k] = {s.x - wd.y, s.y + wd.x};        // s + i w d
+  const cplx<T> wd = cmul(wk, csub(xk, xjc));
+  if (k < NH) row[k] = {s.x - wd.y, s.y + wd.x};            // s + i w d
   if (j != k && j < NH) row[j] = {s.x + wd.y, wd.x - s.y};  // conj(s) + i conj(w d)
 }
 
 // Four consecutive reals at element offset off (a multiple of 4).
 template <typename T>
-BSK_DEV void store_quad(int fmt, void* out, void* out_lo, long long off, cplx<T> u, cplx<T> v);
+BSK_DEV void store_quad(int fmt, void* out, void* out_lo, long long off, cplx<T> u, cplx<T> v,
+                        const SplitConsts& c);
 
 template <>
 BSK_DEV void store_quad<float>(int fmt, void* out, void* out_lo, long long off, cplx<float> u,
-                               cplx<float> v) {
+                               cplx<float> v, const SplitConsts& c) {
   if (fmt == BSK_FMT_BF16X2) {
     uint32_t h0, l0, h1, l1;
-    bf16_split2(u.x, u.y, h0, l0);
-    bf16_split2(v.x, v.y, h1, l1);
+    bf16_split2(u.x, u.y, h0, l0, c);
+    bf16_split2(v.x, v.y, h1, l1, c);
     reinterpret_cast<uint2*>(out)[off >> 2] = make_uint2(h0, h1);
     reinterpret_cast<uint2*>(out_lo)[off >> 2] = make_uint2(l0, l1);
   } else {
@@ -376,12 +385,20 @@ BSK_DEV void store_quad<float>(int fmt, void* out, void* out_lo, long long off, 
 
 template <>
 BSK_DEV void store_quad<double>(int fmt, void* out, void*, long long off, cplx<double> u,
-                                cplx<double> v) {
+                                cplx<double> v, const SplitConsts&) {
   double2* o = reinterpret_cast<double2*>(out) + (off >> 1);
   o[0] = make_double2(u.x, u.y);
   o[1] = make_double2(v.x, v.y);
 }
 
+constexpr int pow2_at_least(int v) {
+  int q = 1;
+  while (q < v) q *= 2;
+  return q;
+}
+
+// Lanes map to (row, k) with a power-of-two lane count per row (KQ for the
+// KH mirrored pairs, QQ for the NH/2 output quads), so no index divides.
 template <typename T, int NH, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, void* out,
                                              void* out_lo, const cplx<T>* __restrict__ twg,
@@ -391,7 +408,10 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   constexpr int N = 2 * NH;
   constexpr int LD = pencil_ld<NH>();
   constexpr int KH = NH / 2 + 1;  // k = 0..NH/2 (each with its mirror NH - k)
+  constexpr int KQ = pow2_at_least(KH), RK = W / KQ, NPK = P / RK;
+  constexpr int QQ = pow2_at_least(NH / 2), RQ = W / QQ, NPQ = P / RQ;
   static_assert(NH % 2 == 0, "z c2r needs N divisible by 4");
+  static_assert(KQ <= W && P % RK == 0 && P % RQ == 0, "row mapping of the z c2r");
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
@@ -402,38 +422,40 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   const int mzb = shell_mz(MZ, n2hi[shell0 + bl]);  // kz from mzb on are zero
   const cplx<T>* src = in + (bl * nrows + row0) * MZP;
   const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
-  // (k, NH - k) pairs, 8 pairs (16 loads) in flight per batch
-  constexpr int TOT = P * KH;
-  constexpr int CNT = (TOT + W - 1) / W;
-  constexpr int B = CNT < 8 ? CNT : 8;
+  // (k, NH - k) pairs: lane k of rows r0, r0 + RK, ...; up to 8 rows (16
+  // loads) in flight per batch
+  const int k = tid & (KQ - 1), r0 = tid / KQ, j = NH - k;
+  const bool kact = k < KH;
+  const cplx<T> wk = kact ? cconj(tw[k]) : cplx<T>{(T)0, (T)0};  // exp(+2 pi i k / N)
+  constexpr int B = NPK < 8 ? NPK : 8;
 #pragma unroll
-  for (int q0 = 0; q0 < CNT; q0 += B) {
+  for (int q0 = 0; q0 < NPK; q0 += B) {
     cplx<T> xk[B], xj[B];
 #pragma unroll
     for (int q = 0; q < B; ++q) {
-      const int e = tid + (q0 + q) * W;
-      const int p = e / KH, k = e - p * KH, j = NH - k;
-      const bool ok = q0 + q < CNT && e < TOT && p < nr;
+      const int p = r0 + (q0 + q) * RK;
+      const bool ok = q0 + q < NPK && kact && p < nr;
       const cplx<T>* r = src + (long long)p * MZP;
       xk[q] = (ok && k < mzb) ? r[k] : cplx<T>{(T)0, (T)0};
       xj[q] = (ok && j < mzb) ? r[j] : cplx<T>{(T)0, (T)0};
     }
 #pragma unroll
-    for (int q = 0; q < B; ++q) {
-      const int e = tid + (q0 + q) * W;
-      if (q0 + q >= CNT || e >= TOT) continue;
-      const int p = e / KH, k = e - p * KH;
-      c2r_pair<T>(buf + p * LD, k, NH, xk[q], xj[q], tw);
-    }
+    for (int q = 0; q < B; ++q)
+      if (q0 + q < NPK && kact) c2r_pair<T>(buf + (r0 + (q0 + q) * RK) * LD, k, NH, xk[q], xj[q], wk);
   }
   __syncthreads();
   fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
-  // two complex values (four reals) per thread and store
+  // two complex values (four reals) per lane and store
+  const SplitConsts sc = split_consts();
   const long long obase = bl * shell_stride + cell0 + row0 * N;
-  for (int e = tid; e < P * (NH / 2); e += W) {
-    const int p = e / (NH / 2), q = e - p * (NH / 2);
-    if (row0 + p >= nrows) continue;
-    const cplx<T>* b = buf + p * LD + 2 * q;
-    store_quad<T>(fmt, out, out_lo, obase + (long long)p * N + 4 * q, b[0], b[1]);
+  const int q = tid & (QQ - 1), rq = tid / QQ;
+  if (q < NH / 2) {
+#pragma unroll
+    for (int i = 0; i < NPQ; ++i) {
+      const int p = rq + i * RQ;
+      if (p >= nr) continue;
+      const cplx<T>* b = buf + p * LD + 2 * q;
+      store_quad<T>(fmt, out, out_lo, obase + (long long)p * N + 4 * q, b[0], b[1], sc);
+    }
   }
 }
