@@ -116,7 +116,7 @@ def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
     from bskit_amd import _lib as L
     # L^6/N^3 sum over the N^3 mesh == L^6/S^3 sum over the S^3 shell mesh
     scale = wl['L'] ** 6 / float(eng.ncell)
-    shells = eng.alloc_shells(fmt, eng.nb)
+    shells = eng.alloc_shells(fmt, eng.nb) if path != 'zfused' else None
     out = torch.empty(len(wl['tri']), dtype=torch.float64, device='cuda')
     dtype = torch.float64 if fmt == L.FMT_F64 else torch.float32
     field = wl['field'].to(dtype)
@@ -127,11 +127,15 @@ def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
         dk = eng.forward(field)
         if t is not None:
             t.mark('fwd')
-        eng.shells(dk, fmt=fmt, dtype=dtype, out=shells)
-        if t is not None:
-            t.mark('shells')
-        kev = t.pair('kernel') if (t is not None and path == 'mfma') else None
-        eng.triples([shells], wl['tri'], path=path, scale=scale, out=out, kernel_events=kev)
+        kev = t.pair('kernel') if (t is not None and path in ('mfma', 'zfused')) else None
+        if path == 'zfused':  # shells x + y passes, then z c2r fused into the sums
+            eng.bispectrum_zfused(dk, wl['tri'], scale=scale, out=out, kernel_events=kev,
+                                  after_shells=(lambda: t.mark('shells')) if t else None)
+        else:
+            eng.shells(dk, fmt=fmt, dtype=dtype, out=shells)
+            if t is not None:
+                t.mark('shells')
+            eng.triples([shells], wl['tri'], path=path, scale=scale, out=out, kernel_events=kev)
         if t is not None:
             t.mark('triples')
         b = out / ntri_dev
@@ -248,6 +252,8 @@ def main():
     ap.add_argument('--no-serve', action='store_true', help='skip the serving (latency) run')
     ap.add_argument('--no-batch', action='store_true', help='skip the graph-batched cross run')
     ap.add_argument('--no-tune', action='store_true', help='skip the autotuner (default variants)')
+    ap.add_argument('--no-zfused', action='store_true',
+                    help='time the separate shell and triple-sum passes only')
     ap.add_argument('--jobs', type=int, default=2000, help='jobs in the serving run')
     ap.add_argument('--load', type=float, default=0.8, help='arrival rate / measured capacity')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
@@ -290,6 +296,7 @@ def main():
         t0 = time.perf_counter()
         tuned = eng.autotune(tri=wl['tri'], field=wl['field'], log=log)
         tuned = {'variants': tuned['variants'], 'mfma_nwg': tuned.get('mfma_nwg'),
+                 'zfused_nwg': tuned.get('zfused_nwg'), 'zfused': tuned.get('zfused'),
                  'seconds': time.perf_counter() - t0}
     t0 = time.perf_counter()
     ntri, kbar = eng.grid_info(wl['tri'])
@@ -298,12 +305,17 @@ def main():
     log('[bench] grid info %.2fs, N_tri range %d..%d' % (t_gridinfo, ntri.min(), ntri.max()))
     ntri_dev = torch.from_numpy(np.where(ntri > 0, ntri, np.nan)).cuda()
 
+    # main path: the z c2r fused into the triple-sum kernel where it is
+    # supported and the autotuner found it faster, else shells on the shell
+    # mesh + the pair kernel
+    zf_ok = not args.no_zfused and eng.zfused_supported(wl['tri'])
+    main_path = 'zfused' if zf_ok and eng.prefer_zfused else 'mfma'
     timer = StageTimer(torch)
-    b_mfma, el_mfma = run_path(eng, wl, 'mfma', L.FMT_BF16X2, args.steps, args.warmup, torch,
+    b_mfma, el_mfma = run_path(eng, wl, main_path, L.FMT_BF16X2, args.steps, args.warmup, torch,
                                ntri_dev, timer)
     ms_step = el_mfma / args.steps * 1e3
     grids_mfma = args.steps / el_mfma
-    log('[bench] mfma: %.3f ms/grid, %.2f grids/s' % (ms_step, grids_mfma))
+    log('[bench] %s: %.3f ms/grid, %.2f grids/s' % (main_path, ms_step, grids_mfma))
     stage_ms = {
         'forward': float(np.mean(timer.durations('start', 'fwd'))),
         'shells': float(np.mean(timer.durations('fwd', 'shells'))),
@@ -312,6 +324,18 @@ def main():
     }
     log('[bench] stages (ms): %s' % stage_ms)
     b_mfma = b_mfma.cpu().numpy()
+    alt = None
+    if zf_ok:  # the other implementation of the same step, reported beside it
+        alt_path = 'mfma' if main_path == 'zfused' else 'zfused'
+        ts = StageTimer(torch)
+        b_alt, el_alt = run_path(eng, wl, alt_path, L.FMT_BF16X2, args.steps, args.warmup,
+                                 torch, ntri_dev, ts)
+        alt = {'path': alt_path, 'ms_per_grid': el_alt / args.steps * 1e3,
+               'grids_per_s': args.steps / el_alt,
+               'shells_ms': float(np.mean(ts.durations('fwd', 'shells'))),
+               'triples_ms': float(np.mean(ts.durations('shells', 'triples'))),
+               'kernel_ms': float(np.mean(ts.pair_durations('kernel')))}
+        log('[bench] %s: %.3f ms/grid' % (alt_path, alt['ms_per_grid']))
 
     res_plain = None
     if not args.no_plain:
@@ -345,20 +369,34 @@ def main():
     nsh = eng.nb
     shell_bytes = nsh * 4 * ncell               # split-BF16 hi+lo, each value read once
     flops = 2.0 * T * ncell                      # one multiply-add per triangle per cell
-    kern_ms = float(np.mean(timer.pair_durations('kernel')))
-    t_k = kern_ms * 1e-3
-    ai = flops / shell_bytes
-    bound = 'hbm' if ai * HBM_PEAK_GBS * 1e9 < BF16_PEAK_TFLOPS * 1e12 else 'mfma'
-    achieved = shell_bytes / t_k / 1e9
     plan = eng.pair_plan(wl['tri'], nsh, 0)
-    roof = {'kernel': 'k_triple_pairs', 'bound': bound, 'achieved': achieved,
-            'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
-            'traffic': profiled_traffic('k_triple_pairs', N, eng.S),
-            'kernel_ms': kern_ms, 'algorithmic_bytes': shell_bytes,
-            'useful_tflops': flops / t_k / 1e12, 'arith_intensity_flop_per_byte': ai,
-            'pairs': plan.npair, 'tiles': plan.ntile, 'items': plan.nitem}
+
+    def roofline(kernel, kern_ms, alg_bytes):
+        t_k = kern_ms * 1e-3
+        ai = flops / alg_bytes
+        bound = 'hbm' if ai * HBM_PEAK_GBS * 1e9 < BF16_PEAK_TFLOPS * 1e12 else 'mfma'
+        achieved = alg_bytes / t_k / 1e9
+        return {'kernel': kernel, 'bound': bound, 'achieved': achieved,
+                'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                'traffic': profiled_traffic(kernel, N, eng.S),
+                'kernel_ms': kern_ms, 'algorithmic_bytes': alg_bytes,
+                'useful_tflops': flops / t_k / 1e12, 'arith_intensity_flop_per_byte': ai,
+                'pairs': plan.npair, 'tiles': plan.ntile, 'items': plan.nitem}
+
+    # the fused kernel reads each shell's inverse y output once: kz < mz_b
+    # complex values of every (x, y) row
+    mz = np.minimum(eng.MZ, np.floor(np.sqrt(np.maximum(eng.n2hi, 0))).astype(int) + 1)
+    d2_bytes = float(np.sum(mz)) * 8.0 * eng.S * eng.S
+
+    def path_roofline(path, ms):
+        return (roofline('k_triple_zrows', ms, d2_bytes) if path == 'zfused'
+                else roofline('k_triple_pairs', ms, shell_bytes))
+
+    roof = path_roofline(main_path, float(np.mean(timer.pair_durations('kernel'))))
+    if alt is not None:
+        alt['roofline'] = path_roofline(alt['path'], alt['kernel_ms'])
     H = eng.MX * eng.MY * eng.MZ                 # compact delta(k) cube
-    t_sh = stage_ms['shells'] * 1e-3
+    t_sh = (alt['shells_ms'] if main_path == 'zfused' else stage_ms['shells']) * 1e-3
     shells_alg = 8.0 * H + nsh * 4 * ncell       # delta(k) read + shells written
     stage_gbs = {'shells_GBps': shells_alg / t_sh / 1e9,
                  'forward_GBps': (4.0 * N ** 3 + 8.0 * H) / (stage_ms['forward'] * 1e-3) / 1e9}
@@ -404,11 +442,15 @@ def main():
                                 'of 2k_f from k_f; all+equilateral+isosceles(m=2)+squeezed(|k1|=k_f)'
                                 % (N, args.npart_side, wl['L'], args.nbin)),
                    'nmesh': N, 'shell_mesh': eng.S, 'nbin': args.nbin, 'nshell': nsh, 'ntri': T,
-                   'path': 'split-BF16 MFMA (hi*hi+hi*lo+lo*hi), FP32 accumulate, FP64 reduce',
+                   'path': ('split-BF16 MFMA (hi*hi+hi*lo+lo*hi), FP32 accumulate, FP64 '
+                            'reduce; %s' % ('z c2r fused into the triple sums'
+                                            if main_path == 'zfused' else
+                                            'shells on the shell mesh')),
                    'parallelism': 'single GPU'},
         'fp32_plain': res_plain,
         'speedup_vs_fp32_plain': (grids_mfma / res_plain['grids_per_s']) if res_plain else None,
         'stage_ms': stage_ms,
+        'other_path': alt,
         'stage_GBps': stage_gbs,
         'gridinfo_once_s': t_gridinfo,
         'autotune': tuned,

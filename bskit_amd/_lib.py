@@ -19,8 +19,8 @@ DT_F32, DT_F64 = 0, 1
 FMT_F32, FMT_BF16X2, FMT_F64 = 0, 1, 2
 KIND_DELTA, KIND_UNIT, KIND_KMAG = 0, 1, 2
 MODE_AUTO, MODE_AAB, MODE_ABC = 0, 1, 2
-STAGE_X, STAGE_YZ = 1, 2
-ABI_VERSION = 3
+STAGE_X, STAGE_YZ, STAGE_Y = 1, 2, 4
+ABI_VERSION = 4
 
 _c_int, _c_ll, _c_double, _vp = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 _c_size_p = ctypes.POINTER(ctypes.c_size_t)
@@ -63,6 +63,8 @@ _SIGS = {
     'bsk_triples_mfma': [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _c_int, _c_double, _vp,
                          _vp, _vp],
     'bsk_triples_mfma_reduce': [_vp, _c_int, _c_double, _vp, _vp, _vp],
+    'bsk_triples_zfused_supported': [_vp, _vp, ctypes.POINTER(_c_int)],
+    'bsk_triples_zfused': [_vp, _vp, _vp, _c_int, _c_int, _c_double, _vp, _vp, _vp],
     'bsk_cic_paint': [_c_int, _c_double, _c_ll, _vp, _vp, _c_int, _vp],
     'bsk_direct_triangles': [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp,
                              _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp],

@@ -116,6 +116,8 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
     rc = bsk_inv_x<T>(N, p->variant[3], dk, d1, tw, g, p->d_lo, p->d_hi, shell0, nsh, kind, kf, s);
     if (rc) return rc;
   }
+  if (stages & BSK_STAGE_Y)  // y only: D2 stays in the workspace
+    return bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, nsh, p->d_hi, shell0, s);
   if (!(stages & BSK_STAGE_YZ)) return BSK_OK;
   rc = bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, nsh, p->d_hi, shell0, s);
   if (rc) return rc;
@@ -381,7 +383,8 @@ int bsk_shells(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int
   const long long slab_cells = (long long)xslab * p->S * p->S;
   const bool yz = (stages & BSK_STAGE_YZ) != 0;
   if (nshell <= 0 || shell0 < 0 || shell0 + nshell > p->nbin || x0 < 0 || xslab <= 0 ||
-      x0 + xslab > p->S || !work || stages < 1 || stages > 3 ||
+      x0 + xslab > p->S || !work || stages < 1 || stages > 7 ||
+      ((stages & BSK_STAGE_Y) && (stages & BSK_STAGE_YZ)) ||
       ((stages & BSK_STAGE_X) && kind == BSK_KIND_DELTA && !dk) ||
       (kind < BSK_KIND_DELTA || kind > BSK_KIND_KMAG) ||
       (yz && (!out || out_cell0 < 0 || shell_stride < out_cell0 + slab_cells))) {
@@ -958,6 +961,78 @@ int bsk_triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* l
     if (rc) return rc;
   }
   if (!out_dev) return BSK_OK;  // partials only; bsk_triples_mfma_reduce finishes
+  return bsk_reduce_mfma_launch(a.partial, nwg, q->item_total, q->ntri, q->d_slot, scale,
+                                out_dev, s);
+}
+
+namespace {
+// Why the fused z/triple kernel cannot run this plan (nullptr: it can).
+const char* zfused_gap(const bsk_plan* p, const bsk_pairs* q) {
+  if (!p || !q) return "null plan";
+  if (q->mode != BSK_MODE_AUTO) return "auto spectra only";
+  if (q->nshell > kZMaxShells) return "too many shells";
+  if (q->ni < 1 || q->ni > 4) return "items per wave above 4";
+  bool size = false;
+#define C(n) size |= p->S == n;
+  BSK_ZFUSED_SIZES(C)
+#undef C
+  if (!size) return "no kernel for this shell mesh";
+  if (bsk_zrows_lds_bytes(p->S, q->nshell, q->nkb) > 160 * 1024) return "LDS";
+  return nullptr;
+}
+}  // namespace
+
+int bsk_triples_zfused_supported(const bsk_plan* p, const bsk_pairs* q, int* ok) {
+  if (!ok) {
+    bsk_set_error("bsk_triples_zfused_supported: null result");
+    return BSK_EINVAL;
+  }
+  *ok = zfused_gap(p, q) == nullptr;
+  return BSK_OK;
+}
+
+int bsk_triples_zfused(bsk_plan* p, const bsk_pairs* q, const void* shell_work, int shell0,
+                       int nwg, double scale, double* out_dev, void* work, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!q || !shell_work || !work || nwg <= 0 || shell0 < 0 || shell0 + q->nshell > p->nbin) {
+    bsk_set_error("bsk_triples_zfused: invalid argument");
+    return BSK_EINVAL;
+  }
+  if (const char* why = zfused_gap(p, q)) {
+    bsk_set_error("bsk_triples_zfused: unsupported plan (%s)", why);
+    return BSK_EUNSUPPORTED;
+  }
+  if (q->ntri == 0) return BSK_OK;
+  if (!q->d_items) {
+    bsk_set_error("bsk_triples_zfused: pair plan not uploaded (bsk_pairs_upload)");
+    return BSK_EINVAL;
+  }
+  const Geom& g = p->g;
+  const int S = p->S;
+  ZArgs a;
+  a.d2 = reinterpret_cast<const cplx<float>*>(shell_work) +
+         (size_t)q->nshell * g.MY * S * g.MZP;  // after D1 (shells_t layout)
+  a.nrows = (long long)S * S;
+  a.MZ = g.MZ;
+  a.MZP = g.MZP;
+  a.n2hi = p->d_hi;
+  a.shell0 = shell0;
+  a.nsh = q->nshell;
+  a.nkb = q->nkb;
+  a.nwg = nwg;
+  a.pairs = q->d_pairs;
+  a.partial = reinterpret_cast<float*>(work);
+  a.item_total = q->item_total;
+  a.tw = shell_tw(p, 0.f);
+  hipStream_t s = as_stream(stream);
+  for (int pass = 0; pass < q->npass; ++pass) {
+    a.item0 = pass * kPairWaves * q->ni;
+    a.items = q->d_items + a.item0;
+    const int rc = p->variant[5] ? bsk_triple_zrows_launch_v1(a, S, q->ni, s)
+                                 : bsk_triple_zrows_launch_v0(a, S, q->ni, s);
+    if (rc) return rc;
+  }
+  if (!out_dev) return BSK_OK;
   return bsk_reduce_mfma_launch(a.partial, nwg, q->item_total, q->ntri, q->d_slot, scale,
                                 out_dev, s);
 }

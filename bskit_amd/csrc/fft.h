@@ -170,6 +170,67 @@ BSK_DEV void fft_lds(cplx<T>* buf, const cplx<T>* tw, Radices<Rs...>) {
   stockham_run<T, L, INV, P, W, LD, TWN, 1, Rs...>(buf, tw);
 }
 
+// ------------------------------------------- runtime pencil count (np <= PMAX)
+// The same stages over np pencils known only at run time: butterfly b maps to
+// column j = b / np and pencil p = b - j*np, the quotient by a multiply-high
+// with mdiv = ceil(2^32 / np) (exact for b < 2^32 / np, far above PMAX * L).
+template <typename T, int L, int R, int NS, bool INV, int PMAX, int W, int LD, int TWN>
+BSK_DEV void stockham_stage_n(cplx<T>* buf, const cplx<T>* tw, int np, unsigned mdiv, int tid) {
+  constexpr int NB = L / R;
+  constexpr int BPT = (PMAX * NB + W - 1) / W;
+  const int tot = np * NB;
+  cplx<T> v[BPT][R];
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int b = tid + q * W;
+    if (b < tot) {
+      const int j = (int)__umulhi((unsigned)b, mdiv), p = b - j * np;
+      const cplx<T>* src = buf + p * LD + j;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = src[r * NB];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < BPT; ++q) {
+    const int b = tid + q * W;
+    if (b < tot) {
+      const int j = (int)__umulhi((unsigned)b, mdiv), p = b - j * np;
+      const int k = j % NS;
+      if (NS > 1) {
+        const int ks = k * (TWN / (NS * R));
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cplx<T> w = tw[ks * r];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      Dft<T, R, INV>::run(v[q]);
+      cplx<T>* dst = buf + p * LD + (j / NS) * NS * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[r * NS] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int L, bool INV, int PMAX, int W, int LD, int TWN, int NS, int R,
+          int... Rest>
+BSK_DEV void stockham_run_n(cplx<T>* buf, const cplx<T>* tw, int np, unsigned mdiv, int tid) {
+  stockham_stage_n<T, L, R, NS, INV, PMAX, W, LD, TWN>(buf, tw, np, mdiv, tid);
+  if constexpr (sizeof...(Rest) > 0)
+    stockham_run_n<T, L, INV, PMAX, W, LD, TWN, NS * R, Rest...>(buf, tw, np, mdiv, tid);
+}
+
+// tid: the thread's index in the workgroup (passed in so a caller inside a
+// loop can keep the butterfly index math from being held across iterations)
+template <typename T, int L, bool INV, int PMAX, int W, int LD, int TWN, int... Rs>
+BSK_DEV void fft_lds_n(cplx<T>* buf, const cplx<T>* tw, int np, unsigned mdiv, int tid,
+                       Radices<Rs...>) {
+  stockham_run_n<T, L, INV, PMAX, W, LD, TWN, 1, Rs...>(buf, tw, np, mdiv, tid);
+}
+
 // ---------------------------------------------------------------- radix plans
 // Two variants per length: 0 = radix-8 heavy, 1 = radix-4 heavy.
 template <int L, int V>

@@ -79,6 +79,29 @@ int bsk_triple_pairs_launch(const PairArgs& a, int sets, int ni, hipStream_t s);
 int bsk_reduce_mfma_launch(const float* partial, int nwg, int items_per_wg, int ntri,
                            const int* slot_dev, double scale, double* out, hipStream_t s);
 
+// Fused z c2r + split-BF16 triple sums (zfused.hip): shell meshes with a
+// compiled kernel, at most kZMaxShells shells.
+#define BSK_ZFUSED_SIZES(X) \
+  X(64) X(72) X(80) X(96) X(120) X(128) X(144) X(160) X(192) X(200) X(216) X(240) X(256)
+constexpr int kZMaxShells = 40;
+struct ZArgs {
+  const cplx<float>* d2;  // inverse y output [nsh][nrows][MZP] (whole x range)
+  long long nrows;        // S * S (x, y) rows
+  int MZ, MZP;
+  const int* n2hi;        // shell limits (device), from shell0
+  int shell0;
+  int nsh, nkb;
+  int nwg;
+  const int* items;       // [kPairWaves * NI] item descriptors of this launch
+  const int* pairs;       // [tiles + 1][16]
+  float* partial;         // [nwg][item_total][256]
+  int item0, item_total;
+  const cplx<float>* tw;  // S twiddles exp(-2 pi i t / S)
+};
+size_t bsk_zrows_lds_bytes(int S, int nsh, int nkb);
+int bsk_triple_zrows_launch_v0(const ZArgs& a, int S, int ni, hipStream_t s);
+int bsk_triple_zrows_launch_v1(const ZArgs& a, int S, int ni, hipStream_t s);
+
 // Power spectrum sums (spectrum.hip).
 int bsk_spectrum_launch(int dtype, const void* dk, Geom g, int N, const int* n2lo,
                         const int* n2hi, int nbin, double kf, int nchunk, double* partial,

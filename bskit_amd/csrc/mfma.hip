@@ -23,53 +23,16 @@
 //    the previous item's tile), loads its B column 16*kb + col, and issues
 //    three MFMAs; padding items do nothing (keeping the third-side fragment
 //    live across items for reuse costs more in registers than it saves);
-//  * FP32 MFMA chains run for at most F K steps and are then added into an
+//  * FP32 MFMA chains run for at most kChain K steps and are then added into an
 //    FP64 accumulator (long FP32 chains lose accuracy: per-region partial
 //    sums are spatially coherent and large against the cancelling total).
 // Per-workgroup tiles are written as FP32 partials [wg][item][16][16]; a
 // second kernel sums them over workgroups in a fixed order in FP64.
-#include <type_traits>
-
-#include "launch.h"
+#include "pair_mma.h"
 
 namespace {
 
-typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
-typedef float f32x2 __attribute__((ext_vector_type(2)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-
-template <int B, int E, typename F>
-BSK_DEV void static_for(F&& f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, E>(f);
-  }
-}
-
-// The bf16 pairs (-1, 0), (0, -1) as dot2 operands for the product splits,
-// made once per thread and kept in registers (as inline constants the packed
-// pairs are not encodable).
-struct PairConsts {
-  bf16x2 neg_lo, neg_hi;
-};
-
-BSK_DEV PairConsts pair_consts() {
-  return PairConsts{bf16_pair_bits(kBf16NegOneLo), bf16_pair_bits(kBf16NegOneHi)};
-}
-
-// hi = rne_bf16(v), lo = rne_bf16(v - hi), packed two values per word.
-// v - hi comes from v_dot2_f32_bf16 (hi.x * -1 + hi.y * 0 + v): one op per
-// value instead of an unpack and a subtraction, and exact (v - hi is
-// representable in FP32).
-BSK_DEV void split2(float a, float b, uint32_t& hi, uint32_t& lo, const PairConsts& k) {
-  const f32x2 v = {a, b};
-  const bf16x2 h = __builtin_convertvector(v, bf16x2);
-  const f32x2 r = {__builtin_amdgcn_fdot2_f32_bf16(h, k.neg_lo, a, false),
-                   __builtin_amdgcn_fdot2_f32_bf16(h, k.neg_hi, b, false)};
-  hi = __builtin_bit_cast(uint32_t, h);
-  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
-}
+using namespace pairmma;
 
 // hi + lo of the low / high bf16 of a pair of words, as one dot2 each (the
 // (1, 0) / (0, 1) operands are made per staged chunk, not kept live).
@@ -85,7 +48,6 @@ BSK_DEV float bf_sum_hi(uint32_t h, uint32_t l, bf16x2 one_hi) {
 constexpr int kW = kPairWaves;   // waves per workgroup
 constexpr int kNT = 64 * kW;
 constexpr int kCK = kPairChunk;  // cells per staged chunk
-constexpr int kF = 8;            // K steps per FP32 accumulation chain
 // LDS layout (strides chosen by measured ds_read_b128 throughput on gfx950):
 //  * FP32 pair-side rows of 136 floats (544 B), cells in natural order; in a
 //    K step lane group g reads cells 4g..4g+3 and 16+4g..16+4g+3, the pattern
@@ -191,30 +153,14 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
   }
 
   // item descriptors of this wave and the LDS rows of this lane's pair
-  // (row offsets in floats, both < 2^16, packed in one register per item)
-  int desc[NI];
-  int oij[NI];
-#pragma unroll
-  for (int s = 0; s < NI; ++s) {
-    desc[s] = __builtin_amdgcn_readfirstlane(a.items[wave * NI + s]);
-    const int tile = desc[s] & 0xffff;
-    const int pr = a.pairs[tile * 16 + col];
-    oij[s] = ((pr & 0xffff) * kLDF) | (((pr >> 16) * kLDF) << 16);
-  }
-
-  f32x4 acc[NI];
-  double acc2[NI][4];
-#pragma unroll
-  for (int s = 0; s < NI; ++s) {
-    acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc2[s][r] = 0.0;
-  }
+  int desc[NI], oij[NI];
+  setup<NI, kLDF>(desc, oij, a.items, a.pairs, wave, col);
+  State<NI> st;
+  init(st);
 
   const long long nch = a.ncell / kCK;
   const long long ch0 = nch * blockIdx.x / a.nwg;
   const long long ch1 = nch * (blockIdx.x + 1) / a.nwg;
-  int since_flush = 0;
   Pref<PV> p0, p1, p2;
   if (ch0 < ch1) {
     fetch(p0, a.hi[0], a.lo[0], a.stride, n0, ch0 * kCK);
@@ -234,57 +180,12 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
       if (SETS >= 2) fetch(p2, a.hi[2], a.lo[2], a.stride, n2, cb);
     }
 #pragma unroll  // constant LDS offsets per K step
-    for (int ks = 0; ks < kKSTEPS; ++ks) {
-      const int c = ks * 32 + grp * 4;   // FP32 rows: cells c..c+3, c+16..c+19
-      const int cb = ks * 32 + grp * 8;  // bf16 rows (permuted): eight at cb
-      bf16x8 Ah, Al;
-      static_for<0, NI>([&](auto sc) {
-        constexpr int s = decltype(sc)::value;
-        if (desc[s] & (1 << 29)) return;  // uniform: padding item
-        if (s == 0 || desc[s] < 0) {      // uniform: new tile -> new pair products
-          const float4* pi = reinterpret_cast<const float4*>(fA + (oij[s] & 0xffff) + c);
-          const float4* pj = reinterpret_cast<const float4*>(fB + ((unsigned)oij[s] >> 16) + c);
-          const float4 x0 = pi[0], x1 = pi[4], y0 = pj[0], y1 = pj[4];
-          uint32_t ah[4], al[4];
-          split2(x0.x * y0.x, x0.y * y0.y, ah[0], al[0], kc);
-          split2(x0.z * y0.z, x0.w * y0.w, ah[1], al[1], kc);
-          split2(x1.x * y1.x, x1.y * y1.y, ah[2], al[2], kc);
-          split2(x1.z * y1.z, x1.w * y1.w, ah[3], al[3], kc);
-          Ah = __builtin_bit_cast(bf16x8, ah);
-          Al = __builtin_bit_cast(bf16x8, al);
-        }
-        const int kb = (desc[s] >> 16) & 0xff;
-        const int ko = (16 * kb + col) * kLDH + cb;
-        const bf16x8 Bh = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(hK + ko));
-        const bf16x8 Bl = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(lK + ko));
-        f32x4 t = acc[s];
-        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bh, t, 0, 0, 0);
-        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ah, Bl, t, 0, 0, 0);
-        t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Al, Bh, t, 0, 0, 0);
-        acc[s] = t;
-      });
-      if (++since_flush == kF) {
-        since_flush = 0;
-#pragma unroll
-        for (int s = 0; s < NI; ++s) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) acc2[s][r] += (double)acc[s][r];
-          acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
-      }
-    }
+    for (int ks = 0; ks < kKSTEPS; ++ks)
+      kstep<NI, kLDH>(st, desc, oij, fA, fB, hK, lK, ks * 32 + grp * 4, ks * 32 + grp * 8, col,
+                      kc);
   }
-#pragma unroll
-  for (int s = 0; s < NI; ++s)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) acc2[s][r] += (double)acc[s][r];
-
-  // C[row][col]: row = 4*grp + r (pair row of the tile), col = lane%16 (k)
-  float* out = a.partial + ((long long)blockIdx.x * a.item_total + a.item0 + wave * NI) * 256;
-#pragma unroll
-  for (int s = 0; s < NI; ++s)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) out[s * 256 + (4 * grp + r) * 16 + col] = (float)acc2[s][r];
+  store(st, a.partial + ((long long)blockIdx.x * a.item_total + a.item0 + wave * NI) * 256, grp,
+        col);
 }
 
 __global__ __launch_bounds__(64) void k_reduce_mfma(const float* __restrict__ partial, int nwg,

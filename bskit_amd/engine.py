@@ -175,6 +175,8 @@ class GridEngine:
         self.work = Workspace(self.device)
         self._tri_cache = {}
         self.tuned_nwg = None
+        self.tuned_zfused_nwg = None
+        self.prefer_zfused = False  # set by the autotuner when the fused kernel is faster
         self.cic_window = 0
         if cic_window:
             self.set_compensation(cic_window)
@@ -304,6 +306,61 @@ class GridEngine:
     def mfma_supported(nshell, nfield, tri):
         """Whether the MFMA kernel covers this shell count / field roles."""
         return nshell <= (96 if nfield == 3 else 128)
+
+    def zfused_supported(self, tri):
+        """Whether the fused z c2r + split-BF16 triple-sum kernel covers the
+        auto spectrum of all nb shells with this triangle list (shell mesh
+        size, shell count, pair-plan shape)."""
+        plan = self.pair_plan(tri, self.nb, L.MODE_AUTO)
+        ok = ctypes.c_int()
+        L.call('bsk_triples_zfused_supported', self._plan, plan.handle, ctypes.byref(ok))
+        return bool(ok.value)
+
+    def bispectrum_zfused(self, dk, tri, scale=1.0, out=None, stream=None, nwg=None, work=None,
+                          kernel_events=None, after_shells=None):
+        """scale * sum_x d_i d_j d_k over the shell mesh for every (i, j, k) of
+        tri, from the FP32 spectrum dk, with the z transform of the shells
+        inside the triple-sum kernel: the quantity triples([shells(dk,
+        fmt=BF16X2)], tri) computes, without the shell fields in HBM."""
+        tri = np.asarray(tri, dtype=np.int32).reshape(-1, 3)
+        T = len(tri)
+        if dk is None or dk.dtype != torch.complex64:
+            raise ValueError('the fused path needs an FP32 (complex64) spectrum')
+        if T and (tri.min() < 0 or tri.max() >= self.nb):
+            raise IndexError('triangle shell index out of range')
+        if out is None:
+            out = torch.empty(T, dtype=torch.float64, device=self.device)
+        if T == 0:
+            return out
+        plan = self.pair_plan(tri, self.nb, L.MODE_AUTO)
+        s = stream_handle(stream)
+        nbytes = L.size_out()
+        L.call('bsk_shells_workspace', self._plan, L.DT_F32, self.nb, self.S, ctypes.byref(nbytes))
+        w = (work or self.work).get('shell', nbytes.value)
+        L.call('bsk_shells', self._plan, L.DT_F32, L.KIND_DELTA, L.ptr(dk), 0, self.nb, 0, self.S,
+               L.STAGE_X | L.STAGE_Y, L.FMT_BF16X2, None, None, self.ncell, 0, L.ptr(w), s)
+        if after_shells is not None:  # e.g. a timing mark between the two stages
+            after_shells()
+        nwg = self.zfused_nwg() if nwg is None else int(nwg)
+        L.call('bsk_triples_mfma_workspace', plan.handle, nwg, ctypes.byref(nbytes))
+        pw = (work or self.work).get('mfma', nbytes.value)
+        if kernel_events is None:
+            L.call('bsk_triples_zfused', self._plan, plan.handle, L.ptr(w), 0, nwg, float(scale),
+                   L.ptr(out), L.ptr(pw), s)
+            return out
+        kernel_events[0].record(stream)
+        L.call('bsk_triples_zfused', self._plan, plan.handle, L.ptr(w), 0, nwg, float(scale), None,
+               L.ptr(pw), s)
+        kernel_events[1].record(stream)
+        L.call('bsk_triples_mfma_reduce', plan.handle, nwg, float(scale), L.ptr(out), L.ptr(pw), s)
+        return out
+
+    def zfused_nwg(self):
+        """Workgroups of the fused kernel: the tuned count, else 512 (two per
+        CU) with at least 8 rows each."""
+        if self.tuned_zfused_nwg:
+            return self.tuned_zfused_nwg
+        return int(max(1, min(512, self.S * self.S // 8)))
 
     def plain_nchunk(self, ncell=None):
         ncell = self.ncell if ncell is None else ncell

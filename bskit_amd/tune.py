@@ -5,7 +5,11 @@ Tunables of one GridEngine key (Nmesh, shell mesh, retained cube, bins):
 * the radix-plan variant (0: radix-8 heavy, 1: radix-4 heavy) of each of
   the six FFT passes; the three forward passes and the three inverse passes
   are timed as groups (all 2^3 combinations each);
-* the workgroup count of the MFMA triple kernel.
+* the workgroup count of the MFMA triple kernel, and of the fused z c2r +
+  triple kernel where it applies;
+* whether the fused kernel (z c2r inside the triple sums, no shell fields in
+  HBM) or the separate z pass + triple kernel is faster for the workload
+  (`zfused`: the engine's choice for bispectra of all its shells).
 
 Results live in a process-wide cache (a dict guarded by a lock) persisted
 as JSON at $BSK_TUNE_CACHE (default ~/.cache/bskit_amd/tune.json), keyed by
@@ -72,6 +76,9 @@ def apply(eng, cfg):
         eng.set_variant(p, v)
     if cfg.get('mfma_nwg'):
         eng.tuned_nwg = int(cfg['mfma_nwg'])
+    if cfg.get('zfused_nwg'):
+        eng.tuned_zfused_nwg = int(cfg['zfused_nwg'])
+    eng.prefer_zfused = bool(cfg.get('zfused', False))
 
 
 def _time(fn, reps=3):
@@ -146,6 +153,23 @@ def tune(eng, tri=None, field=None, reps=3, force=False, log=None):
             if best is None or t < best[0]:
                 best = (t, n)
         cfg['mfma_nwg'] = int(best[1])
+        if eng.zfused_supported(tri):
+            best = None
+            for n in (256, 512, 768, 1024):
+                t = _time(lambda: eng.bispectrum_zfused(dk, tri, out=out, nwg=n), reps)
+                timings['zfused_nwg%d' % n] = t
+                if best is None or t < best[0]:
+                    best = (t, n)
+            cfg['zfused_nwg'] = int(best[1])
+            # whole shells + sums stage, both ways, each at its best count
+            sep = _time(lambda: (eng.shells(dk, fmt=L.FMT_BF16X2, out=shells),
+                                 eng.triples([shells], tri, path='mfma', out=out,
+                                             nwg=cfg['mfma_nwg'])), reps)
+            fus = _time(lambda: eng.bispectrum_zfused(dk, tri, out=out, nwg=cfg['zfused_nwg']),
+                        reps)
+            timings['shells_triples_separate'] = sep
+            timings['shells_triples_zfused'] = fus
+            cfg['zfused'] = bool(fus < sep)
     if log:
         log('[tune] %s -> %s' % (k, {x: cfg[x] for x in cfg if x != 'timings_ms'}))
     with _LOCK:

@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define BSK_ABI_VERSION 3
+#define BSK_ABI_VERSION 4
 
 /* status codes */
 #define BSK_OK 0
@@ -117,6 +117,9 @@ int bsk_forward(bsk_plan* plan, int dtype, const void* field, void* dk, void* wo
  * stages: BSK_STAGE_X runs the masked inverse x transform of every
  * retained (ky,kz) line into work; BSK_STAGE_YZ runs the y and z passes of
  * the slab from that work (so one X stage serves many slabs); both = 3.
+ * BSK_STAGE_Y instead of BSK_STAGE_YZ runs the y pass only and leaves its
+ * output in work for bsk_triples_zfused (FP32, x0 = 0, xslab = S; out
+ * unused).
  * Output shell b (relative to shell0) lands at
  * out + (shell0+b)*shell_stride + out_cell0 (elements) with the slab's cells
  * in C order; for BSK_FMT_BF16X2 out is the hi plane and out_lo the lo
@@ -127,6 +130,7 @@ int bsk_forward(bsk_plan* plan, int dtype, const void* field, void* dk, void* wo
  * work: bsk_shells_workspace(nshell, xslab) bytes. */
 #define BSK_STAGE_X 1
 #define BSK_STAGE_YZ 2
+#define BSK_STAGE_Y 4
 int bsk_shells_workspace(const bsk_plan* plan, int dtype, int nshell, int xslab, size_t* bytes);
 int bsk_shells(bsk_plan* plan, int dtype, int kind, const void* dk, int shell0, int nshell, int x0,
                int xslab, int stages, int fmt, void* out, void* out_lo, long long shell_stride,
@@ -193,7 +197,8 @@ int bsk_pairs_info(const bsk_pairs* pairs, int* npair, int* ntile, int* nitem,
 /* Partial-buffer slot of each triangle (item * 256 + row * 16 + column). */
 int bsk_pairs_slots(const bsk_pairs* pairs, int* slot_host);
 /* Host copies of the launch tables, for inspection: items
- * [npass * 8 * items_per_wave] (tile | kb << 16 | new-tile flag << 31) and
+ * [npass * 8 * items_per_wave] (tile | kb << 16 | padding flag << 29 |
+ * new-tile flag << 31) and
  * pair rows [(ntile + 1) * 16] (row in set 0 | row in set 1 << 16; padding
  * rows and the last, all-padding tile point at row nshell, a zero row). */
 int bsk_pairs_tables(const bsk_pairs* pairs, int* items, int* pair_rows);
@@ -207,6 +212,20 @@ int bsk_triples_mfma(const bsk_pairs* pairs, const uint16_t* hi_i, const uint16_
  * same result as one call with out_dev). */
 int bsk_triples_mfma_reduce(const bsk_pairs* pairs, int nwg, double scale, double* out_dev,
                             const void* work, void* stream);
+/* The same split-BF16 sums for an auto spectrum with the z c2r of the shells
+ * done inside the triple-sum kernel: shell_work is the bsk_shells workspace
+ * after stages BSK_STAGE_X | BSK_STAGE_Y of shells [shell0, shell0 +
+ * nshell) of an FP32 spectrum (nshell = the pair plan's shell count, x0 = 0,
+ * xslab = S), so the shell fields never reach HBM.  Replaces the shell
+ * paint and the per-triangle sum of measure_bispectrum_faster
+ * (bskit/main.py:1846-1882) like bsk_shells + bsk_triples_mfma.  work:
+ * bsk_triples_mfma_workspace(pairs, nwg) bytes; out_dev == NULL leaves the
+ * partials for bsk_triples_mfma_reduce.  *ok (supported query): 1 if this
+ * plan (shell mesh size, shell count, items per wave, LDS) has a kernel. */
+int bsk_triples_zfused_supported(const bsk_plan* plan, const bsk_pairs* pairs, int* ok);
+int bsk_triples_zfused(bsk_plan* plan, const bsk_pairs* pairs, const void* shell_work,
+                       int shell0, int nwg, double scale, double* out_dev, void* work,
+                       void* stream);
 
 /* Cloud-in-cell assignment of npart particles (pos: npart x 3 float, box
  * units, C order) to the periodic mesh grid (nmesh^3 float, overwritten).

@@ -319,6 +319,39 @@ def test_mfma_random_lists_match_fp64(mode, nb, ntri):
     assert err.max() < 5e-5
 
 
+@pytest.mark.parametrize('N,nb,width,S', [(64, 8, 2, 64), (128, 12, 2, 80), (256, 8, 8, 200),
+                                          (256, 32, 2, 200)])
+def test_zfused_matches_separate_passes_and_fp64(N, nb, width, S):
+    """The fused z c2r + split-BF16 sums against the separate z pass + MFMA
+    kernel (same shells up to FP32 summation order) and the FP64 plain path."""
+    from bskit_amd import _lib as L
+    from bskit_amd import bins
+    box = 1000.0
+    lo, hi = _bins(nb, width, box)
+    eng = _engine(N, box, lo, hi)
+    assert eng.S == S
+    kLU = np.stack([lo, hi], axis=1)
+    tri = bins.closed_triangle_indices(kLU, 1)
+    assert eng.zfused_supported(tri)
+    f = _field(N, 11, np.float64)
+    d32 = eng.forward(torch.from_numpy(f.astype(np.float32)).cuda())
+    sc = 1.0 / eng.ncell
+    got = eng.bispectrum_zfused(d32, tri, scale=sc).cpu().numpy()
+    sbf = eng.shells(d32, fmt=L.FMT_BF16X2)
+    sep = eng.triples([sbf], tri, path='mfma', scale=sc).cpu().numpy()
+    s64 = eng.shells(eng.forward(torch.from_numpy(f).cuda()), fmt=L.FMT_F64, dtype=torch.float64)
+    ref = eng.triples([s64], tri, path='plain', scale=sc).cpu().numpy()
+    a = s64.data
+    absum = torch.stack([(a[i].abs() * a[j].abs() * a[k].abs()).sum() for i, j, k in tri])
+    absum = absum.cpu().numpy() * sc
+    e_sep = (np.abs(got - sep) / absum).max()
+    e_ref = (np.abs(got - ref) / absum).max()
+    print('zfused N=%d S=%d nb=%d T=%d vs separate %.2e vs fp64 %.2e'
+          % (N, S, nb, len(tri), e_sep, e_ref))
+    assert e_sep < 2e-6
+    assert e_ref < 5e-5
+
+
 def test_autotune_picks_valid_config_and_keeps_results(tmp_path, monkeypatch):
     """The tuner's choice is cached, re-applied, and changes only rounding."""
     from bskit_amd import _lib as L
