@@ -254,6 +254,8 @@ def main():
     ap.add_argument('--no-tune', action='store_true', help='skip the autotuner (default variants)')
     ap.add_argument('--no-zfused', action='store_true',
                     help='time the separate shell and triple-sum passes only')
+    ap.add_argument('--no-levels', action='store_true',
+                    help='sum every triangle over the full shell mesh (no decimated copies)')
     ap.add_argument('--jobs', type=int, default=2000, help='jobs in the serving run')
     ap.add_argument('--load', type=float, default=0.8, help='arrival rate / measured capacity')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
@@ -289,6 +291,7 @@ def main():
     wl = build_workload(args, torch, np)
     T = len(wl['tri'])
     eng = GridEngine(wl['N'], wl['L'], wl['k_low'], wl['k_high'], cic_window=wl['N'])
+    eng.levels = not args.no_levels
     log('[bench] T=%d shells=%d compact m=%d shell mesh S=%d' % (T, eng.nb, eng.m, eng.S))
 
     tuned = None
@@ -363,23 +366,30 @@ def main():
         log('[bench] accuracy vs FP64: %s' % acc)
 
     # roofline of the dominant kernel: the split-BF16 MFMA pair kernel, timed
-    # by HIP events around its launch inside every timed step
+    # by HIP events around its launches inside every timed step.  The
+    # separate path runs the triangles exact on the decimated mesh over the
+    # shells' decimated copies (S/2)^3 and the rest over S^3, one launch per
+    # pass of each level; figures are per launch (grid totals / launches).
     N = wl['N']
     ncell = float(eng.ncell)                     # shell-mesh cells S^3
     nsh = eng.nb
-    shell_bytes = nsh * 4 * ncell               # split-BF16 hi+lo, each value read once
-    flops = 2.0 * T * ncell                      # one multiply-add per triangle per cell
-    plan = eng.pair_plan(wl['tri'], nsh, 0)
+    levels = eng.levels and eng.dec is not None
+    dcell = float((eng.S // 2) ** 3) if levels else 0.0
+    lv = eng.triangle_levels(wl['tri']) if levels else np.zeros(T, np.int32)
+    # one multiply-add per triangle per cell summed
+    flops_sep = 2.0 * (np.sum(lv == 0) * ncell + np.sum(lv == 1) * dcell)
+    flops_fused = 2.0 * T * ncell
 
-    def roofline(kernel, kern_ms, alg_bytes):
-        t_k = kern_ms * 1e-3
+    def roofline(kernel, grid_ms, alg_bytes, launches, flops, plan):
+        t_k = grid_ms * 1e-3
         ai = flops / alg_bytes
         bound = 'hbm' if ai * HBM_PEAK_GBS * 1e9 < BF16_PEAK_TFLOPS * 1e12 else 'mfma'
         achieved = alg_bytes / t_k / 1e9
         return {'kernel': kernel, 'bound': bound, 'achieved': achieved,
                 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                 'traffic': profiled_traffic(kernel, N, eng.S),
-                'kernel_ms': kern_ms, 'algorithmic_bytes': alg_bytes,
+                'kernel_ms': grid_ms / launches, 'launches_per_grid': launches,
+                'algorithmic_bytes': alg_bytes / launches,
                 'useful_tflops': flops / t_k / 1e12, 'arith_intensity_flop_per_byte': ai,
                 'pairs': plan.npair, 'tiles': plan.ntile, 'items': plan.nitem}
 
@@ -389,15 +399,22 @@ def main():
     d2_bytes = float(np.sum(mz)) * 8.0 * eng.S * eng.S
 
     def path_roofline(path, ms):
-        return (roofline('k_triple_zrows', ms, d2_bytes) if path == 'zfused'
-                else roofline('k_triple_pairs', ms, shell_bytes))
+        if path == 'zfused':
+            plan = eng.pair_plan(wl['tri'], nsh, 0)
+            return roofline('k_triple_zrows', ms, d2_bytes, plan.npass, flops_fused, plan)
+        plan = eng.pair_plan(wl['tri'], nsh, 0, levels=levels)
+        # split-BF16 hi+lo of every shell read once, full mesh and decimated copy
+        dec_alg = float(eng.dec[1]) if plan.npass_dec else 0.0
+        return roofline('k_triple_pairs', ms, nsh * 4 * (ncell + dec_alg), plan.npass,
+                        flops_sep, plan)
 
     roof = path_roofline(main_path, float(np.mean(timer.pair_durations('kernel'))))
     if alt is not None:
         alt['roofline'] = path_roofline(alt['path'], alt['kernel_ms'])
     H = eng.MX * eng.MY * eng.MZ                 # compact delta(k) cube
     t_sh = (alt['shells_ms'] if main_path == 'zfused' else stage_ms['shells']) * 1e-3
-    shells_alg = 8.0 * H + nsh * 4 * ncell       # delta(k) read + shells written
+    # delta(k) read + shells (and their decimated copies) written
+    shells_alg = 8.0 * H + nsh * 4 * (ncell + (eng.dec[1] if levels else 0))
     stage_gbs = {'shells_GBps': shells_alg / t_sh / 1e9,
                  'forward_GBps': (4.0 * N ** 3 + 8.0 * H) / (stage_ms['forward'] * 1e-3) / 1e9}
 
@@ -445,7 +462,9 @@ def main():
                    'path': ('split-BF16 MFMA (hi*hi+hi*lo+lo*hi), FP32 accumulate, FP64 '
                             'reduce; %s' % ('z c2r fused into the triple sums'
                                             if main_path == 'zfused' else
-                                            'shells on the shell mesh')),
+                                            'shells on the shell mesh; %d of the triangles '
+                                            'summed over its decimated (S/2)^3 copy (exact)'
+                                            % int(lv.sum()))),
                    'parallelism': 'single GPU'},
         'fp32_plain': res_plain,
         'speedup_vs_fp32_plain': (grids_mfma / res_plain['grids_per_s']) if res_plain else None,

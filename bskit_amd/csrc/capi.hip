@@ -63,6 +63,13 @@ int auto_shell_mesh(int N, int m) {
   return (s > 0 && s < N) ? s : N;
 }
 
+// Decimated copy of a shell (the cells with even x, y, z: an (S/2)^3 mesh)
+// at this element offset within the shell, padded to a multiple of 128
+// cells (the MFMA kernel's chunk).
+long long round128(long long v) { return (v + 127) / 128 * 128; }
+long long decimated_offset(int S) { return round128((long long)S * S * S); }
+long long decimated_cells(int S) { return round128((long long)(S / 2) * (S / 2) * (S / 2)); }
+
 template <typename T>
 int make_twiddles(int n, cplx<T>** out) {
   std::vector<cplx<T>> h(n);
@@ -125,9 +132,9 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
   const size_t esz = (fmt == BSK_FMT_F64) ? 8 : (fmt == BSK_FMT_F32 ? 4 : 2);
   char* o = reinterpret_cast<char*>(out) + (size_t)shell0 * stride * esz;
   char* ol = out_lo ? reinterpret_cast<char*>(out_lo) + (size_t)shell0 * stride * esz : nullptr;
+  const long long dec_off = (stages & BSK_STAGE_DECIMATE) ? decimated_offset(N) : -1;
   return bsk_inv_z<T>(N, p->variant[5], d2, o, ol, tw, g.MZ, g.MZP, (long long)xs * N, cell0,
-                      stride, fmt,
-                      nsh, p->d_hi, shell0, s);
+                      stride, fmt, nsh, p->d_hi, shell0, dec_off, s);
 }
 
 const cplx<float>* shell_tw(const bsk_plan* p, float) { return p->tws_f ? p->tws_f : p->tw_f; }
@@ -292,6 +299,49 @@ int bsk_plan_shell_mesh(const bsk_plan* p, int* shell_mesh) {
   return BSK_OK;
 }
 
+int bsk_plan_decimated(const bsk_plan* p, long long* offset, long long* ncell) {
+  if (int rc = check_plan(p)) return rc;
+  if (!offset || !ncell) {
+    bsk_set_error("bsk_plan_decimated: null output");
+    return BSK_EINVAL;
+  }
+  if (p->S % 2) {
+    bsk_set_error("bsk_plan_decimated: odd shell mesh %d has no decimated copy", p->S);
+    return BSK_EUNSUPPORTED;
+  }
+  *offset = decimated_offset(p->S);
+  *ncell = decimated_cells(p->S);
+  return BSK_OK;
+}
+
+int bsk_plan_triangle_levels(const bsk_plan* p, int ntri, const int* tri, int* level) {
+  if (int rc = check_plan(p)) return rc;
+  if (ntri < 0 || (ntri && (!tri || !level))) {
+    bsk_set_error("bsk_plan_triangle_levels: invalid argument");
+    return BSK_EINVAL;
+  }
+  // largest |n| component of each shell's modes
+  std::vector<long long> mb(p->nbin);
+  for (int b = 0; b < p->nbin; ++b) {
+    long long r = 0;
+    while ((r + 1) * (r + 1) <= (long long)p->n2hi[b]) ++r;
+    mb[b] = r;
+  }
+  for (int t = 0; t < ntri; ++t) {
+    long long sum = 0;
+    for (int e = 0; e < 3; ++e) {
+      const int b = tri[3 * t + e];
+      if (b < 0 || b >= p->nbin) {
+        bsk_set_error("bsk_plan_triangle_levels: triangle %d has a bin index out of range", t);
+        return BSK_EINVAL;
+      }
+      sum += mb[b];
+    }
+    level[t] = (p->S % 2 == 0 && 2 * sum < p->S) ? 1 : 0;
+  }
+  return BSK_OK;
+}
+
 int bsk_plan_set_shell_mesh(bsk_plan* p, int shell_mesh) {
   if (int rc = check_plan(p)) return rc;
   const int S = shell_mesh == 0 ? auto_shell_mesh(p->N, p->m) : shell_mesh;
@@ -382,8 +432,16 @@ int bsk_shells(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int
   if (int rc = check_plan(p)) return rc;
   const long long slab_cells = (long long)xslab * p->S * p->S;
   const bool yz = (stages & BSK_STAGE_YZ) != 0;
+  if ((stages & BSK_STAGE_DECIMATE) &&
+      (!yz || p->S % 2 || x0 != 0 || xslab != p->S || out_cell0 != 0 ||
+       shell_stride < decimated_offset(p->S) + decimated_cells(p->S))) {
+    bsk_set_error("bsk_shells: the decimated copy needs the y and z stages over the whole "
+                  "mesh (x0 = 0, xslab = S, out_cell0 = 0), an even S and a shell stride of at "
+                  "least bsk_plan_decimated offset + cells");
+    return BSK_EINVAL;
+  }
   if (nshell <= 0 || shell0 < 0 || shell0 + nshell > p->nbin || x0 < 0 || xslab <= 0 ||
-      x0 + xslab > p->S || !work || stages < 1 || stages > 7 ||
+      x0 + xslab > p->S || !work || stages < 1 || stages > 15 ||
       ((stages & BSK_STAGE_Y) && (stages & BSK_STAGE_YZ)) ||
       ((stages & BSK_STAGE_X) && kind == BSK_KIND_DELTA && !dk) ||
       (kind < BSK_KIND_DELTA || kind > BSK_KIND_KMAG) ||
@@ -522,6 +580,7 @@ struct bsk_pairs {
   int nkb = 0;
   int ni = 1;          // items per wave and launch (compiled count)
   int npass = 1;
+  int npass0 = 1;      // passes of level 0 (full mesh); the rest run on the decimated mesh
   int item_total = 0;  // npass * waves * ni
   std::vector<int> slot;
   std::vector<int> h_items, h_pairs;
@@ -681,94 +740,78 @@ int upload(int** dst, const std::vector<int>& v) {
   return BSK_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int bsk_pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri) {
-  if (!out || nshell < 1 || mode < 0 || mode > 2 || ntri < 0 || (ntri && !tri)) {
-    bsk_set_error("bsk_pairs_create: invalid argument");
-    return BSK_EINVAL;
-  }
-  *out = nullptr;
-  for (int t = 0; t < ntri; ++t)
-    for (int e = 0; e < 3; ++e)
-      if (tri[3 * t + e] < 0 || tri[3 * t + e] >= nshell) {
-        bsk_set_error("bsk_pairs_create: triangle %d has a shell index out of range", t);
-        return BSK_EINVAL;
-      }
-  const int sets = mode == BSK_MODE_AUTO ? 1 : (mode == BSK_MODE_AAB ? 2 : 3);
-  const int nkb = (nshell + 15) / 16;
-  int nsh[3] = {nshell, nshell, nshell};
-  if (!bsk_pair_prefetch(nshell) || bsk_pair_lds_bytes(sets, nsh, nkb) > 160 * 1024 ||
-      nkb > 16) {
-    bsk_set_error("bsk_pairs_create: %d shells exceed the MFMA kernel's LDS staging "
-                  "(128 shells, 96 for <ABC>)", nshell);
-    return BSK_EUNSUPPORTED;
-  }
-  // role assignment: the fixed roles of <AAB>/<ABC>; for auto spectra the
-  // choice of the pair sides that needs the fewest items
+// One level's share of a pair plan: role choice, tiles, units (a tile with
+// at most `cap` of its kb blocks; each unit forms the tile's pair products
+// once) and their (pass, wave) bins.
+struct LevelPlan {
+  std::vector<int> tri;  // global triangle indices of this level
   std::vector<Role> roles;
   PairTiles tiles;
+  std::vector<int> unit_tile;
+  std::vector<unsigned> unit_mask;
+  std::vector<std::vector<int>> bins;  // [npass * waves] unit lists
+  int npass = 0, maxload = 0;
+};
+
+LevelPlan plan_level(int mode, int nkb, int cap, const std::vector<int>& idx, const int* tri) {
+  LevelPlan lp;
+  lp.tri = idx;
+  std::vector<int> sub((size_t)3 * idx.size());
+  for (size_t t = 0; t < idx.size(); ++t)
+    for (int e = 0; e < 3; ++e) sub[3 * t + e] = tri[3 * (size_t)idx[t] + e];
+  const int ntri = (int)idx.size();
+  // role assignment: the fixed roles of <AAB>/<ABC>; for auto spectra the
+  // choice of the pair sides that needs the fewest items
   const int nopt = mode == BSK_MODE_AUTO ? 4 : 1;
   bool first = true;
   for (int o = 0; o < nopt; ++o) {
-    std::vector<Role> r = o == 3 ? assign_roles_cover(ntri, tri) : assign_roles(mode, o, ntri, tri);
+    std::vector<Role> r =
+        o == 3 ? assign_roles_cover(ntri, sub.data()) : assign_roles(mode, o, ntri, sub.data());
     for (int exact = 1; exact >= 0; --exact) {
       PairTiles t = build_tiles(r, exact != 0);
-      if (first || plan_cost(t) < plan_cost(tiles)) {
-        roles = r;
-        tiles = t;
+      if (first || plan_cost(t) < plan_cost(lp.tiles)) {
+        lp.roles = r;
+        lp.tiles = t;
         first = false;
       }
     }
   }
-  const int W = kPairWaves, cap = bsk_pair_item_cap(sets, nshell);
-  const int ntile = (int)tiles.tile_mask.size();
-  if (ntile >= 0xffff) {
-    bsk_set_error("bsk_pairs_create: %d pair tiles exceed 65534; split the triangle list",
-                  ntile);
-    return BSK_EUNSUPPORTED;
-  }
-  // units: a tile with its kb blocks, split into groups of at most `cap`
-  // blocks (each group forms the tile's pair products once)
-  std::vector<int> unit_tile;
-  std::vector<unsigned> unit_mask;
+  const int W = kPairWaves, ntile = (int)lp.tiles.tile_mask.size();
   for (int ti = 0; ti < ntile; ++ti) {
     unsigned m = 0;
     int n = 0;
     for (int kb = 0; kb < nkb; ++kb) {
-      if (!(tiles.tile_mask[ti] >> kb & 1u)) continue;
+      if (!(lp.tiles.tile_mask[ti] >> kb & 1u)) continue;
       m |= 1u << kb;
       if (++n == cap) {
-        unit_tile.push_back(ti);
-        unit_mask.push_back(m);
+        lp.unit_tile.push_back(ti);
+        lp.unit_mask.push_back(m);
         m = 0;
         n = 0;
       }
     }
     if (m) {
-      unit_tile.push_back(ti);
-      unit_mask.push_back(m);
+      lp.unit_tile.push_back(ti);
+      lp.unit_mask.push_back(m);
     }
   }
-  const int nunit = (int)unit_tile.size();
+  const int nunit = (int)lp.unit_tile.size();
+  if (nunit == 0) return lp;  // no triangles: no passes
   // units to (pass, wave) bins, costliest first (forming a tile's products
   // weighs three items), each to the cheapest bin with room for its items
-  auto ucost = [&](int u) { return 3 + __builtin_popcount(unit_mask[u]); };
+  auto ucost = [&](int u) { return 3 + __builtin_popcount(lp.unit_mask[u]); };
   std::vector<int> order(nunit);
   for (int i = 0; i < nunit; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ucost(x) > ucost(y); });
-  int npass = std::max(1, (tiles.nitem + W * cap - 1) / (W * cap));
-  std::vector<std::vector<int>> bins;
-  for (;; ++npass) {  // ends: every unit fits an empty bin (popcount <= cap)
-    bins.assign((size_t)npass * W, {});
-    std::vector<int> load((size_t)npass * W, 0), cost((size_t)npass * W, 0);
+  for (lp.npass = std::max(1, (lp.tiles.nitem + W * cap - 1) / (W * cap));; ++lp.npass) {
+    // ends: every unit fits an empty bin (popcount <= cap)
+    lp.bins.assign((size_t)lp.npass * W, {});
+    std::vector<int> load((size_t)lp.npass * W, 0), cost((size_t)lp.npass * W, 0);
     bool ok = true;
     for (int u : order) {
-      const int n = __builtin_popcount(unit_mask[u]);
+      const int n = __builtin_popcount(lp.unit_mask[u]);
       int b = -1;
-      for (int q = 0; q < (int)bins.size(); ++q)
+      for (int q = 0; q < (int)lp.bins.size(); ++q)
         if (load[q] + n <= cap && (b < 0 || cost[q] < cost[b])) b = q;
       if (b < 0) {
         ok = false;
@@ -776,81 +819,168 @@ int bsk_pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int*
       }
       load[b] += n;
       cost[b] += ucost(u);
-      bins[b].push_back(u);
+      lp.bins[b].push_back(u);
     }
     if (ok) break;
   }
-  int maxload = 1;
-  for (const auto& b : bins) {
+  for (const auto& b : lp.bins) {
     int l = 0;
-    for (int u : b) l += __builtin_popcount(unit_mask[u]);
-    maxload = std::max(maxload, l);
+    for (int u : b) l += __builtin_popcount(lp.unit_mask[u]);
+    lp.maxload = std::max(lp.maxload, l);
   }
-  const int ni = bsk_pair_items_round(maxload);
+  return lp;
+}
+
+int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri,
+                 const int* level, const char* fn) {
+  if (!out || nshell < 1 || mode < 0 || mode > 2 || ntri < 0 || (ntri && !tri)) {
+    bsk_set_error("%s: invalid argument", fn);
+    return BSK_EINVAL;
+  }
+  *out = nullptr;
+  for (int t = 0; t < ntri; ++t) {
+    for (int e = 0; e < 3; ++e)
+      if (tri[3 * t + e] < 0 || tri[3 * t + e] >= nshell) {
+        bsk_set_error("%s: triangle %d has a shell index out of range", fn, t);
+        return BSK_EINVAL;
+      }
+    if (level && (level[t] < 0 || level[t] > 1)) {
+      bsk_set_error("%s: triangle %d has level %d (0 or 1)", fn, t, level[t]);
+      return BSK_EINVAL;
+    }
+  }
+  const int sets = mode == BSK_MODE_AUTO ? 1 : (mode == BSK_MODE_AAB ? 2 : 3);
+  const int nkb = (nshell + 15) / 16;
+  int nsh[3] = {nshell, nshell, nshell};
+  if (!bsk_pair_prefetch(nshell) || bsk_pair_lds_bytes(sets, nsh, nkb) > 160 * 1024 ||
+      nkb > 16) {
+    bsk_set_error("%s: %d shells exceed the MFMA kernel's LDS staging "
+                  "(128 shells, 96 for <ABC>)", fn, nshell);
+    return BSK_EUNSUPPORTED;
+  }
+  const int W = kPairWaves, cap = bsk_pair_item_cap(sets, nshell);
+  std::vector<int> idx[2];
+  for (int t = 0; t < ntri; ++t) idx[level ? level[t] : 0].push_back(t);
+  LevelPlan lp[2] = {plan_level(mode, nkb, cap, idx[0], tri),
+                     plan_level(mode, nkb, cap, idx[1], tri)};
+  const int tile0[2] = {0, (int)lp[0].tiles.tile_mask.size()};
+  const int ntile = tile0[1] + (int)lp[1].tiles.tile_mask.size();
+  if (ntile >= 0xffff) {
+    bsk_set_error("%s: %d pair tiles exceed 65534; split the triangle list", fn, ntile);
+    return BSK_EUNSUPPORTED;
+  }
+  // an empty list still gets one pass of padding items
+  const int npass0 = ntri ? lp[0].npass : 1;
+  const int npass = npass0 + lp[1].npass;
+  const int ni = bsk_pair_items_round(std::max({1, lp[0].maxload, lp[1].maxload}));
   // item descriptors: tile | kb << 16 | padding << 29 | (first item of a
-  // unit: new pair products) << 31
+  // unit: new pair products) << 31; level 0 passes first
   std::vector<int> items((size_t)npass * W * ni);
   std::vector<int> item_of((size_t)ntile * nkb, -1);  // global item of (tile, kb)
-  for (size_t b = 0; b < bins.size(); ++b) {
-    int s = 0;
-    for (int u : bins[b]) {
-      const int ti = unit_tile[u];
-      bool first = true;
-      for (int kb = 0; kb < nkb; ++kb) {
-        if (!(unit_mask[u] >> kb & 1u)) continue;
-        const int g = (int)b * ni + s;
-        items[g] = ti | (kb << 16) | (first ? (int)0x80000000u : 0);
-        item_of[(size_t)ti * nkb + kb] = g;
-        first = false;
-        ++s;
+  for (int L = 0; L < 2; ++L) {
+    const int pass0 = L ? npass0 : 0;
+    for (size_t b = 0; b < lp[L].bins.size(); ++b) {
+      const int g0 = (pass0 * W + (int)b) * ni;
+      int s = 0;
+      for (int u : lp[L].bins[b]) {
+        const int ti = tile0[L] + lp[L].unit_tile[u];
+        bool first = true;
+        for (int kb = 0; kb < nkb; ++kb) {
+          if (!(lp[L].unit_mask[u] >> kb & 1u)) continue;
+          items[g0 + s] = ti | (kb << 16) | (first ? (int)0x80000000u : 0);
+          item_of[(size_t)ti * nkb + kb] = g0 + s;
+          first = false;
+          ++s;
+        }
       }
+      for (; s < ni; ++s)  // padding: skipped by the kernel (the zero tile, block 0)
+        items[g0 + s] = ntile | (1 << 29) | (s == 0 ? (int)0x80000000u : 0);
     }
-    for (; s < ni; ++s)  // padding: skipped by the kernel (the zero tile, block 0)
-      items[(int)b * ni + s] = ntile | (1 << 29) | (s == 0 ? (int)0x80000000u : 0);
   }
+  if (!ntri)
+    for (int b = 0; b < W; ++b)
+      for (int s = 0; s < ni; ++s)
+        items[b * ni + s] = ntile | (1 << 29) | (s == 0 ? (int)0x80000000u : 0);
   // pair rows (padding rows point at the zero row of each set)
   std::vector<int> prs((size_t)(ntile + 1) * 16, nshell | (nshell << 16));
-  std::vector<std::pair<int, int>> where(tiles.pairs.size());  // (tile, row)
-  for (int ti = 0; ti < ntile; ++ti) {
-    const int p0 = tiles.tile_of_pair_start[ti];
-    const int p1 = ti + 1 < ntile ? tiles.tile_of_pair_start[ti + 1] : (int)tiles.pairs.size();
-    for (int p = p0; p < p1; ++p) {
-      const long long k = tiles.pairs[p].first;
-      prs[(size_t)ti * 16 + (p - p0)] = (int)(k >> 32) | ((int)(k & 0xffffffff) << 16);
-      where[p] = {ti, p - p0};
-    }
-  }
-  // triangle slots
-  std::vector<long long> keys(tiles.pairs.size());
-  for (size_t p = 0; p < keys.size(); ++p) keys[p] = tiles.pairs[p].first;
-  std::vector<size_t> kidx(keys.size());
-  for (size_t p = 0; p < kidx.size(); ++p) kidx[p] = p;
-  std::sort(kidx.begin(), kidx.end(), [&](size_t x, size_t y) { return keys[x] < keys[y]; });
   bsk_pairs* q = new bsk_pairs();
+  q->slot.resize(ntri);
+  int npair = 0, nitem = 0;
+  for (int L = 0; L < 2; ++L) {
+    const PairTiles& tl = lp[L].tiles;
+    const int nt = (int)tl.tile_mask.size();
+    std::vector<std::pair<int, int>> where(tl.pairs.size());  // (global tile, row)
+    for (int ti = 0; ti < nt; ++ti) {
+      const int p0 = tl.tile_of_pair_start[ti];
+      const int p1 = ti + 1 < nt ? tl.tile_of_pair_start[ti + 1] : (int)tl.pairs.size();
+      for (int p = p0; p < p1; ++p) {
+        const long long k = tl.pairs[p].first;
+        prs[(size_t)(tile0[L] + ti) * 16 + (p - p0)] =
+            (int)(k >> 32) | ((int)(k & 0xffffffff) << 16);
+        where[p] = {tile0[L] + ti, p - p0};
+      }
+    }
+    // triangle slots: item * 256 + row * 16 + column, level in bit 30
+    std::vector<long long> keys(tl.pairs.size());
+    for (size_t p = 0; p < keys.size(); ++p) keys[p] = tl.pairs[p].first;
+    std::vector<size_t> kidx(keys.size());
+    for (size_t p = 0; p < kidx.size(); ++p) kidx[p] = p;
+    std::sort(kidx.begin(), kidx.end(), [&](size_t x, size_t y) { return keys[x] < keys[y]; });
+    for (size_t t = 0; t < lp[L].tri.size(); ++t) {
+      const Role& r = lp[L].roles[t];
+      const long long k = pkey(r.a, r.b);
+      auto it = std::lower_bound(kidx.begin(), kidx.end(), k,
+                                 [&](size_t x, long long v) { return keys[x] < v; });
+      const int p = (int)*it;
+      const int ti = where[p].first, row = where[p].second, c = r.c;
+      const int g = item_of[(size_t)ti * nkb + c / 16];
+      q->slot[lp[L].tri[t]] = g * 256 + row * 16 + (c % 16) + (L << 30);
+    }
+    npair += (int)tl.pairs.size();
+    nitem += tl.nitem;
+  }
   q->mode = mode;
   q->nshell = nshell;
   q->sets = sets;
   q->ntri = ntri;
-  q->npair = (int)tiles.pairs.size();
+  q->npair = npair;
   q->ntile = ntile;
-  q->nitem = tiles.nitem;
+  q->nitem = nitem;
   q->nkb = nkb;
   q->ni = ni;
   q->npass = npass;
+  q->npass0 = npass0;
   q->item_total = npass * W * ni;
-  q->slot.resize(ntri);
-  for (int t = 0; t < ntri; ++t) {
-    const long long k = pkey(roles[t].a, roles[t].b);
-    auto it = std::lower_bound(kidx.begin(), kidx.end(), k,
-                               [&](size_t x, long long v) { return keys[x] < v; });
-    const int p = (int)*it;
-    const int ti = where[p].first, row = where[p].second, c = roles[t].c;
-    const int g = item_of[(size_t)ti * nkb + c / 16];
-    q->slot[t] = g * 256 + row * 16 + (c % 16);
-  }
   q->h_items.swap(items);
   q->h_pairs.swap(prs);
   *out = q;
+  return BSK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsk_pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri) {
+  return pairs_create(out, mode, nshell, ntri, tri, nullptr, "bsk_pairs_create");
+}
+
+int bsk_pairs_create_levels(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri,
+                            const int* level) {
+  if (ntri > 0 && !level) {
+    bsk_set_error("bsk_pairs_create_levels: null level list");
+    return BSK_EINVAL;
+  }
+  return pairs_create(out, mode, nshell, ntri, tri, level, "bsk_pairs_create_levels");
+}
+
+int bsk_pairs_level_passes(const bsk_pairs* q, int* npass0, int* npass1) {
+  if (!q) {
+    bsk_set_error("bsk_pairs_level_passes: null plan");
+    return BSK_EINVAL;
+  }
+  if (npass0) *npass0 = q->npass0;
+  if (npass1) *npass1 = q->npass - q->npass0;
   return BSK_OK;
 }
 
@@ -923,38 +1053,49 @@ int bsk_triples_mfma_workspace(const bsk_pairs* q, int nwg, size_t* bytes) {
   return BSK_OK;
 }
 
-int bsk_triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
-                     const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
-                     const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
-                     double scale, double* out_dev, void* work, void* stream) {
+namespace {
+int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
+                 const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
+                 const uint16_t* lo_k, long long shell_stride, long long ncell,
+                 long long dec_offset, long long dec_ncell, int nwg, double scale,
+                 double* out_dev, void* work, void* stream, const char* fn) {
   if (!q || !hi_j || !lo_j || !work || nwg <= 0 || shell_stride < ncell ||
       (q->mode >= 1 && (!hi_k || !lo_k)) || (q->mode == 2 && (!hi_i || !lo_i))) {
-    bsk_set_error("bsk_triples_mfma: invalid argument");
+    bsk_set_error("%s: invalid argument", fn);
+    return BSK_EINVAL;
+  }
+  const bool lv1 = q->npass > q->npass0;
+  if (lv1 && (dec_ncell <= 0 || dec_ncell % kPairChunk || dec_offset < 0 ||
+              shell_stride < dec_offset + dec_ncell)) {
+    bsk_set_error("%s: the plan has decimated-mesh triangles; pass the decimated copy's "
+                  "offset and cell count (bsk_plan_decimated)", fn);
     return BSK_EINVAL;
   }
   if (q->ntri == 0) return BSK_OK;
   if (!q->d_items) {
-    bsk_set_error("bsk_triples_mfma: pair plan not uploaded (bsk_pairs_upload)");
+    bsk_set_error("%s: pair plan not uploaded (bsk_pairs_upload)", fn);
     return BSK_EINVAL;
   }
   PairArgs a;
   // roles: set 0 / 1 = pair sides, set 2 = third side
-  a.hi[0] = q->mode == 2 ? hi_i : hi_j;
-  a.lo[0] = q->mode == 2 ? lo_i : lo_j;
-  a.hi[1] = hi_j;
-  a.lo[1] = lo_j;
-  a.hi[2] = q->mode >= 1 ? hi_k : hi_j;
-  a.lo[2] = q->mode >= 1 ? lo_k : lo_j;
+  const uint16_t* hi[3] = {q->mode == 2 ? hi_i : hi_j, hi_j, q->mode >= 1 ? hi_k : hi_j};
+  const uint16_t* lo[3] = {q->mode == 2 ? lo_i : lo_j, lo_j, q->mode >= 1 ? lo_k : lo_j};
   a.stride = shell_stride;
   for (int r = 0; r < 3; ++r) a.nsh[r] = q->nshell;
   a.nkb = q->nkb;
-  a.ncell = ncell;
   a.nwg = nwg;
   a.pairs = q->d_pairs;
   a.partial = reinterpret_cast<float*>(work);
   a.item_total = q->item_total;
   hipStream_t s = as_stream(stream);
   for (int p = 0; p < q->npass; ++p) {
+    // level 0: the full mesh; level 1: each shell's decimated copy
+    const long long off = p < q->npass0 ? 0 : dec_offset;
+    for (int r = 0; r < 3; ++r) {
+      a.hi[r] = hi[r] + off;
+      a.lo[r] = lo[r] + off;
+    }
+    a.ncell = p < q->npass0 ? ncell : dec_ncell;
     a.item0 = p * kPairWaves * q->ni;
     a.items = q->d_items + a.item0;
     int rc = bsk_triple_pairs_launch(a, q->sets, q->ni, s);
@@ -964,12 +1105,31 @@ int bsk_triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* l
   return bsk_reduce_mfma_launch(a.partial, nwg, q->item_total, q->ntri, q->d_slot, scale,
                                 out_dev, s);
 }
+}  // namespace
+
+int bsk_triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
+                     const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
+                     const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
+                     double scale, double* out_dev, void* work, void* stream) {
+  return triples_mfma(q, hi_i, lo_i, hi_j, lo_j, hi_k, lo_k, shell_stride, ncell, -1, 0, nwg,
+                      scale, out_dev, work, stream, "bsk_triples_mfma");
+}
+
+int bsk_triples_mfma_levels(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
+                            const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
+                            const uint16_t* lo_k, long long shell_stride, long long ncell,
+                            long long dec_offset, long long dec_ncell, int nwg, double scale,
+                            double* out_dev, void* work, void* stream) {
+  return triples_mfma(q, hi_i, lo_i, hi_j, lo_j, hi_k, lo_k, shell_stride, ncell, dec_offset,
+                      dec_ncell, nwg, scale, out_dev, work, stream, "bsk_triples_mfma_levels");
+}
 
 namespace {
 // Why the fused z/triple kernel cannot run this plan (nullptr: it can).
 const char* zfused_gap(const bsk_plan* p, const bsk_pairs* q) {
   if (!p || !q) return "null plan";
   if (q->mode != BSK_MODE_AUTO) return "auto spectra only";
+  if (q->npass > q->npass0) return "the plan has decimated-mesh triangles";
   if (q->nshell > kZMaxShells) return "too many shells";
   if (q->ni < 1 || q->ni > 4) return "items per wave above 4";
   bool size = false;

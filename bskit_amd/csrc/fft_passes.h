@@ -391,6 +391,34 @@ BSK_DEV void store_quad<double>(int fmt, void* out, void*, long long off, cplx<d
   o[1] = make_double2(v.x, v.y);
 }
 
+// Two consecutive reals at element offset off (even), and one zero.
+template <typename T>
+BSK_DEV void store_pair(int fmt, void* out, void* out_lo, long long off, T a, T b,
+                        const SplitConsts& c) {
+  if constexpr (sizeof(T) == 8) {
+    reinterpret_cast<double2*>(out)[off >> 1] = make_double2(a, b);
+  } else if (fmt == BSK_FMT_BF16X2) {
+    uint32_t h, l;
+    bf16_split2(a, b, h, l, c);
+    reinterpret_cast<uint32_t*>(out)[off >> 1] = h;
+    reinterpret_cast<uint32_t*>(out_lo)[off >> 1] = l;
+  } else {
+    reinterpret_cast<float2*>(out)[off >> 1] = make_float2(a, b);
+  }
+}
+
+template <typename T>
+BSK_DEV void store_zero(int fmt, void* out, void* out_lo, long long off) {
+  if constexpr (sizeof(T) == 8) {
+    reinterpret_cast<double*>(out)[off] = 0.0;
+  } else if (fmt == BSK_FMT_BF16X2) {
+    reinterpret_cast<uint16_t*>(out)[off] = 0;
+    reinterpret_cast<uint16_t*>(out_lo)[off] = 0;
+  } else {
+    reinterpret_cast<float*>(out)[off] = 0.f;
+  }
+}
+
 constexpr int pow2_at_least(int v) {
   int q = 1;
   while (q < v) q *= 2;
@@ -399,12 +427,21 @@ constexpr int pow2_at_least(int v) {
 
 // Lanes map to (row, k) with a power-of-two lane count per row (KQ for the
 // KH mirrored pairs, QQ for the NH/2 output quads), so no index divides.
+//
+// dec_off >= 0 (whole mesh: nrows = N^2, cell0 = 0): each shell also gets
+// its decimated copy, the cells with even x, y and z, as an NH^3 mesh in C
+// order at element dec_off of the shell, followed by zeros up to a multiple
+// of 128 cells.  The decimated copy is the same field sampled on the NH^3
+// mesh (every retained frequency is an integer vector), so sums over it of
+// products whose frequencies cannot alias to zero on NH^3 equal the full
+// sums / 8.
 template <typename T, int NH, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, void* out,
                                              void* out_lo, const cplx<T>* __restrict__ twg,
                                              int MZ, int MZP, long long nrows, long long cell0,
                                              long long shell_stride, int fmt,
-                                             const int* __restrict__ n2hi, int shell0) {
+                                             const int* __restrict__ n2hi, int shell0,
+                                             long long dec_off) {
   constexpr int N = 2 * NH;
   constexpr int LD = pencil_ld<NH>();
   constexpr int KH = NH / 2 + 1;  // k = 0..NH/2 (each with its mirror NH - k)
@@ -421,6 +458,10 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   const long long bl = blockIdx.y;
   const int mzb = shell_mz(MZ, n2hi[shell0 + bl]);  // kz from mzb on are zero
   const cplx<T>* src = in + (bl * nrows + row0) * MZP;
+  constexpr long long DC = (long long)NH * NH * NH, DCP = (DC + 127) / 128 * 128;
+  if (dec_off >= 0 && blockIdx.x == 0)
+    for (int e = tid; e < (int)(DCP - DC); e += W)
+      store_zero<T>(fmt, out, out_lo, bl * shell_stride + dec_off + DC + e);
   const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
   // (k, NH - k) pairs: lane k of rows r0, r0 + RK, ...; up to 8 rows (16
   // loads) in flight per batch
@@ -456,6 +497,14 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
       if (p >= nr) continue;
       const cplx<T>* b = buf + p * LD + 2 * q;
       store_quad<T>(fmt, out, out_lo, obase + (long long)p * N + 4 * q, b[0], b[1], sc);
+      if (dec_off >= 0) {  // z = 4q and 4q + 2 of rows with even x and y
+        const int row = (int)row0 + p, x = row / N, y = row - x * N;
+        if (((x | y) & 1) == 0)
+          store_pair<T>(fmt, out, out_lo,
+                        bl * shell_stride + dec_off + ((long long)(x >> 1) * NH + (y >> 1)) * NH +
+                            2 * q,
+                        b[0].x, b[1].x, sc);
+      }
     }
   }
 }

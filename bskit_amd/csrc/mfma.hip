@@ -192,12 +192,16 @@ __global__ __launch_bounds__(64) void k_reduce_mfma(const float* __restrict__ pa
                                                     long long wg_stride, const int* __restrict__ slot,
                                                     double scale, double* __restrict__ out) {
   const int t = blockIdx.x;
-  const long long e = slot[t];
+  const int v = slot[t];
+  const long long e = v & 0x3fffffff;
+  // bit 30: summed over the decimated mesh (every second cell per axis), one
+  // eighth of the cells of the full-mesh sum
+  const double f = (v >> 30) & 1 ? 8.0 : 1.0;
   double s = 0.0;
   for (int w = threadIdx.x; w < nwg; w += 64) s += (double)partial[w * wg_stride + e];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (threadIdx.x == 0) out[t] = s * scale;
+  if (threadIdx.x == 0) out[t] = s * scale * f;
 }
 
 template <int NI, int PV, int SETS>

@@ -80,33 +80,58 @@ class Workspace:
 
 
 class ShellSet:
-    """nb shell fields of one input field, [nb, N^3] in one storage format."""
+    """nb shell fields of one input field in one storage format: rows of
+    ``stride`` elements, the S^3 cells first, then (``dec`` = (offset,
+    ncell)) the decimated copy of each shell."""
 
-    def __init__(self, fmt, data, lo=None):
+    def __init__(self, fmt, data, lo=None, ncell=None, dec=None):
         self.fmt = fmt
-        self.data = data      # F32/F64: [nb, ncell] float; BF16X2: hi plane int16
+        self.data = data      # F32/F64: [nb, stride] float; BF16X2: hi plane int16
         self.lo = lo          # BF16X2 lo plane
         self.nb = data.shape[0]
-        self.ncell = data.shape[1]
+        self.stride = data.shape[1]
+        self.ncell = self.stride if ncell is None else int(ncell)
+        self.dec = dec
+
+    @staticmethod
+    def _f(v):
+        return (v.to(torch.int32) << 16).view(torch.float32)
 
     def as_float(self):
-        """FP32/FP64 view of the values (hi + lo for BF16X2), for inspection."""
+        """FP32/FP64 view of the S^3 cells (hi + lo for BF16X2), for inspection."""
         if self.fmt != L.FMT_BF16X2:
-            return self.data
-        def f(p):
-            return (p.to(torch.int32) << 16).view(torch.float32)
-        return f(self.data) + f(self.lo)
+            return self.data[:, :self.ncell]
+        return self._f(self.data[:, :self.ncell]) + self._f(self.lo[:, :self.ncell])
+
+    def decimated_float(self):
+        """The decimated copies [nb, (S/2)^3 padded] as FP32 (hi + lo), or None."""
+        if self.dec is None:
+            return None
+        o, n = self.dec
+        if self.fmt != L.FMT_BF16X2:
+            return self.data[:, o:o + n]
+        return self._f(self.data[:, o:o + n]) + self._f(self.lo[:, o:o + n])
 
 
 class PairPlan:
-    """Owner of a bsk_pairs plan: the MFMA kernel's tiles for one triangle list."""
+    """Owner of a bsk_pairs plan: the MFMA kernel's tiles for one triangle list
+    (with ``levels``: 1 marks triangles summed over the decimated mesh)."""
 
-    def __init__(self, tri, nshell, mode, upload=True):
+    def __init__(self, tri, nshell, mode, upload=True, levels=None):
         tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
         h = ctypes.c_void_p()
-        L.call('bsk_pairs_create', ctypes.byref(h), int(mode), int(nshell), len(tri),
-               tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        ip = ctypes.POINTER(ctypes.c_int)
+        if levels is None:
+            L.call('bsk_pairs_create', ctypes.byref(h), int(mode), int(nshell), len(tri),
+                   tri.ctypes.data_as(ip))
+        else:
+            levels = np.ascontiguousarray(np.asarray(levels, dtype=np.int32).reshape(-1))
+            L.call('bsk_pairs_create_levels', ctypes.byref(h), int(mode), int(nshell), len(tri),
+                   tri.ctypes.data_as(ip), levels.ctypes.data_as(ip))
         self.handle = h
+        p0, p1 = ctypes.c_int(), ctypes.c_int()
+        L.call('bsk_pairs_level_passes', h, ctypes.byref(p0), ctypes.byref(p1))
+        self.npass_full, self.npass_dec = p0.value, p1.value
         if upload:
             L.call('bsk_pairs_upload', h)
         self.ntri = len(tri)
@@ -177,6 +202,9 @@ class GridEngine:
         self.tuned_nwg = None
         self.tuned_zfused_nwg = None
         self.prefer_zfused = False  # set by the autotuner when the fused kernel is faster
+        # BF16X2 shell sets carry decimated copies and the MFMA sums run the
+        # triangles that are exact there over an eighth of the cells
+        self.levels = True
         self.cic_window = 0
         if cic_window:
             self.set_compensation(cic_window)
@@ -200,6 +228,11 @@ class GridEngine:
         L.call('bsk_plan_shell_mesh', self._plan, ctypes.byref(s))
         self.S = s.value
         self.ncell = self.S ** 3
+        self.dec = None  # (offset, cells) of a shell's decimated copy
+        if self.S % 2 == 0:
+            o, n = ctypes.c_longlong(), ctypes.c_longlong()
+            L.call('bsk_plan_decimated', self._plan, ctypes.byref(o), ctypes.byref(n))
+            self.dec = (o.value, n.value)
 
     def set_shell_mesh(self, shell_mesh=0):
         """Shell mesh size (0 = automatic, Nmesh = the reference's own mesh)."""
@@ -260,18 +293,36 @@ class GridEngine:
         nbytes = L.size_out()
         L.call('bsk_shells_workspace', self._plan, dt, nshell, self.S, ctypes.byref(nbytes))
         w = (work or self.work).get('shell', nbytes.value)
+        stages = L.STAGE_X | L.STAGE_YZ | (L.STAGE_DECIMATE if out.dec else 0)
         L.call('bsk_shells', self._plan, dt, kind, L.ptr(dk), shell0, nshell, 0, self.S,
-               L.STAGE_X | L.STAGE_YZ, fmt, L.ptr(out.data), L.ptr(out.lo), self.ncell, 0,
+               stages, fmt, L.ptr(out.data), L.ptr(out.lo), out.stride, 0,
                L.ptr(w), stream_handle(stream))
         return out
 
-    def alloc_shells(self, fmt, nb):
+    def alloc_shells(self, fmt, nb, decimated=None):
+        """Shell storage for nb shells; BF16X2 sets carry the decimated copies
+        (for the two-level MFMA sums) unless decimated=False or S is odd."""
+        if decimated is None:
+            decimated = self.uses_levels(fmt)
+        dec = self.dec if decimated else None
+        stride = self.ncell if dec is None else dec[0] + dec[1]
         if fmt == L.FMT_BF16X2:
-            hi = torch.empty((nb, self.ncell), dtype=torch.int16, device=self.device)
-            lo = torch.empty((nb, self.ncell), dtype=torch.int16, device=self.device)
-            return ShellSet(fmt, hi, lo)
+            hi = torch.empty((nb, stride), dtype=torch.int16, device=self.device)
+            lo = torch.empty((nb, stride), dtype=torch.int16, device=self.device)
+            return ShellSet(fmt, hi, lo, self.ncell, dec)
         dtype = torch.float32 if fmt == L.FMT_F32 else torch.float64
-        return ShellSet(fmt, torch.empty((nb, self.ncell), dtype=dtype, device=self.device))
+        return ShellSet(fmt, torch.empty((nb, stride), dtype=dtype, device=self.device),
+                        None, self.ncell, dec)
+
+    def triangle_levels(self, tri):
+        """1 for the triangles (bin indices) whose sums are exact on the
+        decimated mesh, else 0."""
+        tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
+        out = np.zeros(len(tri), dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.call('bsk_plan_triangle_levels', self._plan, len(tri), tri.ctypes.data_as(ip),
+               out.ctypes.data_as(ip))
+        return out
 
     # --------------------------------------------------------------- triples
     def triangles_device(self, tri):
@@ -283,14 +334,16 @@ class GridEngine:
             self._tri_cache[key] = t
         return t
 
-    def pair_plan(self, tri, nshell, mode):
-        """Cached MFMA pair plan of a triangle list (host tables uploaded once)."""
+    def pair_plan(self, tri, nshell, mode, levels=False):
+        """Cached MFMA pair plan of a triangle list (host tables uploaded once);
+        levels=True splits it between the full and the decimated mesh."""
         tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
-        key = (tri.tobytes(), 'pairs', nshell, mode)
+        key = (tri.tobytes(), 'pairs', nshell, mode, bool(levels))
         r = self._tri_cache.get(key)
         if r is None:
+            lv = self.triangle_levels(tri) if levels else None
             with torch.cuda.device(self.device):
-                r = PairPlan(tri, nshell, mode)
+                r = PairPlan(tri, nshell, mode, levels=lv)
             self._tri_cache[key] = r
         return r
 
@@ -390,6 +443,8 @@ class GridEngine:
             sa, sb, sc = sets
         else:
             raise ValueError('1, 2 or 3 shell sets')
+        if len({x.stride for x in sets}) != 1 or len({x.ncell for x in sets}) != 1:
+            raise ValueError('shell sets of one call need the same layout')
         s = stream_handle(stream)
         ncell = sa.ncell
         if path == PATH_MFMA:
@@ -398,20 +453,21 @@ class GridEngine:
                     raise ValueError('the MFMA path needs BF16X2 shells')
             mode = _MODE[nf]
             nwg = self.mfma_nwg(ncell) if nwg is None else int(nwg)
-            plan = self.pair_plan(tri, sa.nb, mode)
+            dec = sa.dec if all(x.dec == sa.dec for x in sets) else None
+            plan = self.pair_plan(tri, sa.nb, mode, levels=dec is not None)
+            dec = dec or (-1, 0)
             nbytes = L.size_out()
             L.call('bsk_triples_mfma_workspace', plan.handle, nwg, ctypes.byref(nbytes))
             w = (work or self.work).get('mfma', nbytes.value)
+            args = (plan.handle, L.ptr(sa.data), L.ptr(sa.lo), L.ptr(sb.data), L.ptr(sb.lo),
+                    L.ptr(sc.data), L.ptr(sc.lo), sa.stride, ncell, dec[0], dec[1], nwg,
+                    float(scale))
             if kernel_events is None:
-                L.call('bsk_triples_mfma', plan.handle, L.ptr(sa.data), L.ptr(sa.lo),
-                       L.ptr(sb.data), L.ptr(sb.lo), L.ptr(sc.data), L.ptr(sc.lo), ncell, ncell,
-                       nwg, float(scale), L.ptr(out), L.ptr(w), s)
+                L.call('bsk_triples_mfma_levels', *args, L.ptr(out), L.ptr(w), s)
                 return out
-            # events around the MFMA kernel alone (the reduction follows)
+            # events around the MFMA kernels alone (the reduction follows)
             kernel_events[0].record(stream)
-            L.call('bsk_triples_mfma', plan.handle, L.ptr(sa.data), L.ptr(sa.lo), L.ptr(sb.data),
-                   L.ptr(sb.lo), L.ptr(sc.data), L.ptr(sc.lo), ncell, ncell, nwg, float(scale),
-                   None, L.ptr(w), s)
+            L.call('bsk_triples_mfma_levels', *args, None, L.ptr(w), s)
             kernel_events[1].record(stream)
             L.call('bsk_triples_mfma_reduce', plan.handle, nwg, float(scale), L.ptr(out),
                    L.ptr(w), s)
@@ -428,7 +484,7 @@ class GridEngine:
             for t0 in range(0, T, 65535):
                 t1 = min(T, t0 + 65535)
                 L.call('bsk_triples_plain', dt, L.ptr(sa.data), L.ptr(sb.data), L.ptr(sc.data),
-                       ncell, ncell, t1 - t0, L.ptr(tri_d[t0:t1]), nchunk, float(scale),
+                       sa.stride, ncell, t1 - t0, L.ptr(tri_d[t0:t1]), nchunk, float(scale),
                        L.ptr(out[t0:t1]), L.ptr(w), s)
             return out
         if path == PATH_TILED:
@@ -441,7 +497,7 @@ class GridEngine:
             nbytes = L.size_out()
             L.call('bsk_triples_tiled_workspace', T, 1, nchunk, ctypes.byref(nbytes))
             w = (work or self.work).get('tiled', nbytes.value)
-            L.call('bsk_triples_tiled', dt, L.ptr(sa.data), ncell, sa.nb, ncell, T, L.ptr(tri_d), 1,
+            L.call('bsk_triples_tiled', dt, L.ptr(sa.data), sa.stride, sa.nb, ncell, T, L.ptr(tri_d), 1,
                    nchunk, float(scale), L.ptr(out), L.ptr(w), s)
             return out
         raise ValueError('unknown triple path %r' % (path,))
@@ -519,10 +575,17 @@ class GridEngine:
         return p, nmodes, kmean
 
     # ---------------------------------------------------------------- sizing
+    def uses_levels(self, fmt=L.FMT_BF16X2):
+        """Whether default shell sets of this format carry decimated copies
+        (and MFMA sums over them run the two-level plan)."""
+        return fmt == L.FMT_BF16X2 and self.levels and self.dec is not None
+
     def shell_bytes(self, fmt, nb=None):
+        """Device bytes of a default shell set (alloc_shells) of nb shells."""
         nb = self.nb if nb is None else nb
         per = 8 if fmt == L.FMT_F64 else 4
-        return nb * self.ncell * per
+        stride = self.dec[0] + self.dec[1] if self.uses_levels(fmt) else self.ncell
+        return nb * stride * per
 
 
 def cic_paint(positions, nmesh, box_size, to_contrast=True, out=None, stream=None):

@@ -93,7 +93,8 @@ class Server:
         fwd = sz.value
         L.call('bsk_shells_workspace', eng._plan, L.DT_F32, nb, eng.S, ctypes.byref(sz))
         shw = sz.value
-        shells = nfield * nb * eng.ncell * 4
+        fmt = L.FMT_BF16X2 if GridEngine.mfma_supported(nb, nfield, None) else L.FMT_F32
+        shells = nfield * eng.shell_bytes(fmt)
         spectra = nfield * eng.MX * eng.MY * eng.MZ * 8
         return int(fwd + shw + shells + spectra)
 
@@ -109,7 +110,7 @@ class Server:
         job.path = PATH_MFMA if GridEngine.mfma_supported(eng.nb, job.nfield, tri) else PATH_PLAIN
         if job.path == PATH_MFMA and len(tri):
             mode = {1: L.MODE_AUTO, 2: L.MODE_AAB}[job.nfield]
-            eng.pair_plan(tri, eng.nb, mode)
+            eng.pair_plan(tri, eng.nb, mode, levels=eng.uses_levels())
         eng.triangles_device(tri)
         job.footprint = self.footprint(eng, job.nfield)
         if job.footprint > self.budget:

@@ -62,7 +62,8 @@ def _save():
 
 def key_of(eng, ntri=0):
     dev = torch.cuda.get_device_name(eng.device)
-    return '%s|N%d|S%d|m%d|nb%d|T%d' % (dev, eng.N, eng.S, eng.m, eng.nb, ntri)
+    lv = int(bool(eng.levels and eng.dec is not None))  # two-level MFMA sums
+    return '%s|N%d|S%d|m%d|nb%d|T%d|L%d' % (dev, eng.N, eng.S, eng.m, eng.nb, ntri, lv)
 
 
 def lookup(eng, ntri=0):

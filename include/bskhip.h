@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define BSK_ABI_VERSION 4
+#define BSK_ABI_VERSION 5
 
 /* status codes */
 #define BSK_OK 0
@@ -93,6 +93,20 @@ int bsk_plan_thresholds(const bsk_plan* plan, int* n2lo, int* n2hi);
  * mesh. */
 int bsk_plan_shell_mesh(const bsk_plan* plan, int* shell_mesh);
 int bsk_plan_set_shell_mesh(bsk_plan* plan, int shell_mesh);
+/* Decimated shell mesh (even S): the cells of the shell mesh with even x,
+ * y and z form an (S/2)^3 mesh on which each shell field is the same
+ * band-limited sum of modes.  A triangle (i, j, k) whose shells' largest
+ * |n| components m_i + m_j + m_k stay below S/2 has no product frequency
+ * that aliases to zero there either, so its cell sum is exactly 8 times the
+ * sum over the decimated mesh; such triangles cost an eighth of the cells.
+ * bsk_plan_decimated: a shell's decimated copy sits at element *offset of
+ * the shell (S^3 rounded up to 128) and holds *ncell cells ((S/2)^3 in C
+ * order, zero-padded to a multiple of 128).  bsk_plan_triangle_levels:
+ * level[t] = 1 for triangles (bin indices) exact on the decimated mesh,
+ * else 0.  No reference counterpart: the reference sums every triangle
+ * over the full N^3 mesh (bskit/main.py:1871-1882). */
+int bsk_plan_decimated(const bsk_plan* plan, long long* offset, long long* ncell);
+int bsk_plan_triangle_levels(const bsk_plan* plan, int ntri, const int* tri_host, int* level);
 /* Radix-plan variant (0 or 1) of FFT pass 0..5 (z r2c, y, x, masked x^-1,
  * y^-1, z c2r); chosen by the autotuner. */
 int bsk_plan_set_variant(bsk_plan* plan, int pass, int variant);
@@ -119,7 +133,10 @@ int bsk_forward(bsk_plan* plan, int dtype, const void* field, void* dk, void* wo
  * the slab from that work (so one X stage serves many slabs); both = 3.
  * BSK_STAGE_Y instead of BSK_STAGE_YZ runs the y pass only and leaves its
  * output in work for bsk_triples_zfused (FP32, x0 = 0, xslab = S; out
- * unused).
+ * unused).  BSK_STAGE_DECIMATE with BSK_STAGE_YZ over the whole mesh
+ * (x0 = 0, xslab = S, out_cell0 = 0, even S) also writes each shell's
+ * decimated copy (bsk_plan_decimated) after its S^3 cells; shell_stride
+ * must hold both.
  * Output shell b (relative to shell0) lands at
  * out + (shell0+b)*shell_stride + out_cell0 (elements) with the slab's cells
  * in C order; for BSK_FMT_BF16X2 out is the hi plane and out_lo the lo
@@ -131,6 +148,7 @@ int bsk_forward(bsk_plan* plan, int dtype, const void* field, void* dk, void* wo
 #define BSK_STAGE_X 1
 #define BSK_STAGE_YZ 2
 #define BSK_STAGE_Y 4
+#define BSK_STAGE_DECIMATE 8
 int bsk_shells_workspace(const bsk_plan* plan, int dtype, int nshell, int xslab, size_t* bytes);
 int bsk_shells(bsk_plan* plan, int dtype, int kind, const void* dk, int shell0, int nshell, int x0,
                int xslab, int stages, int fmt, void* out, void* out_lo, long long shell_stride,
@@ -194,7 +212,15 @@ int bsk_pairs_upload(bsk_pairs* pairs);
 int bsk_pairs_destroy(bsk_pairs* pairs);
 int bsk_pairs_info(const bsk_pairs* pairs, int* npair, int* ntile, int* nitem,
                    int* items_per_wave, int* npass);
-/* Partial-buffer slot of each triangle (item * 256 + row * 16 + column). */
+/* Two-level plan: triangles with level[t] = 1 (bsk_plan_triangle_levels)
+ * are summed over the shells' decimated copies and scaled by 8, the others
+ * over the full mesh; run it with bsk_triples_mfma_levels.  npass0 / npass1:
+ * launches over the full / decimated mesh. */
+int bsk_pairs_create_levels(bsk_pairs** pairs, int mode, int nshell, int ntri,
+                            const int* tri_host, const int* level);
+int bsk_pairs_level_passes(const bsk_pairs* pairs, int* npass0, int* npass1);
+/* Partial-buffer slot of each triangle (item * 256 + row * 16 + column,
+ * plus 1 << 30 for a decimated-mesh triangle). */
 int bsk_pairs_slots(const bsk_pairs* pairs, int* slot_host);
 /* Host copies of the launch tables, for inspection: items
  * [npass * 8 * items_per_wave] (tile | kb << 16 | padding flag << 29 |
@@ -207,6 +233,15 @@ int bsk_triples_mfma(const bsk_pairs* pairs, const uint16_t* hi_i, const uint16_
                      const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
                      const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
                      double scale, double* out_dev, void* work, void* stream);
+/* bsk_triples_mfma with the decimated copies of the same shells: level-1
+ * triangles of a two-level plan run over cells [dec_offset, dec_offset +
+ * dec_ncell) of each shell (bsk_plan_decimated), the rest over [0, ncell).
+ * bsk_triples_mfma refuses a plan with level-1 triangles. */
+int bsk_triples_mfma_levels(const bsk_pairs* pairs, const uint16_t* hi_i, const uint16_t* lo_i,
+                            const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
+                            const uint16_t* lo_k, long long shell_stride, long long ncell,
+                            long long dec_offset, long long dec_ncell, int nwg, double scale,
+                            double* out_dev, void* work, void* stream);
 /* out_dev == NULL in bsk_triples_mfma leaves the per-workgroup partials in
  * work; bsk_triples_mfma_reduce then sums them in FP64 into out_dev (the
  * same result as one call with out_dev). */

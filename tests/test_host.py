@@ -178,9 +178,11 @@ def test_library_loads_and_host_calls_work():
         PairPlan(np.array([[3, 0, 0]]), 3, L.MODE_AUTO, upload=False)
 
 
-def _emulate_pair_plan(plan, sets, tri, mode, nshell):
+def _emulate_pair_plan(plan, sets, tri, mode, nshell, dec_sets=None):
     """FP64 numpy emulation of the MFMA kernel's tables: per item the 16x16
-    tile sum_x (d_a d_b)(x) d_k(x), read back through the triangle slots."""
+    tile sum_x (d_a d_b)(x) d_k(x), read back through the triangle slots.
+    Items of the decimated-mesh passes (after npass_full) sum dec_sets; the
+    slot's bit 30 scales by 8."""
     items, prs = plan.tables()
     # the kernel re-forms pair products only at flagged items: an unflagged
     # item continues the previous item's tile in the same wave's list
@@ -195,13 +197,14 @@ def _emulate_pair_plan(plan, sets, tri, mode, nshell):
                 break
             if lst[s] >= 0:
                 assert (lst[s] & 0xffff) == (lst[s - 1] & 0xffff)
-    f0, f1, f2 = sets
-    z = np.zeros_like(f0[0])
     nb = nshell
     part = np.zeros((len(items), 16, 16))
     for g, d in enumerate(items):
         if (d >> 29) & 1:
             continue  # padding
+        dec = g // (8 * ni) >= plan.npass_full
+        f0, f1, f2 = dec_sets if dec else sets
+        z = np.zeros_like(f0[0])
         tile = d & 0xffff
         kb = (d >> 16) & 0xff
         for row in range(16):
@@ -213,7 +216,8 @@ def _emulate_pair_plan(plan, sets, tri, mode, nshell):
                 if k < nb:
                     part[g, row, col] = np.sum(pa * f2[k])
     flat = part.reshape(-1)
-    return flat[plan.slots()]
+    slots = plan.slots()
+    return flat[slots & 0x3fffffff] * np.where((slots >> 30) & 1, 8.0, 1.0)
 
 
 @pytest.mark.parametrize('mode,nshell,ntri,seed', [(0, 7, 40, 1), (0, 33, 300, 2), (1, 20, 150, 3),
@@ -238,6 +242,53 @@ def test_pair_plan_tables_compute_every_triangle(mode, nshell, ntri, seed):
     got = _emulate_pair_plan(plan, sets, tri, mode, nshell)
     ref = np.array([np.sum(sets[0][i] * sets[1][j] * sets[2][k]) for i, j, k in tri])
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize('mode,nshell,ntri,seed,frac', [(0, 33, 400, 11, 0.3), (0, 20, 150, 12, 1.0),
+                                                        (1, 20, 150, 13, 0.5),
+                                                        (2, 18, 200, 14, 0.0)])
+def test_two_level_pair_plan(mode, nshell, ntri, seed, frac):
+    """A two-level plan: level-1 triangles are read from the decimated-mesh
+    passes (their own fields here) and scaled by 8, the rest from the full
+    mesh; level 0 passes come first."""
+    from bskit_amd import _lib as L
+    from bskit_amd.engine import PairPlan
+    try:
+        L.lib()
+    except (ImportError, OSError) as e:
+        pytest.skip('library not loadable here: %s' % e)
+    rng = np.random.default_rng(seed)
+    nset = {0: 1, 1: 2, 2: 3}[mode]
+    full = [rng.standard_normal((nshell, 24)) for _ in range(nset)]
+    dec = [rng.standard_normal((nshell, 6)) for _ in range(nset)]
+    roles = {0: (0, 0, 0), 1: (0, 0, 1), 2: (0, 1, 2)}[mode]
+    tri = rng.integers(0, nshell, size=(ntri, 3))
+    levels = (rng.random(ntri) < frac).astype(np.int32)
+    plan = PairPlan(tri, nshell, mode, upload=False, levels=levels)
+    assert (plan.npass_dec > 0) == bool(levels.any())
+    assert (plan.npass_full > 0) == bool((levels == 0).any())
+    sets = [full[r] for r in roles]
+    dsets = [dec[r] for r in roles]
+    got = _emulate_pair_plan(plan, sets, tri, mode, nshell, dsets)
+    ref = np.array([8 * np.sum(dsets[0][i] * dsets[1][j] * dsets[2][k]) if lv else
+                    np.sum(sets[0][i] * sets[1][j] * sets[2][k])
+                    for (i, j, k), lv in zip(tri, levels)])
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+    # a plan without levels, or with all zeros, has no decimated passes
+    p0 = PairPlan(tri, nshell, mode, upload=False, levels=np.zeros(ntri, np.int32))
+    assert p0.npass_dec == 0 and np.array_equal(p0.slots() >> 30, np.zeros(ntri))
+
+
+def test_two_level_pair_plan_rejects_bad_levels():
+    from bskit_amd import _lib as L
+    from bskit_amd.engine import PairPlan
+    try:
+        L.lib()
+    except (ImportError, OSError) as e:
+        pytest.skip('library not loadable here: %s' % e)
+    tri = np.array([[0, 1, 2], [1, 1, 1]])
+    with pytest.raises(L.BskError, match='level'):
+        PairPlan(tri, 4, 0, upload=False, levels=np.array([0, 2], np.int32))
 
 
 @pytest.mark.parametrize('mode,nshell', [(2, 70), (1, 100)])
