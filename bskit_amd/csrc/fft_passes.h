@@ -103,6 +103,20 @@ BSK_DEV void stage_in(LoadF&& load, StoreF&& store) {
   }
 }
 
+// Element e of a workgroup-uniform base through a 32-bit byte offset (the
+// SGPR base + VGPR offset address form, no 64-bit address arithmetic per
+// lane); every offset below stays under 4 GB.
+template <typename V>
+BSK_DEV V get_u32(const V* base, unsigned e) {
+  return *reinterpret_cast<const V*>(reinterpret_cast<const char*>(base) +
+                                     e * (unsigned)sizeof(V));
+}
+
+template <typename V>
+BSK_DEV void set_u32(V* base, unsigned e, V v) {
+  *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + e * (unsigned)sizeof(V)) = v;
+}
+
 template <typename T, int N, int W>
 BSK_DEV void load_twiddles(cplx<T>* tw, const cplx<T>* __restrict__ twg) {
   stage_in<T, N, W>([&](int i) { return twg[i]; }, [&](int i, cplx<T> v) { tw[i] = v; });
@@ -126,7 +140,7 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
   const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
   stage_in<T, P * NH, W>(
       [&](int e) {
-        return e < nr * NH ? src[e] : cplx<T>{(T)0, (T)0};
+        return e < nr * NH ? get_u32(src, (unsigned)e) : cplx<T>{(T)0, (T)0};
       },
       [&](int e, cplx<T> v) {
         const int p = e / NH;
@@ -134,16 +148,16 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
       });
   __syncthreads();
   fft_lds<T, NH, false, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
+  cplx<T>* const dst = out + row0 * MZP;
   for (int e = tid; e < P * MZ; e += W) {
     const int p = e / MZ, k = e - p * MZ;
-    const long long row = row0 + p;
-    if (row >= nrows) continue;
+    if (p >= nr) continue;
     cplx<T> zk = buf[p * LD + (k % NH)];
     cplx<T> zc = cconj(buf[p * LD + ((NH - k) % NH)]);
     cplx<T> xe = cscale(cadd(zk, zc), (T)0.5);
     cplx<T> d = csub(zk, zc);
     cplx<T> xo = {(T)0.5 * d.y, (T)-0.5 * d.x};  // (zk - zc) / (2i)
-    out[row * MZP + k] = cadd(xe, cmul(tw[k], xo));
+    set_u32(dst, (unsigned)(p * MZP + k), cadd(xe, cmul(tw[k], xo)));
   }
 }
 
@@ -164,7 +178,7 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
   stage_in<T, P * N, W>(
       [&](int e) {
         const int y = e / P, p = e - y * P;
-        return p < np ? src[(long long)y * g.MZP + p] : cplx<T>{(T)0, (T)0};
+        return p < np ? get_u32(src, (unsigned)(y * g.MZP + p)) : cplx<T>{(T)0, (T)0};
       },
       [&](int e, cplx<T> v) {
         const int y = e / P, p = e - y * P;
@@ -172,11 +186,12 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
       });
   __syncthreads();
   fft_lds<T, N, false, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
+  cplx<T>* const dst = out + x * g.MY * g.MZP + kz0;
   for (int e = threadIdx.x; e < P * g.MY; e += W) {
-    const int r = e / P, p = e - r * P, kz = kz0 + p;
-    if (kz >= g.MZ) continue;
+    const int r = e / P, p = e - r * P;
+    if (p >= np) continue;
     const int idx = (g.nlo + r + N) % N;
-    out[(x * g.MY + r) * g.MZP + kz] = buf[p * LD + idx];
+    set_u32(dst, (unsigned)(r * g.MZP + p), buf[p * LD + idx]);
   }
 }
 
@@ -197,11 +212,11 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
   const int kz0 = bk.x * P;
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
   const cplx<T>* src = in + ry * g.MZP + kz0;
-  const long long xstride = (long long)g.MY * g.MZP;
+  const int xstride = g.MY * g.MZP;
   stage_in<T, P * N, W>(
       [&](int e) {
         const int x = e / P, p = e - x * P;
-        return p < np ? src[x * xstride + p] : cplx<T>{(T)0, (T)0};
+        return p < np ? get_u32(src, (unsigned)(x * xstride + p)) : cplx<T>{(T)0, (T)0};
       },
       [&](int e, cplx<T> v) {
         const int x = e / P, p = e - x * P;
@@ -210,13 +225,14 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
   __syncthreads();
   fft_lds<T, N, false, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
   const int ny = g.nlo + (int)ry;
+  cplx<T>* const dst = out + ry * g.MX * g.MZ + kz0;
   for (int e = threadIdx.x; e < P * g.MX; e += W) {
     const int r = e / P, p = e - r * P, kz = kz0 + p;
-    if (kz >= g.MZ) continue;
+    if (p >= np) continue;
     const int nx = g.nlo + r;
     T s = scale;
     if (comp) s *= comp[nx + N / 2] * comp[ny + N / 2] * comp[kz + N / 2];
-    out[(ry * g.MX + r) * g.MZ + kz] = cscale(buf[p * LD + (nx + N) % N], s);
+    set_u32(dst, (unsigned)(r * g.MZ + p), cscale(buf[p * LD + (nx + N) % N], s));
   }
 }
 
@@ -252,7 +268,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
           const int n2 = nx * nx + ny * ny + kz * kz;
           if (n2 >= lo && n2 <= hi) {
             if (kind == BSK_KIND_DELTA)
-              v = src[rx * g.MZ + kz];
+              v = get_u32(src, (unsigned)(rx * g.MZ + kz));
             else if (kind == BSK_KIND_UNIT)
               v.x = (T)1;
             else
@@ -267,12 +283,12 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
       });
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
-  cplx<T>* dst = out + ((long long)bl * N * g.MY + ry) * g.MZP;
-  const long long xstride = (long long)g.MY * g.MZP;
+  cplx<T>* const dst = out + ((long long)bl * N * g.MY + ry) * g.MZP + kz0;
+  const int xstride = g.MY * g.MZP;
   for (int e = threadIdx.x; e < P * N; e += W) {
     const int x = e / P, p = e - x * P, kz = kz0 + p;
     if (kz >= g.MZ) continue;
-    dst[x * xstride + kz] = buf[p * LD + x];
+    set_u32(dst, (unsigned)(x * xstride + p), buf[p * LD + x]);
   }
 }
 
@@ -303,7 +319,7 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
         const int yi = e / P, p = e - yi * P;
         const int ny = signed_freq(yi, N), ry = ny - g.nlo, kz = kz0 + p;
         return (p < np && ry >= 0 && ry < g.MY && ny * ny + kz * kz <= hi)
-                   ? src[ry * g.MZP + p]
+                   ? get_u32(src, (unsigned)(ry * g.MZP + p))
                    : cplx<T>{(T)0, (T)0};
       },
       [&](int e, cplx<T> v) {
@@ -312,11 +328,11 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
       });
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
-  const long long obase = (bl * xs + xl) * N;
+  cplx<T>* const dst = out + (bl * xs + xl) * N * g.MZP + kz0;
   for (int e = threadIdx.x; e < P * N; e += W) {
-    const int y = e / P, p = e - y * P, kz = kz0 + p;
-    if (kz >= g.MZ) continue;
-    out[(obase + y) * g.MZP + kz] = buf[p * LD + y];
+    const int y = e / P, p = e - y * P;
+    if (p >= np) continue;
+    set_u32(dst, (unsigned)(y * g.MZP + p), buf[p * LD + y]);
   }
 }
 
@@ -364,58 +380,65 @@ BSK_DEV void c2r_pair(cplx<T>* row, int k, int NH, cplx<T> xk, cplx<T> xj, cplx<
   if (j != k && j < NH) row[j] = {s.x + wd.y, wd.x - s.y};  // conj(s) + i conj(w d)
 }
 
-// Four consecutive reals at element offset off (a multiple of 4).
-template <typename T>
-BSK_DEV void store_quad(int fmt, void* out, void* out_lo, long long off, cplx<T> u, cplx<T> v,
-                        const SplitConsts& c);
+// Stores into output planes addressed from a workgroup-uniform base with a
+// 32-bit byte offset, which the compiler turns into the SGPR base + VGPR
+// offset address form (no 64-bit address arithmetic per lane).  e is an
+// element offset from the base; hi holds the values (F32/F64) or the bf16 hi
+// plane, lo the bf16 lo plane.
+template <typename V>
+BSK_DEV void put_u32(char* base, unsigned byte_off, V v) {
+  *reinterpret_cast<V*>(base + byte_off) = v;
+}
 
-template <>
-BSK_DEV void store_quad<float>(int fmt, void* out, void* out_lo, long long off, cplx<float> u,
-                               cplx<float> v, const SplitConsts& c) {
-  if (fmt == BSK_FMT_BF16X2) {
+// Bytes per stored element.
+template <typename T>
+BSK_DEV int out_esize(int fmt) {
+  return sizeof(T) == 8 ? 8 : (fmt == BSK_FMT_BF16X2 ? 2 : 4);
+}
+
+// Four consecutive reals at element offset e (a multiple of 4).
+template <typename T>
+BSK_DEV void store_quad(int fmt, char* hi, char* lo, unsigned e, cplx<T> u, cplx<T> v,
+                        const SplitConsts& c) {
+  if constexpr (sizeof(T) == 8) {
+    put_u32(hi, e * 8u, make_double2(u.x, u.y));
+    put_u32(hi, e * 8u + 16u, make_double2(v.x, v.y));
+  } else if (fmt == BSK_FMT_BF16X2) {
     uint32_t h0, l0, h1, l1;
     bf16_split2(u.x, u.y, h0, l0, c);
     bf16_split2(v.x, v.y, h1, l1, c);
-    reinterpret_cast<uint2*>(out)[off >> 2] = make_uint2(h0, h1);
-    reinterpret_cast<uint2*>(out_lo)[off >> 2] = make_uint2(l0, l1);
+    put_u32(hi, e * 2u, make_uint2(h0, h1));
+    put_u32(lo, e * 2u, make_uint2(l0, l1));
   } else {
-    reinterpret_cast<float4*>(out)[off >> 2] = make_float4(u.x, u.y, v.x, v.y);
+    put_u32(hi, e * 4u, make_float4(u.x, u.y, v.x, v.y));
   }
 }
 
-template <>
-BSK_DEV void store_quad<double>(int fmt, void* out, void*, long long off, cplx<double> u,
-                                cplx<double> v, const SplitConsts&) {
-  double2* o = reinterpret_cast<double2*>(out) + (off >> 1);
-  o[0] = make_double2(u.x, u.y);
-  o[1] = make_double2(v.x, v.y);
-}
-
-// Two consecutive reals at element offset off (even), and one zero.
+// Two consecutive reals at element offset e (even), and one zero.
 template <typename T>
-BSK_DEV void store_pair(int fmt, void* out, void* out_lo, long long off, T a, T b,
+BSK_DEV void store_pair(int fmt, char* hi, char* lo, unsigned e, T a, T b,
                         const SplitConsts& c) {
   if constexpr (sizeof(T) == 8) {
-    reinterpret_cast<double2*>(out)[off >> 1] = make_double2(a, b);
+    put_u32(hi, e * 8u, make_double2(a, b));
   } else if (fmt == BSK_FMT_BF16X2) {
     uint32_t h, l;
     bf16_split2(a, b, h, l, c);
-    reinterpret_cast<uint32_t*>(out)[off >> 1] = h;
-    reinterpret_cast<uint32_t*>(out_lo)[off >> 1] = l;
+    put_u32(hi, e * 2u, h);
+    put_u32(lo, e * 2u, l);
   } else {
-    reinterpret_cast<float2*>(out)[off >> 1] = make_float2(a, b);
+    put_u32(hi, e * 4u, make_float2(a, b));
   }
 }
 
 template <typename T>
-BSK_DEV void store_zero(int fmt, void* out, void* out_lo, long long off) {
+BSK_DEV void store_zero(int fmt, char* hi, char* lo, unsigned e) {
   if constexpr (sizeof(T) == 8) {
-    reinterpret_cast<double*>(out)[off] = 0.0;
+    put_u32(hi, e * 8u, 0.0);
   } else if (fmt == BSK_FMT_BF16X2) {
-    reinterpret_cast<uint16_t*>(out)[off] = 0;
-    reinterpret_cast<uint16_t*>(out_lo)[off] = 0;
+    put_u32(hi, e * 2u, (uint16_t)0);
+    put_u32(lo, e * 2u, (uint16_t)0);
   } else {
-    reinterpret_cast<float*>(out)[off] = 0.f;
+    put_u32(hi, e * 4u, 0.f);
   }
 }
 
@@ -458,10 +481,14 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   const long long bl = blockIdx.y;
   const int mzb = shell_mz(MZ, n2hi[shell0 + bl]);  // kz from mzb on are zero
   const cplx<T>* src = in + (bl * nrows + row0) * MZP;
+  const int esz = out_esize<T>(fmt);
+  // uniform bases: this shell's decimated copy, this workgroup's rows
+  char* const dhi = static_cast<char*>(out) + (bl * shell_stride + dec_off) * esz;
+  char* const dlo = out_lo ? static_cast<char*>(out_lo) + (bl * shell_stride + dec_off) * esz
+                           : nullptr;
   constexpr long long DC = (long long)NH * NH * NH, DCP = (DC + 127) / 128 * 128;
   if (dec_off >= 0 && blockIdx.x == 0)
-    for (int e = tid; e < (int)(DCP - DC); e += W)
-      store_zero<T>(fmt, out, out_lo, bl * shell_stride + dec_off + DC + e);
+    for (int e = tid; e < (int)(DCP - DC); e += W) store_zero<T>(fmt, dhi, dlo, (unsigned)(DC + e));
   const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
   // (k, NH - k) pairs: lane k of rows r0, r0 + RK, ...; up to 8 rows (16
   // loads) in flight per batch
@@ -476,9 +503,9 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
     for (int q = 0; q < B; ++q) {
       const int p = r0 + (q0 + q) * RK;
       const bool ok = q0 + q < NPK && kact && p < nr;
-      const cplx<T>* r = src + (long long)p * MZP;
-      xk[q] = (ok && k < mzb) ? r[k] : cplx<T>{(T)0, (T)0};
-      xj[q] = (ok && j < mzb) ? r[j] : cplx<T>{(T)0, (T)0};
+      const unsigned r = (unsigned)(p * MZP);
+      xk[q] = (ok && k < mzb) ? get_u32(src, r + k) : cplx<T>{(T)0, (T)0};
+      xj[q] = (ok && j < mzb) ? get_u32(src, r + j) : cplx<T>{(T)0, (T)0};
     }
 #pragma unroll
     for (int q = 0; q < B; ++q)
@@ -488,7 +515,9 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
   // two complex values (four reals) per lane and store
   const SplitConsts sc = split_consts();
-  const long long obase = bl * shell_stride + cell0 + row0 * N;
+  const long long obase = (bl * shell_stride + cell0 + row0 * N) * esz;
+  char* const ohi = static_cast<char*>(out) + obase;
+  char* const olo = out_lo ? static_cast<char*>(out_lo) + obase : nullptr;
   const int q = tid & (QQ - 1), rq = tid / QQ;
   if (q < NH / 2) {
 #pragma unroll
@@ -496,13 +525,11 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
       const int p = rq + i * RQ;
       if (p >= nr) continue;
       const cplx<T>* b = buf + p * LD + 2 * q;
-      store_quad<T>(fmt, out, out_lo, obase + (long long)p * N + 4 * q, b[0], b[1], sc);
+      store_quad<T>(fmt, ohi, olo, (unsigned)(p * N + 4 * q), b[0], b[1], sc);
       if (dec_off >= 0) {  // z = 4q and 4q + 2 of rows with even x and y
         const int row = (int)row0 + p, x = row / N, y = row - x * N;
         if (((x | y) & 1) == 0)
-          store_pair<T>(fmt, out, out_lo,
-                        bl * shell_stride + dec_off + ((long long)(x >> 1) * NH + (y >> 1)) * NH +
-                            2 * q,
+          store_pair<T>(fmt, dhi, dlo, (unsigned)(((x >> 1) * NH + (y >> 1)) * NH + 2 * q),
                         b[0].x, b[1].x, sc);
       }
     }
