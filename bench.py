@@ -255,7 +255,7 @@ def main():
     ap.add_argument('--no-zfused', action='store_true',
                     help='time the separate shell and triple-sum passes only')
     ap.add_argument('--no-levels', action='store_true',
-                    help='sum every triangle over the full shell mesh (no decimated copies)')
+                    help='sum every triangle over the whole shell mesh (no sublattice levels)')
     ap.add_argument('--jobs', type=int, default=2000, help='jobs in the serving run')
     ap.add_argument('--load', type=float, default=0.8, help='arrival rate / measured capacity')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='BYTES',
@@ -367,17 +367,18 @@ def main():
 
     # roofline of the dominant kernel: the split-BF16 MFMA pair kernel, timed
     # by HIP events around its launches inside every timed step.  The
-    # separate path runs the triangles exact on the decimated mesh over the
-    # shells' decimated copies (S/2)^3 and the rest over S^3, one launch per
-    # pass of each level; figures are per launch (grid totals / launches).
+    # separate path sums each triangle over the sparsest sublattice of the
+    # shell mesh on which its sum is exact (level L: S^3 / 2^L cells), one
+    # launch per pass of each level; figures are per launch (grid totals /
+    # launches).
     N = wl['N']
     ncell = float(eng.ncell)                     # shell-mesh cells S^3
     nsh = eng.nb
-    levels = eng.levels and eng.dec is not None
-    dcell = float((eng.S // 2) ** 3) if levels else 0.0
+    levels = eng.uses_levels()
     lv = eng.triangle_levels(wl['tri']) if levels else np.zeros(T, np.int32)
+    lcount = np.bincount(lv, minlength=L.LEVELS)
     # one multiply-add per triangle per cell summed
-    flops_sep = 2.0 * (np.sum(lv == 0) * ncell + np.sum(lv == 1) * dcell)
+    flops_sep = 2.0 * sum(lcount[k] * ncell / 2 ** k for k in range(L.LEVELS))
     flops_fused = 2.0 * T * ncell
 
     def roofline(kernel, grid_ms, alg_bytes, launches, flops, plan):
@@ -403,18 +404,20 @@ def main():
             plan = eng.pair_plan(wl['tri'], nsh, 0)
             return roofline('k_triple_zrows', ms, d2_bytes, plan.npass, flops_fused, plan)
         plan = eng.pair_plan(wl['tri'], nsh, 0, levels=levels)
-        # split-BF16 hi+lo of every shell read once, full mesh and decimated copy
-        dec_alg = float(eng.dec[1]) if plan.npass_dec else 0.0
-        return roofline('k_triple_pairs', ms, nsh * 4 * (ncell + dec_alg), plan.npass,
-                        flops_sep, plan)
+        # split-BF16 hi+lo of every shell read once per level summed over
+        cells = (sum(eng.layout.ncell[k] for k in range(L.LEVELS) if plan.level_passes[k])
+                 if levels else ncell)
+        return roofline('k_triple_pairs', ms, nsh * 4 * float(cells), plan.npass, flops_sep,
+                        plan)
 
     roof = path_roofline(main_path, float(np.mean(timer.pair_durations('kernel'))))
     if alt is not None:
         alt['roofline'] = path_roofline(alt['path'], alt['kernel_ms'])
     H = eng.MX * eng.MY * eng.MZ                 # compact delta(k) cube
     t_sh = (alt['shells_ms'] if main_path == 'zfused' else stage_ms['shells']) * 1e-3
-    # delta(k) read + shells (and their decimated copies) written
-    shells_alg = 8.0 * H + nsh * 4 * (ncell + (eng.dec[1] if levels else 0))
+    # delta(k) read + shells written (level layout: the cosets, then the
+    # all-even and all-odd cells again)
+    shells_alg = 8.0 * H + nsh * 4 * (float(eng.layout.stride) if levels else ncell)
     stage_gbs = {'shells_GBps': shells_alg / t_sh / 1e9,
                  'forward_GBps': (4.0 * N ** 3 + 8.0 * H) / (stage_ms['forward'] * 1e-3) / 1e9}
 
@@ -462,9 +465,10 @@ def main():
                    'path': ('split-BF16 MFMA (hi*hi+hi*lo+lo*hi), FP32 accumulate, FP64 '
                             'reduce; %s' % ('z c2r fused into the triple sums'
                                             if main_path == 'zfused' else
-                                            'shells on the shell mesh; %d of the triangles '
-                                            'summed over its decimated (S/2)^3 copy (exact)'
-                                            % int(lv.sum()))),
+                                            'shells on the shell mesh, each triangle summed '
+                                            'over the sparsest exact sublattice (triangles at '
+                                            '1, 1/2, 1/4, 1/8 of the cells: %s)'
+                                            % '/'.join(str(int(c)) for c in lcount))),
                    'parallelism': 'single GPU'},
         'fp32_plain': res_plain,
         'speedup_vs_fp32_plain': (grids_mfma / res_plain['grids_per_s']) if res_plain else None,

@@ -19,7 +19,8 @@ DT_F32, DT_F64 = 0, 1
 FMT_F32, FMT_BF16X2, FMT_F64 = 0, 1, 2
 KIND_DELTA, KIND_UNIT, KIND_KMAG = 0, 1, 2
 MODE_AUTO, MODE_AAB, MODE_ABC = 0, 1, 2
-STAGE_X, STAGE_YZ, STAGE_Y, STAGE_DECIMATE = 1, 2, 4, 8
+STAGE_X, STAGE_YZ, STAGE_Y, STAGE_LEVELS = 1, 2, 4, 8
+LEVELS = 4
 ABI_VERSION = 5
 
 _c_int, _c_ll, _c_double, _vp = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
@@ -38,7 +39,8 @@ _SIGS = {
     'bsk_plan_thresholds': [_vp, _int_p, _int_p],
     'bsk_plan_shell_mesh': [_vp, _int_p],
     'bsk_plan_set_shell_mesh': [_vp, ctypes.c_int],
-    'bsk_plan_decimated': [_vp, ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll)],
+    'bsk_plan_levels': [_vp, ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll),
+                        ctypes.POINTER(_c_ll)],
     'bsk_plan_triangle_levels': [_vp, _c_int, _int_p, _int_p],
     'bsk_plan_set_variant': [_vp, _c_int, _c_int],
     'bsk_plan_set_compensation': [_vp, _c_int],
@@ -57,7 +59,8 @@ _SIGS = {
                           _vp, _vp, _vp],
     'bsk_pairs_create': [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _int_p],
     'bsk_pairs_create_levels': [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _int_p, _int_p],
-    'bsk_pairs_level_passes': [_vp, _int_p, _int_p],
+    'bsk_pairs_level_passes': [_vp, _int_p],
+    'bsk_pairs_level_windows': [_vp, _int_p, _int_p],
     'bsk_pairs_upload': [_vp],
     'bsk_pairs_destroy': [_vp],
     'bsk_pairs_info': [_vp, _int_p, _int_p, _int_p, _int_p, _int_p],
@@ -67,8 +70,9 @@ _SIGS = {
     'bsk_triples_mfma': [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _c_int, _c_double, _vp,
                          _vp, _vp],
     'bsk_triples_mfma_reduce': [_vp, _c_int, _c_double, _vp, _vp, _vp],
-    'bsk_triples_mfma_levels': [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _c_ll, _c_ll,
-                                _c_int, _c_double, _vp, _vp, _vp],
+    'bsk_triples_mfma_levels': [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll,
+                                ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll), _c_int, _c_double,
+                                _vp, _vp, _vp],
     'bsk_triples_zfused_supported': [_vp, _vp, ctypes.POINTER(_c_int)],
     'bsk_triples_zfused': [_vp, _vp, _vp, _c_int, _c_int, _c_double, _vp, _vp, _vp],
     'bsk_cic_paint': [_c_int, _c_double, _c_ll, _vp, _vp, _c_int, _vp],

@@ -63,12 +63,44 @@ int auto_shell_mesh(int N, int m) {
   return (s > 0 && s < N) ? s : N;
 }
 
-// Decimated copy of a shell (the cells with even x, y, z: an (S/2)^3 mesh)
-// at this element offset within the shell, padded to a multiple of 128
-// cells (the MFMA kernel's chunk).
+// Sublattice level layout of a shell (the parts of ShellLevels in
+// fft_passes.h: P0 = C0 minus E, P1 = E, P2 = O, P3 = C1 minus O, each padded
+// with zeros to a multiple of 128 cells, the MFMA kernel's chunk): level L's
+// cells are the contiguous range [off[L], off[L] + ncell[L]) of the shell
+// row.  Level 0: the whole mesh P0..P3, 1: C0 = P0 P1 (x+y+z even, 1/2 of
+// the cells), 2: E and O = P1 P2 (x, y, z of equal parity, 1/4), 3: E = P1
+// (all even, 1/8).
 long long round128(long long v) { return (v + 127) / 128 * 128; }
-long long decimated_offset(int S) { return round128((long long)S * S * S); }
-long long decimated_cells(int S) { return round128((long long)(S / 2) * (S / 2) * (S / 2)); }
+struct LevelLayout {
+  long long off[BSK_LEVELS], ncell[BSK_LEVELS], stride;
+};
+LevelLayout level_layout(int S) {
+  const long long nh = S / 2, ne = round128(nh * nh * nh), nc = round128(3 * nh * nh * nh);
+  return LevelLayout{{0, 0, nc, nc}, {2 * nc + 2 * ne, nc + ne, 2 * ne, ne}, 2 * nc + 2 * ne};
+}
+
+// Whether the sum over level L's sublattice is exact (|H|/S^3 of the full sum)
+// for a product whose frequencies q satisfy |q_a| <= rinf (every component)
+// and |q| <= r2: no nonzero vector of the dual lattice is such a q.  The dual
+// vectors are (S/2)(a, b, c) with a, b, c all even (level 0), all of one
+// parity (1), of even sum (2) or any (3).
+bool level_exact(int S, int L, long long rinf, double r2) {
+  const long long h = S / 2;
+  const long long A = rinf / h;  // |a| * h <= rinf
+  for (long long a = -A; a <= A; ++a)
+    for (long long b = -A; b <= A; ++b)
+      for (long long c = -A; c <= A; ++c) {
+        if (!a && !b && !c) continue;
+        const bool ok = L == 0   ? !(a & 1) && !(b & 1) && !(c & 1)
+                        : L == 1 ? ((a ^ b) & 1) == 0 && ((b ^ c) & 1) == 0
+                        : L == 2 ? ((a + b + c) & 1) == 0
+                                 : true;
+        if (!ok) continue;
+        const double n2 = (double)(h * h) * (double)(a * a + b * b + c * c);
+        if (n2 <= r2 * r2 * (1.0 + 1e-12)) return false;  // a reachable alias
+      }
+  return true;
+}
 
 template <typename T>
 int make_twiddles(int n, cplx<T>** out) {
@@ -132,9 +164,8 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
   const size_t esz = (fmt == BSK_FMT_F64) ? 8 : (fmt == BSK_FMT_F32 ? 4 : 2);
   char* o = reinterpret_cast<char*>(out) + (size_t)shell0 * stride * esz;
   char* ol = out_lo ? reinterpret_cast<char*>(out_lo) + (size_t)shell0 * stride * esz : nullptr;
-  const long long dec_off = (stages & BSK_STAGE_DECIMATE) ? decimated_offset(N) : -1;
   return bsk_inv_z<T>(N, p->variant[5], d2, o, ol, tw, g.MZ, g.MZP, (long long)xs * N, cell0,
-                      stride, fmt, nsh, p->d_hi, shell0, dec_off, s);
+                      stride, fmt, nsh, p->d_hi, shell0, (stages & BSK_STAGE_LEVELS) ? 1 : 0, s);
 }
 
 const cplx<float>* shell_tw(const bsk_plan* p, float) { return p->tws_f ? p->tws_f : p->tw_f; }
@@ -299,18 +330,22 @@ int bsk_plan_shell_mesh(const bsk_plan* p, int* shell_mesh) {
   return BSK_OK;
 }
 
-int bsk_plan_decimated(const bsk_plan* p, long long* offset, long long* ncell) {
+int bsk_plan_levels(const bsk_plan* p, long long* offset, long long* ncell, long long* stride) {
   if (int rc = check_plan(p)) return rc;
-  if (!offset || !ncell) {
-    bsk_set_error("bsk_plan_decimated: null output");
+  if (!offset || !ncell || !stride) {
+    bsk_set_error("bsk_plan_levels: null output");
     return BSK_EINVAL;
   }
-  if (p->S % 2) {
-    bsk_set_error("bsk_plan_decimated: odd shell mesh %d has no decimated copy", p->S);
+  if (p->S % 4) {
+    bsk_set_error("bsk_plan_levels: shell mesh %d is not a multiple of 4", p->S);
     return BSK_EUNSUPPORTED;
   }
-  *offset = decimated_offset(p->S);
-  *ncell = decimated_cells(p->S);
+  const LevelLayout l = level_layout(p->S);
+  for (int L = 0; L < BSK_LEVELS; ++L) {
+    offset[L] = l.off[L];
+    ncell[L] = l.ncell[L];
+  }
+  *stride = l.stride;
   return BSK_OK;
 }
 
@@ -320,24 +355,35 @@ int bsk_plan_triangle_levels(const bsk_plan* p, int ntri, const int* tri, int* l
     bsk_set_error("bsk_plan_triangle_levels: invalid argument");
     return BSK_EINVAL;
   }
-  // largest |n| component of each shell's modes
+  // largest |n| component and |n| of each shell's modes
   std::vector<long long> mb(p->nbin);
+  std::vector<double> rb(p->nbin);
   for (int b = 0; b < p->nbin; ++b) {
     long long r = 0;
     while ((r + 1) * (r + 1) <= (long long)p->n2hi[b]) ++r;
     mb[b] = r;
+    rb[b] = p->n2hi[b] > 0 ? std::sqrt((double)p->n2hi[b]) : 0.0;
   }
   for (int t = 0; t < ntri; ++t) {
-    long long sum = 0;
+    long long rinf = 0;
+    double r2 = 0.0;
     for (int e = 0; e < 3; ++e) {
       const int b = tri[3 * t + e];
       if (b < 0 || b >= p->nbin) {
         bsk_set_error("bsk_plan_triangle_levels: triangle %d has a bin index out of range", t);
         return BSK_EINVAL;
       }
-      sum += mb[b];
+      rinf += mb[b];
+      r2 += rb[b];
     }
-    level[t] = (p->S % 2 == 0 && 2 * sum < p->S) ? 1 : 0;
+    int best = 0;
+    if (p->S % 4 == 0)
+      for (int L = BSK_LEVELS - 1; L > 0; --L)
+        if (level_exact(p->S, L, rinf, r2)) {
+          best = L;
+          break;
+        }
+    level[t] = best;
   }
   return BSK_OK;
 }
@@ -432,12 +478,12 @@ int bsk_shells(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int
   if (int rc = check_plan(p)) return rc;
   const long long slab_cells = (long long)xslab * p->S * p->S;
   const bool yz = (stages & BSK_STAGE_YZ) != 0;
-  if ((stages & BSK_STAGE_DECIMATE) &&
-      (!yz || p->S % 2 || x0 != 0 || xslab != p->S || out_cell0 != 0 ||
-       shell_stride < decimated_offset(p->S) + decimated_cells(p->S))) {
-    bsk_set_error("bsk_shells: the decimated copy needs the y and z stages over the whole "
-                  "mesh (x0 = 0, xslab = S, out_cell0 = 0), an even S and a shell stride of at "
-                  "least bsk_plan_decimated offset + cells");
+  if ((stages & BSK_STAGE_LEVELS) &&
+      (!yz || p->S % 4 || x0 != 0 || xslab != p->S || out_cell0 != 0 ||
+       shell_stride < level_layout(p->S).stride)) {
+    bsk_set_error("bsk_shells: the level layout needs the y and z stages over the whole "
+                  "mesh (x0 = 0, xslab = S, out_cell0 = 0), S a multiple of 4 and a shell "
+                  "stride of at least the bsk_plan_levels stride");
     return BSK_EINVAL;
   }
   if (nshell <= 0 || shell0 < 0 || shell0 + nshell > p->nbin || x0 < 0 || xslab <= 0 ||
@@ -580,7 +626,9 @@ struct bsk_pairs {
   int nkb = 0;
   int ni = 1;          // items per wave and launch (compiled count)
   int npass = 1;
-  int npass0 = 1;      // passes of level 0 (full mesh); the rest run on the decimated mesh
+  int lpass[BSK_LEVELS] = {1, 0, 0, 0};  // passes per sublattice level, in level order
+  // shell window (first shell, count) each level's passes stage
+  int lwin[BSK_LEVELS][2] = {{0, 1}, {0, 1}, {0, 1}, {0, 1}};
   int item_total = 0;  // npass * waves * ni
   std::vector<int> slot;
   std::vector<int> h_items, h_pairs;
@@ -751,14 +799,26 @@ struct LevelPlan {
   std::vector<unsigned> unit_mask;
   std::vector<std::vector<int>> bins;  // [npass * waves] unit lists
   int npass = 0, maxload = 0;
+  int lo = 0, nsh = 1, nkb = 1;  // shell window (shell indices of the plan are relative to lo)
 };
 
-LevelPlan plan_level(int mode, int nkb, int cap, const std::vector<int>& idx, const int* tri) {
+LevelPlan plan_level(int mode, int cap, const std::vector<int>& idx, const int* tri) {
   LevelPlan lp;
   lp.tri = idx;
+  // shell window of the level: its passes stage only shells [lo, lo + nsh)
+  int lo = INT_MAX, hi = -1;
+  for (int t : idx)
+    for (int e = 0; e < 3; ++e) {
+      lo = std::min(lo, tri[3 * (size_t)t + e]);
+      hi = std::max(hi, tri[3 * (size_t)t + e]);
+    }
+  lp.lo = idx.empty() ? 0 : lo;
+  lp.nsh = idx.empty() ? 1 : hi + 1 - lo;
+  lp.nkb = (lp.nsh + 15) / 16;
+  const int nkb = lp.nkb;
   std::vector<int> sub((size_t)3 * idx.size());
   for (size_t t = 0; t < idx.size(); ++t)
-    for (int e = 0; e < 3; ++e) sub[3 * t + e] = tri[3 * (size_t)idx[t] + e];
+    for (int e = 0; e < 3; ++e) sub[3 * t + e] = tri[3 * (size_t)idx[t] + e] - lp.lo;
   const int ntri = (int)idx.size();
   // role assignment: the fixed roles of <AAB>/<ABC>; for auto spectra the
   // choice of the pair sides that needs the fewest items
@@ -844,8 +904,8 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
         bsk_set_error("%s: triangle %d has a shell index out of range", fn, t);
         return BSK_EINVAL;
       }
-    if (level && (level[t] < 0 || level[t] > 1)) {
-      bsk_set_error("%s: triangle %d has level %d (0 or 1)", fn, t, level[t]);
+    if (level && (level[t] < 0 || level[t] >= BSK_LEVELS)) {
+      bsk_set_error("%s: triangle %d has level %d (0 to %d)", fn, t, level[t], BSK_LEVELS - 1);
       return BSK_EINVAL;
     }
   }
@@ -859,28 +919,44 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
     return BSK_EUNSUPPORTED;
   }
   const int W = kPairWaves, cap = bsk_pair_item_cap(sets, nshell);
-  std::vector<int> idx[2];
+  std::vector<int> idx[BSK_LEVELS];
   for (int t = 0; t < ntri; ++t) idx[level ? level[t] : 0].push_back(t);
-  LevelPlan lp[2] = {plan_level(mode, nkb, cap, idx[0], tri),
-                     plan_level(mode, nkb, cap, idx[1], tri)};
-  const int tile0[2] = {0, (int)lp[0].tiles.tile_mask.size()};
-  const int ntile = tile0[1] + (int)lp[1].tiles.tile_mask.size();
+  LevelPlan lp[BSK_LEVELS];
+  int q_win[BSK_LEVELS][2] = {{0, nshell}, {0, nshell}, {0, nshell}, {0, nshell}};
+  int tile0[BSK_LEVELS + 1] = {0};
+  int lpass[BSK_LEVELS], pass0[BSK_LEVELS], maxload = 1;
+  for (int L = 0; L < BSK_LEVELS; ++L) {
+    lp[L] = plan_level(mode, cap, idx[L], tri);
+    q_win[L][0] = lp[L].lo;
+    q_win[L][1] = lp[L].nsh;
+    tile0[L + 1] = tile0[L] + (int)lp[L].tiles.tile_mask.size();
+    lpass[L] = lp[L].npass;
+    maxload = std::max(maxload, lp[L].maxload);
+  }
+  if (!ntri) lpass[0] = 1;  // an empty list still gets one pass of padding items
+  int npass = 0;
+  for (int L = 0; L < BSK_LEVELS; ++L) {
+    pass0[L] = npass;
+    npass += lpass[L];
+  }
+  const int ntile = tile0[BSK_LEVELS];
   if (ntile >= 0xffff) {
     bsk_set_error("%s: %d pair tiles exceed 65534; split the triangle list", fn, ntile);
     return BSK_EUNSUPPORTED;
   }
-  // an empty list still gets one pass of padding items
-  const int npass0 = ntri ? lp[0].npass : 1;
-  const int npass = npass0 + lp[1].npass;
-  const int ni = bsk_pair_items_round(std::max({1, lp[0].maxload, lp[1].maxload}));
+  const int ni = bsk_pair_items_round(maxload);
+  if ((long long)npass * W * ni * 256 >= (1LL << 28)) {  // slot field below the level bits
+    bsk_set_error("%s: %lld items exceed the slot range; split the triangle list", fn,
+                  (long long)npass * W * ni);
+    return BSK_EUNSUPPORTED;
+  }
   // item descriptors: tile | kb << 16 | padding << 29 | (first item of a
-  // unit: new pair products) << 31; level 0 passes first
+  // unit: new pair products) << 31; the passes of each level in level order
   std::vector<int> items((size_t)npass * W * ni);
   std::vector<int> item_of((size_t)ntile * nkb, -1);  // global item of (tile, kb)
-  for (int L = 0; L < 2; ++L) {
-    const int pass0 = L ? npass0 : 0;
+  for (int L = 0; L < BSK_LEVELS; ++L) {
     for (size_t b = 0; b < lp[L].bins.size(); ++b) {
-      const int g0 = (pass0 * W + (int)b) * ni;
+      const int g0 = (pass0[L] * W + (int)b) * ni;
       int s = 0;
       for (int u : lp[L].bins[b]) {
         const int ti = tile0[L] + lp[L].unit_tile[u];
@@ -906,9 +982,13 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
   bsk_pairs* q = new bsk_pairs();
   q->slot.resize(ntri);
   int npair = 0, nitem = 0;
-  for (int L = 0; L < 2; ++L) {
+  for (int L = 0; L < BSK_LEVELS; ++L) {
     const PairTiles& tl = lp[L].tiles;
     const int nt = (int)tl.tile_mask.size();
+    const int zr = lp[L].nsh | (lp[L].nsh << 16);  // the window's zero rows
+    for (int ti = 0; ti < nt; ++ti)
+      std::fill(prs.begin() + (size_t)(tile0[L] + ti) * 16,
+                prs.begin() + (size_t)(tile0[L] + ti + 1) * 16, zr);
     std::vector<std::pair<int, int>> where(tl.pairs.size());  // (global tile, row)
     for (int ti = 0; ti < nt; ++ti) {
       const int p0 = tl.tile_of_pair_start[ti];
@@ -920,7 +1000,7 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
         where[p] = {tile0[L] + ti, p - p0};
       }
     }
-    // triangle slots: item * 256 + row * 16 + column, level in bit 30
+    // triangle slots: item * 256 + row * 16 + column, level in bits 28-29
     std::vector<long long> keys(tl.pairs.size());
     for (size_t p = 0; p < keys.size(); ++p) keys[p] = tl.pairs[p].first;
     std::vector<size_t> kidx(keys.size());
@@ -934,7 +1014,7 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
       const int p = (int)*it;
       const int ti = where[p].first, row = where[p].second, c = r.c;
       const int g = item_of[(size_t)ti * nkb + c / 16];
-      q->slot[lp[L].tri[t]] = g * 256 + row * 16 + (c % 16) + (L << 30);
+      q->slot[lp[L].tri[t]] = g * 256 + row * 16 + (c % 16) + (L << 28);
     }
     npair += (int)tl.pairs.size();
     nitem += tl.nitem;
@@ -949,7 +1029,11 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
   q->nkb = nkb;
   q->ni = ni;
   q->npass = npass;
-  q->npass0 = npass0;
+  std::copy(lpass, lpass + BSK_LEVELS, q->lpass);
+  for (int L = 0; L < BSK_LEVELS; ++L) {
+    q->lwin[L][0] = ntri ? q_win[L][0] : 0;
+    q->lwin[L][1] = ntri ? q_win[L][1] : nshell;
+  }
   q->item_total = npass * W * ni;
   q->h_items.swap(items);
   q->h_pairs.swap(prs);
@@ -974,13 +1058,24 @@ int bsk_pairs_create_levels(bsk_pairs** out, int mode, int nshell, int ntri, con
   return pairs_create(out, mode, nshell, ntri, tri, level, "bsk_pairs_create_levels");
 }
 
-int bsk_pairs_level_passes(const bsk_pairs* q, int* npass0, int* npass1) {
-  if (!q) {
-    bsk_set_error("bsk_pairs_level_passes: null plan");
+int bsk_pairs_level_windows(const bsk_pairs* q, int* first, int* count) {
+  if (!q || !first || !count) {
+    bsk_set_error("bsk_pairs_level_windows: invalid argument");
     return BSK_EINVAL;
   }
-  if (npass0) *npass0 = q->npass0;
-  if (npass1) *npass1 = q->npass - q->npass0;
+  for (int L = 0; L < BSK_LEVELS; ++L) {
+    first[L] = q->lwin[L][0];
+    count[L] = q->lwin[L][1];
+  }
+  return BSK_OK;
+}
+
+int bsk_pairs_level_passes(const bsk_pairs* q, int* npass) {
+  if (!q || !npass) {
+    bsk_set_error("bsk_pairs_level_passes: invalid argument");
+    return BSK_EINVAL;
+  }
+  std::copy(q->lpass, q->lpass + BSK_LEVELS, npass);
   return BSK_OK;
 }
 
@@ -1057,20 +1152,33 @@ namespace {
 int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
                  const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
                  const uint16_t* lo_k, long long shell_stride, long long ncell,
-                 long long dec_offset, long long dec_ncell, int nwg, double scale,
-                 double* out_dev, void* work, void* stream, const char* fn) {
+                 const long long* level_off, const long long* level_ncell, int nwg,
+                 double scale, double* out_dev, void* work, void* stream, const char* fn) {
   if (!q || !hi_j || !lo_j || !work || nwg <= 0 || shell_stride < ncell ||
       (q->mode >= 1 && (!hi_k || !lo_k)) || (q->mode == 2 && (!hi_i || !lo_i))) {
     bsk_set_error("%s: invalid argument", fn);
     return BSK_EINVAL;
   }
-  const bool lv1 = q->npass > q->npass0;
-  if (lv1 && (dec_ncell <= 0 || dec_ncell % kPairChunk || dec_offset < 0 ||
-              shell_stride < dec_offset + dec_ncell)) {
-    bsk_set_error("%s: the plan has decimated-mesh triangles; pass the decimated copy's "
-                  "offset and cell count (bsk_plan_decimated)", fn);
-    return BSK_EINVAL;
+  long long off[BSK_LEVELS] = {0}, cells[BSK_LEVELS] = {ncell};
+  for (int L = 1; L < BSK_LEVELS; ++L) {
+    if (!q->lpass[L]) continue;
+    if (!level_off || !level_ncell) {
+      bsk_set_error("%s: the plan has sublattice-level triangles; use bsk_triples_mfma_levels "
+                    "with the bsk_plan_levels layout", fn);
+      return BSK_EINVAL;
+    }
   }
+  if (level_off && level_ncell)
+    for (int L = 0; L < BSK_LEVELS; ++L) {
+      off[L] = level_off[L];
+      cells[L] = level_ncell[L];
+      if (q->lpass[L] && (cells[L] <= 0 || cells[L] % kPairChunk || off[L] < 0 ||
+                          shell_stride < off[L] + cells[L])) {
+        bsk_set_error("%s: level %d range [%lld, +%lld) does not fit the shell stride %lld "
+                      "in 128-cell chunks", fn, L, off[L], cells[L], shell_stride);
+        return BSK_EINVAL;
+      }
+    }
   if (q->ntri == 0) return BSK_OK;
   if (!q->d_items) {
     bsk_set_error("%s: pair plan not uploaded (bsk_pairs_upload)", fn);
@@ -1081,26 +1189,28 @@ int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
   const uint16_t* hi[3] = {q->mode == 2 ? hi_i : hi_j, hi_j, q->mode >= 1 ? hi_k : hi_j};
   const uint16_t* lo[3] = {q->mode == 2 ? lo_i : lo_j, lo_j, q->mode >= 1 ? lo_k : lo_j};
   a.stride = shell_stride;
-  for (int r = 0; r < 3; ++r) a.nsh[r] = q->nshell;
-  a.nkb = q->nkb;
   a.nwg = nwg;
   a.pairs = q->d_pairs;
   a.partial = reinterpret_cast<float*>(work);
   a.item_total = q->item_total;
   hipStream_t s = as_stream(stream);
-  for (int p = 0; p < q->npass; ++p) {
-    // level 0: the full mesh; level 1: each shell's decimated copy
-    const long long off = p < q->npass0 ? 0 : dec_offset;
-    for (int r = 0; r < 3; ++r) {
-      a.hi[r] = hi[r] + off;
-      a.lo[r] = lo[r] + off;
+  for (int L = 0, p = 0; L < BSK_LEVELS; ++L)
+    for (int k = 0; k < q->lpass[L]; ++k, ++p) {
+      // the cells of level L (one contiguous range of every shell row) of
+      // the shells of its window
+      const long long base = off[L] + (long long)q->lwin[L][0] * shell_stride;
+      for (int r = 0; r < 3; ++r) {
+        a.hi[r] = hi[r] + base;
+        a.lo[r] = lo[r] + base;
+        a.nsh[r] = q->lwin[L][1];
+      }
+      a.nkb = (q->lwin[L][1] + 15) / 16;
+      a.ncell = cells[L];
+      a.item0 = p * kPairWaves * q->ni;
+      a.items = q->d_items + a.item0;
+      int rc = bsk_triple_pairs_launch(a, q->sets, q->ni, s);
+      if (rc) return rc;
     }
-    a.ncell = p < q->npass0 ? ncell : dec_ncell;
-    a.item0 = p * kPairWaves * q->ni;
-    a.items = q->d_items + a.item0;
-    int rc = bsk_triple_pairs_launch(a, q->sets, q->ni, s);
-    if (rc) return rc;
-  }
   if (!out_dev) return BSK_OK;  // partials only; bsk_triples_mfma_reduce finishes
   return bsk_reduce_mfma_launch(a.partial, nwg, q->item_total, q->ntri, q->d_slot, scale,
                                 out_dev, s);
@@ -1111,17 +1221,22 @@ int bsk_triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* l
                      const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
                      const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
                      double scale, double* out_dev, void* work, void* stream) {
-  return triples_mfma(q, hi_i, lo_i, hi_j, lo_j, hi_k, lo_k, shell_stride, ncell, -1, 0, nwg,
-                      scale, out_dev, work, stream, "bsk_triples_mfma");
+  return triples_mfma(q, hi_i, lo_i, hi_j, lo_j, hi_k, lo_k, shell_stride, ncell, nullptr,
+                      nullptr, nwg, scale, out_dev, work, stream, "bsk_triples_mfma");
 }
 
 int bsk_triples_mfma_levels(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
                             const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
-                            const uint16_t* lo_k, long long shell_stride, long long ncell,
-                            long long dec_offset, long long dec_ncell, int nwg, double scale,
-                            double* out_dev, void* work, void* stream) {
-  return triples_mfma(q, hi_i, lo_i, hi_j, lo_j, hi_k, lo_k, shell_stride, ncell, dec_offset,
-                      dec_ncell, nwg, scale, out_dev, work, stream, "bsk_triples_mfma_levels");
+                            const uint16_t* lo_k, long long shell_stride,
+                            const long long* level_off, const long long* level_ncell, int nwg,
+                            double scale, double* out_dev, void* work, void* stream) {
+  if (!level_off || !level_ncell) {
+    bsk_set_error("bsk_triples_mfma_levels: null level layout");
+    return BSK_EINVAL;
+  }
+  return triples_mfma(q, hi_i, lo_i, hi_j, lo_j, hi_k, lo_k, shell_stride, level_ncell[0],
+                      level_off, level_ncell, nwg, scale, out_dev, work, stream,
+                      "bsk_triples_mfma_levels");
 }
 
 namespace {
@@ -1129,15 +1244,17 @@ namespace {
 const char* zfused_gap(const bsk_plan* p, const bsk_pairs* q) {
   if (!p || !q) return "null plan";
   if (q->mode != BSK_MODE_AUTO) return "auto spectra only";
-  if (q->npass > q->npass0) return "the plan has decimated-mesh triangles";
-  if (q->nshell > kZMaxShells) return "too many shells";
+  for (int L = 1; L < BSK_LEVELS; ++L)
+    if (q->lpass[L]) return "the plan has sublattice-level triangles";
+  const int nsh = q->lwin[0][1];  // shells staged: the plan's window
+  if (nsh > kZMaxShells) return "too many shells";
   if (q->ni < 1 || q->ni > 4) return "items per wave above 4";
   bool size = false;
 #define C(n) size |= p->S == n;
   BSK_ZFUSED_SIZES(C)
 #undef C
   if (!size) return "no kernel for this shell mesh";
-  if (bsk_zrows_lds_bytes(p->S, q->nshell, q->nkb) > 160 * 1024) return "LDS";
+  if (bsk_zrows_lds_bytes(p->S, nsh, (nsh + 15) / 16) > 160 * 1024) return "LDS";
   return nullptr;
 }
 }  // namespace
@@ -1170,15 +1287,18 @@ int bsk_triples_zfused(bsk_plan* p, const bsk_pairs* q, const void* shell_work, 
   const Geom& g = p->g;
   const int S = p->S;
   ZArgs a;
+  // the plan's shell window [lo, lo + nsh) of the shells of the workspace
+  const int lo = q->lwin[0][0], nsh = q->lwin[0][1];
   a.d2 = reinterpret_cast<const cplx<float>*>(shell_work) +
-         (size_t)q->nshell * g.MY * S * g.MZP;  // after D1 (shells_t layout)
+         (size_t)q->nshell * g.MY * S * g.MZP +  // after D1 (shells_t layout)
+         (size_t)lo * S * S * g.MZP;
   a.nrows = (long long)S * S;
   a.MZ = g.MZ;
   a.MZP = g.MZP;
   a.n2hi = p->d_hi;
-  a.shell0 = shell0;
-  a.nsh = q->nshell;
-  a.nkb = q->nkb;
+  a.shell0 = shell0 + lo;
+  a.nsh = nsh;
+  a.nkb = (nsh + 15) / 16;
   a.nwg = nwg;
   a.pairs = q->d_pairs;
   a.partial = reinterpret_cast<float*>(work);

@@ -448,28 +448,45 @@ constexpr int pow2_at_least(int v) {
   return q;
 }
 
+// Sublattice level layout of one shell of the S^3 mesh (S = 2 NH): the
+// cells in four parts, each zero-padded to a multiple of 128 cells,
+//   P0 = C0 minus E,  P1 = E,  P2 = O,  P3 = C1 minus O,
+// with C0 / C1 the cells of even / odd x+y+z and E / O the cells with x, y,
+// z all even / all odd (E lies in C0, O in C1).  A cell (x, y, z) of a part
+// sits at ((c NH + x/2) NH + y/2) NH + z/2, c its row class: 0 in E and O,
+// (x%2) 2 + y%2 - 1 in P0 (rows whose x, y are not both even), (x%2) 2 + y%2
+// in P3 (rows whose x, y are not both odd).  The sum over a sublattice H of
+// the mesh is |H|/S^3 of the full sum whenever no nonzero vector of its dual
+// lattice is a frequency of the product, and every level is one contiguous
+// range: the whole mesh P0..P3 (index 1), C0 = P0 P1 (index 2, dual vectors
+// (S/2)(+-1,+-1,+-1) + S Z^3), E and O = P1 P2 (index 4, dual (S/2)(+-1,+-1,0)
+// and permutations + S Z^3) and E = P1 (index 8, dual (S/2) Z^3).
+template <int NH>
+struct ShellLevels {
+  static constexpr long long E = (long long)NH * NH * NH, NE = (E + 127) / 128 * 128;
+  static constexpr long long C = 3 * E, NC = (C + 127) / 128 * 128;  // P0, P3
+  static constexpr long long OFF[4] = {0, NC, NC + NE, NC + 2 * NE};
+  static constexpr long long CNT[4] = {C, E, E, C};
+};
+
 // Lanes map to (row, k) with a power-of-two lane count per row (KQ for the
 // KH mirrored pairs, QQ for the NH/2 output quads), so no index divides.
 //
-// dec_off >= 0 (whole mesh: nrows = N^2, cell0 = 0): each shell also gets
-// its decimated copy, the cells with even x, y and z, as an NH^3 mesh in C
-// order at element dec_off of the shell, followed by zeros up to a multiple
-// of 128 cells.  The decimated copy is the same field sampled on the NH^3
-// mesh (every retained frequency is an integer vector), so sums over it of
-// products whose frequencies cannot alias to zero on NH^3 equal the full
-// sums / 8.
+// lv != 0 (whole mesh: nrows = N^2, cell0 = 0): each shell is written in the
+// sublattice level layout instead of C order (ShellLevels: a permutation of
+// the cells into four parts, zero-padded).
 template <typename T, int NH, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, void* out,
                                              void* out_lo, const cplx<T>* __restrict__ twg,
                                              int MZ, int MZP, long long nrows, long long cell0,
                                              long long shell_stride, int fmt,
-                                             const int* __restrict__ n2hi, int shell0,
-                                             long long dec_off) {
+                                             const int* __restrict__ n2hi, int shell0, int lv) {
   constexpr int N = 2 * NH;
   constexpr int LD = pencil_ld<NH>();
   constexpr int KH = NH / 2 + 1;  // k = 0..NH/2 (each with its mirror NH - k)
   constexpr int KQ = pow2_at_least(KH), RK = W / KQ, NPK = P / RK;
   constexpr int QQ = pow2_at_least(NH / 2), RQ = W / QQ, NPQ = P / RQ;
+  using Lay = ShellLevels<NH>;
   static_assert(NH % 2 == 0, "z c2r needs N divisible by 4");
   static_assert(KQ <= W && P % RK == 0 && P % RQ == 0, "row mapping of the z c2r");
   __shared__ cplx<T> buf[P * LD];
@@ -482,13 +499,16 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   const int mzb = shell_mz(MZ, n2hi[shell0 + bl]);  // kz from mzb on are zero
   const cplx<T>* src = in + (bl * nrows + row0) * MZP;
   const int esz = out_esize<T>(fmt);
-  // uniform bases: this shell's decimated copy, this workgroup's rows
-  char* const dhi = static_cast<char*>(out) + (bl * shell_stride + dec_off) * esz;
-  char* const dlo = out_lo ? static_cast<char*>(out_lo) + (bl * shell_stride + dec_off) * esz
-                           : nullptr;
-  constexpr long long DC = (long long)NH * NH * NH, DCP = (DC + 127) / 128 * 128;
-  if (dec_off >= 0 && blockIdx.x == 0)
-    for (int e = tid; e < (int)(DCP - DC); e += W) store_zero<T>(fmt, dhi, dlo, (unsigned)(DC + e));
+  // uniform base of the level layout: this shell's row
+  char* const shi = static_cast<char*>(out) + bl * shell_stride * esz;
+  char* const slo = out_lo ? static_cast<char*>(out_lo) + bl * shell_stride * esz : nullptr;
+  if (lv && blockIdx.x == 0)  // zero padding of the four parts
+#pragma unroll
+    for (int part = 0; part < 4; ++part) {
+      const long long end = part + 1 < 4 ? Lay::OFF[part + 1] : Lay::OFF[3] + Lay::NC;
+      for (long long e = Lay::OFF[part] + Lay::CNT[part] + tid; e < end; e += W)
+        store_zero<T>(fmt, shi, slo, (unsigned)e);
+    }
   const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
   // (k, NH - k) pairs: lane k of rows r0, r0 + RK, ...; up to 8 rows (16
   // loads) in flight per batch
@@ -525,12 +545,25 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
       const int p = rq + i * RQ;
       if (p >= nr) continue;
       const cplx<T>* b = buf + p * LD + 2 * q;
-      store_quad<T>(fmt, ohi, olo, (unsigned)(p * N + 4 * q), b[0], b[1], sc);
-      if (dec_off >= 0) {  // z = 4q and 4q + 2 of rows with even x and y
-        const int row = (int)row0 + p, x = row / N, y = row - x * N;
-        if (((x | y) & 1) == 0)
-          store_pair<T>(fmt, dhi, dlo, (unsigned)(((x >> 1) * NH + (y >> 1)) * NH + 2 * q),
-                        b[0].x, b[1].x, sc);
+      if (!lv) {
+        store_quad<T>(fmt, ohi, olo, (unsigned)(p * N + 4 * q), b[0], b[1], sc);
+        continue;
+      }
+      // z = 4q, 4q + 2 (b[0].x, b[1].x) and z = 4q + 1, 4q + 3 (b[0].y, b[1].y):
+      // the same z/2 = 2q, 2q + 1 in the part of each z parity
+      const int row = (int)row0 + p, x = row / N, y = row - x * N;
+      const int xy = (x & 1) * 2 + (y & 1);  // 0: both even, 3: both odd
+      const unsigned pos = (unsigned)(((x >> 1) * NH + (y >> 1)) * NH + 2 * q);
+#pragma unroll
+      for (int zp = 0; zp < 2; ++zp) {
+        unsigned off;
+        if (((x + y + zp) & 1) == 0)  // C0: E if all even, else P0
+          off = (xy == 0 && zp == 0) ? (unsigned)Lay::OFF[1]
+                                     : (unsigned)(Lay::OFF[0] + (long long)(xy - 1) * Lay::E);
+        else  // C1: O if all odd, else P3
+          off = (xy == 3 && zp == 1) ? (unsigned)Lay::OFF[2]
+                                     : (unsigned)(Lay::OFF[3] + (long long)xy * Lay::E);
+        store_pair<T>(fmt, shi, slo, off + pos, zp ? b[0].y : b[0].x, zp ? b[1].y : b[1].x, sc);
       }
     }
   }

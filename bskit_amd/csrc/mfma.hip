@@ -193,10 +193,9 @@ __global__ __launch_bounds__(64) void k_reduce_mfma(const float* __restrict__ pa
                                                     double scale, double* __restrict__ out) {
   const int t = blockIdx.x;
   const int v = slot[t];
-  const long long e = v & 0x3fffffff;
-  // bit 30: summed over the decimated mesh (every second cell per axis), one
-  // eighth of the cells of the full-mesh sum
-  const double f = (v >> 30) & 1 ? 8.0 : 1.0;
+  const long long e = v & 0x0fffffff;
+  // bits 28-29: sublattice level L, whose cells are 2^-L of the mesh
+  const double f = (double)(1 << ((v >> 28) & 3));
   double s = 0.0;
   for (int w = threadIdx.x; w < nwg; w += 64) s += (double)partial[w * wg_stride + e];
 #pragma unroll

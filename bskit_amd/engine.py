@@ -79,43 +79,85 @@ class Workspace:
         self._bufs.clear()
 
 
+class LevelLayout:
+    """Sublattice level layout of a shell row on an S^3 mesh (S % 4 == 0):
+    level L's cells are data[off[L]:off[L] + ncell[L]] (include/bskhip.h,
+    bsk_plan_levels); the parts C0 minus E, E, O, C1 minus O (C0 / C1: even /
+    odd x+y+z, E / O: x, y, z all even / all odd), each zero-padded to a
+    multiple of 128 cells."""
+
+    def __init__(self, S, off, ncell, stride):
+        self.S = S
+        self.off = tuple(int(v) for v in off)
+        self.ncell = tuple(int(v) for v in ncell)
+        self.stride = int(stride)
+        self._perm = None
+
+    def __eq__(self, other):
+        return isinstance(other, LevelLayout) and (self.S, self.off, self.ncell, self.stride) == (
+            other.S, other.off, other.ncell, other.stride)
+
+    def c_order_index(self, device):
+        """Row position of every cell in C order (x, y, z), for inspection."""
+        if self._perm is None or self._perm.device != torch.device(device):
+            S, nh = self.S, self.S // 2
+            nc, ne, e = self.off[2], self.ncell[3], nh ** 3  # P0 / P3 span, E / O span
+            i = torch.arange(S, device=device)
+            x, y, z = i.view(S, 1, 1), i.view(1, S, 1), i.view(1, 1, S)
+            xy = 2 * (x % 2) + y % 2  # row class: 0 both even .. 3 both odd
+            pos = ((x // 2) * nh + y // 2) * nh + z // 2
+            c0 = (x + y + z) % 2 == 0
+            part_off = torch.where(
+                c0, torch.where((xy == 0) & (z % 2 == 0), nc, (xy - 1) * e),
+                torch.where((xy == 3) & (z % 2 == 1), nc + ne, nc + 2 * ne + xy * e))
+            self._perm = (part_off + pos).reshape(-1)
+        return self._perm
+
+
 class ShellSet:
     """nb shell fields of one input field in one storage format: rows of
-    ``stride`` elements, the S^3 cells first, then (``dec`` = (offset,
-    ncell)) the decimated copy of each shell."""
+    ``stride`` elements holding the S^3 cells in C order, or (``layout``) in
+    the sublattice level layout."""
 
-    def __init__(self, fmt, data, lo=None, ncell=None, dec=None):
+    def __init__(self, fmt, data, lo=None, ncell=None, layout=None):
         self.fmt = fmt
         self.data = data      # F32/F64: [nb, stride] float; BF16X2: hi plane int16
         self.lo = lo          # BF16X2 lo plane
         self.nb = data.shape[0]
         self.stride = data.shape[1]
         self.ncell = self.stride if ncell is None else int(ncell)
-        self.dec = dec
+        self.layout = layout
+
+    @property
+    def sum_cells(self):
+        """Leading elements of a row that hold every cell (and zeros)."""
+        return self.ncell if self.layout is None else self.layout.ncell[0]
 
     @staticmethod
     def _f(v):
         return (v.to(torch.int32) << 16).view(torch.float32)
 
-    def as_float(self):
-        """FP32/FP64 view of the S^3 cells (hi + lo for BF16X2), for inspection."""
+    def _values(self, a, b):
         if self.fmt != L.FMT_BF16X2:
-            return self.data[:, :self.ncell]
-        return self._f(self.data[:, :self.ncell]) + self._f(self.lo[:, :self.ncell])
+            return self.data[:, a:b]
+        return self._f(self.data[:, a:b]) + self._f(self.lo[:, a:b])
 
-    def decimated_float(self):
-        """The decimated copies [nb, (S/2)^3 padded] as FP32 (hi + lo), or None."""
-        if self.dec is None:
-            return None
-        o, n = self.dec
-        if self.fmt != L.FMT_BF16X2:
-            return self.data[:, o:o + n]
-        return self._f(self.data[:, o:o + n]) + self._f(self.lo[:, o:o + n])
+    def as_float(self):
+        """FP32/FP64 values (hi + lo for BF16X2) of the S^3 cells in C order,
+        for inspection."""
+        if self.layout is None:
+            return self._values(0, self.ncell)
+        return self._values(0, self.layout.ncell[0])[:, self.layout.c_order_index(self.data.device)]
+
+    def level_values(self, level):
+        """Row range of a sublattice level as FP32/FP64 values."""
+        o, n = self.layout.off[level], self.layout.ncell[level]
+        return self._values(o, o + n)
 
 
 class PairPlan:
     """Owner of a bsk_pairs plan: the MFMA kernel's tiles for one triangle list
-    (with ``levels``: 1 marks triangles summed over the decimated mesh)."""
+    (with ``levels``: the sublattice level each triangle is summed over)."""
 
     def __init__(self, tri, nshell, mode, upload=True, levels=None):
         tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
@@ -129,9 +171,12 @@ class PairPlan:
             L.call('bsk_pairs_create_levels', ctypes.byref(h), int(mode), int(nshell), len(tri),
                    tri.ctypes.data_as(ip), levels.ctypes.data_as(ip))
         self.handle = h
-        p0, p1 = ctypes.c_int(), ctypes.c_int()
-        L.call('bsk_pairs_level_passes', h, ctypes.byref(p0), ctypes.byref(p1))
-        self.npass_full, self.npass_dec = p0.value, p1.value
+        lp = (ctypes.c_int * L.LEVELS)()
+        L.call('bsk_pairs_level_passes', h, lp)
+        self.level_passes = list(lp)  # launches per sublattice level
+        w0, wn = (ctypes.c_int * L.LEVELS)(), (ctypes.c_int * L.LEVELS)()
+        L.call('bsk_pairs_level_windows', h, w0, wn)
+        self.level_windows = list(zip(w0, wn))  # (first shell, count) staged per level
         if upload:
             L.call('bsk_pairs_upload', h)
         self.ntri = len(tri)
@@ -202,8 +247,8 @@ class GridEngine:
         self.tuned_nwg = None
         self.tuned_zfused_nwg = None
         self.prefer_zfused = False  # set by the autotuner when the fused kernel is faster
-        # BF16X2 shell sets carry decimated copies and the MFMA sums run the
-        # triangles that are exact there over an eighth of the cells
+        # BF16X2 shell sets use the sublattice level layout and the MFMA sums
+        # run each triangle over the sparsest sublattice exact for it
         self.levels = True
         self.cic_window = 0
         if cic_window:
@@ -228,11 +273,12 @@ class GridEngine:
         L.call('bsk_plan_shell_mesh', self._plan, ctypes.byref(s))
         self.S = s.value
         self.ncell = self.S ** 3
-        self.dec = None  # (offset, cells) of a shell's decimated copy
-        if self.S % 2 == 0:
-            o, n = ctypes.c_longlong(), ctypes.c_longlong()
-            L.call('bsk_plan_decimated', self._plan, ctypes.byref(o), ctypes.byref(n))
-            self.dec = (o.value, n.value)
+        self.layout = None  # sublattice level layout of a shell row
+        if self.S % 4 == 0:
+            o, n = (ctypes.c_longlong * L.LEVELS)(), (ctypes.c_longlong * L.LEVELS)()
+            st = ctypes.c_longlong()
+            L.call('bsk_plan_levels', self._plan, o, n, ctypes.byref(st))
+            self.layout = LevelLayout(self.S, o, n, st.value)
 
     def set_shell_mesh(self, shell_mesh=0):
         """Shell mesh size (0 = automatic, Nmesh = the reference's own mesh)."""
@@ -293,30 +339,30 @@ class GridEngine:
         nbytes = L.size_out()
         L.call('bsk_shells_workspace', self._plan, dt, nshell, self.S, ctypes.byref(nbytes))
         w = (work or self.work).get('shell', nbytes.value)
-        stages = L.STAGE_X | L.STAGE_YZ | (L.STAGE_DECIMATE if out.dec else 0)
+        stages = L.STAGE_X | L.STAGE_YZ | (L.STAGE_LEVELS if out.layout else 0)
         L.call('bsk_shells', self._plan, dt, kind, L.ptr(dk), shell0, nshell, 0, self.S,
                stages, fmt, L.ptr(out.data), L.ptr(out.lo), out.stride, 0,
                L.ptr(w), stream_handle(stream))
         return out
 
-    def alloc_shells(self, fmt, nb, decimated=None):
-        """Shell storage for nb shells; BF16X2 sets carry the decimated copies
-        (for the two-level MFMA sums) unless decimated=False or S is odd."""
-        if decimated is None:
-            decimated = self.uses_levels(fmt)
-        dec = self.dec if decimated else None
-        stride = self.ncell if dec is None else dec[0] + dec[1]
+    def alloc_shells(self, fmt, nb, levels=None):
+        """Shell storage for nb shells: C order, or the sublattice level
+        layout (default for BF16X2, the MFMA path; not for S % 4 != 0)."""
+        if levels is None:
+            levels = self.uses_levels(fmt)
+        lay = self.layout if levels else None
+        stride = self.ncell if lay is None else lay.stride
         if fmt == L.FMT_BF16X2:
             hi = torch.empty((nb, stride), dtype=torch.int16, device=self.device)
             lo = torch.empty((nb, stride), dtype=torch.int16, device=self.device)
-            return ShellSet(fmt, hi, lo, self.ncell, dec)
+            return ShellSet(fmt, hi, lo, self.ncell, lay)
         dtype = torch.float32 if fmt == L.FMT_F32 else torch.float64
         return ShellSet(fmt, torch.empty((nb, stride), dtype=dtype, device=self.device),
-                        None, self.ncell, dec)
+                        None, self.ncell, lay)
 
     def triangle_levels(self, tri):
-        """1 for the triangles (bin indices) whose sums are exact on the
-        decimated mesh, else 0."""
+        """Sublattice level (0..3) each triangle (bin indices) is summed over:
+        the sparsest one on which its sum is exact."""
         tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
         out = np.zeros(len(tri), dtype=np.int32)
         ip = ctypes.POINTER(ctypes.c_int)
@@ -336,7 +382,7 @@ class GridEngine:
 
     def pair_plan(self, tri, nshell, mode, levels=False):
         """Cached MFMA pair plan of a triangle list (host tables uploaded once);
-        levels=True splits it between the full and the decimated mesh."""
+        levels=True sums each triangle over its sublattice level."""
         tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
         key = (tri.tobytes(), 'pairs', nshell, mode, bool(levels))
         r = self._tri_cache.get(key)
@@ -453,15 +499,21 @@ class GridEngine:
                     raise ValueError('the MFMA path needs BF16X2 shells')
             mode = _MODE[nf]
             nwg = self.mfma_nwg(ncell) if nwg is None else int(nwg)
-            dec = sa.dec if all(x.dec == sa.dec for x in sets) else None
-            plan = self.pair_plan(tri, sa.nb, mode, levels=dec is not None)
-            dec = dec or (-1, 0)
+            lay = sa.layout
+            if any(x.layout != lay for x in sets):
+                raise ValueError('shell sets of one call need the same layout')
+            plan = self.pair_plan(tri, sa.nb, mode, levels=lay is not None)
+            if lay is None:
+                off = (ctypes.c_longlong * L.LEVELS)(0, 0, 0, 0)
+                cells = (ctypes.c_longlong * L.LEVELS)(ncell, 0, 0, 0)
+            else:
+                off = (ctypes.c_longlong * L.LEVELS)(*lay.off)
+                cells = (ctypes.c_longlong * L.LEVELS)(*lay.ncell)
             nbytes = L.size_out()
             L.call('bsk_triples_mfma_workspace', plan.handle, nwg, ctypes.byref(nbytes))
             w = (work or self.work).get('mfma', nbytes.value)
             args = (plan.handle, L.ptr(sa.data), L.ptr(sa.lo), L.ptr(sb.data), L.ptr(sb.lo),
-                    L.ptr(sc.data), L.ptr(sc.lo), sa.stride, ncell, dec[0], dec[1], nwg,
-                    float(scale))
+                    L.ptr(sc.data), L.ptr(sc.lo), sa.stride, off, cells, nwg, float(scale))
             if kernel_events is None:
                 L.call('bsk_triples_mfma_levels', *args, L.ptr(out), L.ptr(w), s)
                 return out
@@ -484,7 +536,7 @@ class GridEngine:
             for t0 in range(0, T, 65535):
                 t1 = min(T, t0 + 65535)
                 L.call('bsk_triples_plain', dt, L.ptr(sa.data), L.ptr(sb.data), L.ptr(sc.data),
-                       sa.stride, ncell, t1 - t0, L.ptr(tri_d[t0:t1]), nchunk, float(scale),
+                       sa.stride, sa.sum_cells, t1 - t0, L.ptr(tri_d[t0:t1]), nchunk, float(scale),
                        L.ptr(out[t0:t1]), L.ptr(w), s)
             return out
         if path == PATH_TILED:
@@ -497,7 +549,8 @@ class GridEngine:
             nbytes = L.size_out()
             L.call('bsk_triples_tiled_workspace', T, 1, nchunk, ctypes.byref(nbytes))
             w = (work or self.work).get('tiled', nbytes.value)
-            L.call('bsk_triples_tiled', dt, L.ptr(sa.data), sa.stride, sa.nb, ncell, T, L.ptr(tri_d), 1,
+            L.call('bsk_triples_tiled', dt, L.ptr(sa.data), sa.stride, sa.nb, sa.sum_cells, T,
+                   L.ptr(tri_d), 1,
                    nchunk, float(scale), L.ptr(out), L.ptr(w), s)
             return out
         raise ValueError('unknown triple path %r' % (path,))
@@ -576,15 +629,15 @@ class GridEngine:
 
     # ---------------------------------------------------------------- sizing
     def uses_levels(self, fmt=L.FMT_BF16X2):
-        """Whether default shell sets of this format carry decimated copies
-        (and MFMA sums over them run the two-level plan)."""
-        return fmt == L.FMT_BF16X2 and self.levels and self.dec is not None
+        """Whether default shell sets of this format use the sublattice level
+        layout (and MFMA sums over them the level plan)."""
+        return fmt == L.FMT_BF16X2 and self.levels and self.layout is not None
 
     def shell_bytes(self, fmt, nb=None):
         """Device bytes of a default shell set (alloc_shells) of nb shells."""
         nb = self.nb if nb is None else nb
         per = 8 if fmt == L.FMT_F64 else 4
-        stride = self.dec[0] + self.dec[1] if self.uses_levels(fmt) else self.ncell
+        stride = self.layout.stride if self.uses_levels(fmt) else self.ncell
         return nb * stride * per
 
 

@@ -62,7 +62,7 @@ def _save():
 
 def key_of(eng, ntri=0):
     dev = torch.cuda.get_device_name(eng.device)
-    lv = int(bool(eng.levels and eng.dec is not None))  # two-level MFMA sums
+    lv = int(bool(eng.uses_levels()))  # sublattice-level MFMA sums
     return '%s|N%d|S%d|m%d|nb%d|T%d|L%d' % (dev, eng.N, eng.S, eng.m, eng.nb, ntri, lv)
 
 

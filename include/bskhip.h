@@ -93,19 +93,31 @@ int bsk_plan_thresholds(const bsk_plan* plan, int* n2lo, int* n2hi);
  * mesh. */
 int bsk_plan_shell_mesh(const bsk_plan* plan, int* shell_mesh);
 int bsk_plan_set_shell_mesh(bsk_plan* plan, int shell_mesh);
-/* Decimated shell mesh (even S): the cells of the shell mesh with even x,
- * y and z form an (S/2)^3 mesh on which each shell field is the same
- * band-limited sum of modes.  A triangle (i, j, k) whose shells' largest
- * |n| components m_i + m_j + m_k stay below S/2 has no product frequency
- * that aliases to zero there either, so its cell sum is exactly 8 times the
- * sum over the decimated mesh; such triangles cost an eighth of the cells.
- * bsk_plan_decimated: a shell's decimated copy sits at element *offset of
- * the shell (S^3 rounded up to 128) and holds *ncell cells ((S/2)^3 in C
- * order, zero-padded to a multiple of 128).  bsk_plan_triangle_levels:
- * level[t] = 1 for triangles (bin indices) exact on the decimated mesh,
- * else 0.  No reference counterpart: the reference sums every triangle
- * over the full N^3 mesh (bskit/main.py:1871-1882). */
-int bsk_plan_decimated(const bsk_plan* plan, long long* offset, long long* ncell);
+/* Sublattice levels (S a multiple of 4).  The sum over a sublattice H of
+ * the S^3 mesh is |H|/S^3 times the full sum whenever no nonzero vector of
+ * H's dual lattice is a frequency of the product, i.e. for a triangle whose
+ * shells' largest |n| components sum to R_inf and whose largest |n| sum to
+ * R_2, when every nonzero dual vector v has a component above R_inf or
+ * |v| > R_2.  The level layout (BSK_STAGE_LEVELS) orders a shell row as
+ * four parts, P0 = C0 minus E, P1 = E, P2 = O, P3 = C1 minus O (C0 / C1:
+ * cells of even / odd x+y+z; E / O: x, y, z all even / all odd), each
+ * zero-padded to a multiple of 128 cells, a permutation of the mesh in
+ * which every level is one contiguous range:
+ *   0: the whole mesh P0..P3 (dual S Z^3);
+ *   1: C0 = P0 P1, 1/2 of the cells (dual (S/2)(1,1,1) + S Z^3: |v| = 0.87 S);
+ *   2: E and O = P1 P2, 1/4 (dual (S/2)(1,1,0) and permutations + S Z^3:
+ *      |v| = 0.71 S);
+ *   3: E = P1, 1/8 (dual (S/2) Z^3: a component S/2).
+ * A cell (x, y, z) of a part sits at ((c S/2 + x/2) S/2 + y/2) S/2 + z/2,
+ * c its row class: 0 in E and O, 2 (x%2) + y%2 - 1 in P0, 2 (x%2) + y%2 in P3.
+ * bsk_plan_levels: offset[L] and ncell[L] of level L within a shell row
+ * (BSK_LEVELS entries each) and the smallest shell stride of the layout.
+ * bsk_plan_triangle_levels: level[t] = the cheapest exact level of each
+ * triangle (bin indices).  No reference counterpart: the reference sums
+ * every triangle over the whole N^3 mesh (bskit/main.py:1871-1882). */
+#define BSK_LEVELS 4
+int bsk_plan_levels(const bsk_plan* plan, long long* offset, long long* ncell,
+                    long long* stride);
 int bsk_plan_triangle_levels(const bsk_plan* plan, int ntri, const int* tri_host, int* level);
 /* Radix-plan variant (0 or 1) of FFT pass 0..5 (z r2c, y, x, masked x^-1,
  * y^-1, z c2r); chosen by the autotuner. */
@@ -133,10 +145,10 @@ int bsk_forward(bsk_plan* plan, int dtype, const void* field, void* dk, void* wo
  * the slab from that work (so one X stage serves many slabs); both = 3.
  * BSK_STAGE_Y instead of BSK_STAGE_YZ runs the y pass only and leaves its
  * output in work for bsk_triples_zfused (FP32, x0 = 0, xslab = S; out
- * unused).  BSK_STAGE_DECIMATE with BSK_STAGE_YZ over the whole mesh
- * (x0 = 0, xslab = S, out_cell0 = 0, even S) also writes each shell's
- * decimated copy (bsk_plan_decimated) after its S^3 cells; shell_stride
- * must hold both.
+ * unused).  BSK_STAGE_LEVELS with BSK_STAGE_YZ over the whole mesh
+ * (x0 = 0, xslab = S, out_cell0 = 0, S a multiple of 4) writes each shell
+ * in the sublattice level layout (bsk_plan_levels) instead of C order;
+ * shell_stride must hold the layout.
  * Output shell b (relative to shell0) lands at
  * out + (shell0+b)*shell_stride + out_cell0 (elements) with the slab's cells
  * in C order; for BSK_FMT_BF16X2 out is the hi plane and out_lo the lo
@@ -148,7 +160,7 @@ int bsk_forward(bsk_plan* plan, int dtype, const void* field, void* dk, void* wo
 #define BSK_STAGE_X 1
 #define BSK_STAGE_YZ 2
 #define BSK_STAGE_Y 4
-#define BSK_STAGE_DECIMATE 8
+#define BSK_STAGE_LEVELS 8
 int bsk_shells_workspace(const bsk_plan* plan, int dtype, int nshell, int xslab, size_t* bytes);
 int bsk_shells(bsk_plan* plan, int dtype, int kind, const void* dk, int shell0, int nshell, int x0,
                int xslab, int stages, int fmt, void* out, void* out_lo, long long shell_stride,
@@ -212,15 +224,20 @@ int bsk_pairs_upload(bsk_pairs* pairs);
 int bsk_pairs_destroy(bsk_pairs* pairs);
 int bsk_pairs_info(const bsk_pairs* pairs, int* npair, int* ntile, int* nitem,
                    int* items_per_wave, int* npass);
-/* Two-level plan: triangles with level[t] = 1 (bsk_plan_triangle_levels)
- * are summed over the shells' decimated copies and scaled by 8, the others
- * over the full mesh; run it with bsk_triples_mfma_levels.  npass0 / npass1:
- * launches over the full / decimated mesh. */
+/* Plan over sublattice levels: triangle t is summed over the cells of
+ * level[t] (bsk_plan_triangle_levels) and scaled by 2^level[t]; run it with
+ * bsk_triples_mfma_levels.  npass[L] (BSK_LEVELS entries): launches over
+ * level L. */
 int bsk_pairs_create_levels(bsk_pairs** pairs, int mode, int nshell, int ntri,
                             const int* tri_host, const int* level);
-int bsk_pairs_level_passes(const bsk_pairs* pairs, int* npass0, int* npass1);
+int bsk_pairs_level_passes(const bsk_pairs* pairs, int* npass);
+/* Shell window of each level: its passes stage only shells [first[L],
+ * first[L] + count[L]) (the smallest range holding its triangles' shells);
+ * the pair rows and third-side blocks of its tiles count from first[L], and
+ * row count[L] is the zero row. */
+int bsk_pairs_level_windows(const bsk_pairs* pairs, int* first, int* count);
 /* Partial-buffer slot of each triangle (item * 256 + row * 16 + column,
- * plus 1 << 30 for a decimated-mesh triangle). */
+ * plus level << 28). */
 int bsk_pairs_slots(const bsk_pairs* pairs, int* slot_host);
 /* Host copies of the launch tables, for inspection: items
  * [npass * 8 * items_per_wave] (tile | kb << 16 | padding flag << 29 |
@@ -233,15 +250,15 @@ int bsk_triples_mfma(const bsk_pairs* pairs, const uint16_t* hi_i, const uint16_
                      const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
                      const uint16_t* lo_k, long long shell_stride, long long ncell, int nwg,
                      double scale, double* out_dev, void* work, void* stream);
-/* bsk_triples_mfma with the decimated copies of the same shells: level-1
- * triangles of a two-level plan run over cells [dec_offset, dec_offset +
- * dec_ncell) of each shell (bsk_plan_decimated), the rest over [0, ncell).
- * bsk_triples_mfma refuses a plan with level-1 triangles. */
+/* bsk_triples_mfma over shells in the level layout: the passes of level L
+ * run over cells [level_off[L], level_off[L] + level_ncell[L]) of each
+ * shell row (host arrays of BSK_LEVELS entries, from bsk_plan_levels).
+ * bsk_triples_mfma refuses a plan with triangles above level 0. */
 int bsk_triples_mfma_levels(const bsk_pairs* pairs, const uint16_t* hi_i, const uint16_t* lo_i,
                             const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,
-                            const uint16_t* lo_k, long long shell_stride, long long ncell,
-                            long long dec_offset, long long dec_ncell, int nwg, double scale,
-                            double* out_dev, void* work, void* stream);
+                            const uint16_t* lo_k, long long shell_stride,
+                            const long long* level_off, const long long* level_ncell, int nwg,
+                            double scale, double* out_dev, void* work, void* stream);
 /* out_dev == NULL in bsk_triples_mfma leaves the per-workgroup partials in
  * work; bsk_triples_mfma_reduce then sums them in FP64 into out_dev (the
  * same result as one call with out_dev). */

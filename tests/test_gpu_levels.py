@@ -1,11 +1,12 @@
-"""Decimated shell copies and the two-level MFMA triple sums (GPU).
+"""Sublattice level layout and the level-split MFMA triple sums (GPU).
 
-The cells of the S^3 shell mesh with even x, y and z form an (S/2)^3 mesh
-on which each shell field is the same sum of modes.  A triangle whose
-shells' largest |n| components add up to less than S/2 has no product
-frequency that aliases to zero on that mesh, so its cell sum there is
-exactly 1/8 of the full one.
+The sum over a sublattice H of the S^3 shell mesh equals |H|/S^3 of the
+full sum when no nonzero vector of H's dual lattice is a frequency of the
+triple product.  Levels (include/bskhip.h): 0 the whole mesh, 1 the cells
+with x+y+z even, 2 the cells with x, y, z of equal parity, 3 the all-even
+cells; each is a contiguous range of a shell row in the level layout.
 """
+import itertools
 import math
 
 import numpy as np
@@ -33,38 +34,88 @@ def _field(N, seed, dtype):
     return (f + 0.3 * (f * f - 1)).astype(dtype)
 
 
+def _layout_positions(S, nc, ne):
+    """Row position of every cell (C order) in the level layout, from its
+    definition: parts P0 = C0 minus E at 0, P1 = E at nc, P2 = O at nc + ne,
+    P3 = C1 minus O at nc + 2 ne; within a part ((c S/2 + x/2) S/2 + y/2) S/2
+    + z/2 with the row class c = 0 in E and O, 2 (x%2) + y%2 - 1 in P0 and
+    2 (x%2) + y%2 in P3."""
+    h = S // 2
+    pos = np.empty((S, S, S), np.int64)
+    part = np.empty((S, S, S), np.int64)
+    for x in range(S):
+        for y in range(S):
+            for z in range(S):
+                xy = 2 * (x % 2) + y % 2
+                if (x + y + z) % 2 == 0:
+                    p, base, c = (1, nc, 0) if x % 2 == y % 2 == z % 2 == 0 else (0, 0, xy - 1)
+                else:
+                    p, base, c = (2, nc + ne, 0) if x % 2 == y % 2 == z % 2 == 1 else (
+                        3, nc + 2 * ne, xy)
+                pos[x, y, z] = base + ((c * h + x // 2) * h + y // 2) * h + z // 2
+                part[x, y, z] = p
+    return pos.reshape(-1), part.reshape(-1)
+
+
 @pytest.mark.parametrize('N,nb,width,fmt', [(64, 8, 2, 'bf16x2'), (128, 12, 2, 'f32'),
                                             (64, 6, 3, 'f64'), (256, 4, 16, 'bf16x2')])
-def test_decimated_copy_is_the_even_cells(N, nb, width, fmt):
-    """The z pass's decimated copy holds exactly the even cells (bit for bit)
-    and zeros in its padding, whatever the buffer held before."""
+def test_level_layout_holds_each_part(N, nb, width, fmt):
+    """The z pass's level layout: each part holds exactly its cells, bit for
+    bit the values of the C-order shells, in the documented order, and
+    zeros in its padding whatever the buffer held before."""
     from bskit_amd import _lib as L
     box = 800.0
     lo, hi = _bins(nb, width, box)
     eng = _engine(N, box, lo, hi)
     S, H = eng.S, eng.S // 2
+    lay = eng.layout
     code = {'bf16x2': L.FMT_BF16X2, 'f32': L.FMT_F32, 'f64': L.FMT_F64}[fmt]
     dt = np.float64 if fmt == 'f64' else np.float32
+    tdt = torch.float64 if fmt == 'f64' else torch.float32
     dk = eng.forward(torch.from_numpy(_field(N, 5, dt)).cuda())
-    out = eng.alloc_shells(code, nb, decimated=True)
-    off, cells = out.dec
-    assert off >= S ** 3 and off % 128 == 0 and cells == -(-H ** 3 // 128) * 128
-    planes = [out.data] + ([out.lo] if out.lo is not None else [])
-    for p in planes:
+    ref = eng.shells(dk, fmt=code, dtype=tdt, out=eng.alloc_shells(code, nb, levels=False))
+    out = eng.alloc_shells(code, nb, levels=True)
+    assert out.layout == lay and out.stride == lay.stride
+    nc, ne = lay.off[2], lay.ncell[3]
+    assert lay.off == (0, 0, nc, nc) and lay.ncell == (2 * nc + 2 * ne, nc + ne, 2 * ne, ne)
+    assert nc == -(-3 * H ** 3 // 128) * 128 and ne == -(-H ** 3 // 128) * 128
+    planes = [(out.data, ref.data)] + ([(out.lo, ref.lo)] if out.lo is not None else [])
+    for p, _ in planes:
         p.fill_(7)
-    sh = eng.shells(dk, fmt=code, dtype=torch.float64 if fmt == 'f64' else torch.float32,
-                    out=out)
-    for p in planes:
-        full = p[:, :S ** 3].reshape(nb, S, S, S)
-        dec = p[:, off:off + H ** 3].reshape(nb, H, H, H)
-        assert torch.equal(dec, full[:, ::2, ::2, ::2])
-        assert bool((p[:, off + H ** 3:off + cells] == 0).all())
-    print('decimated copy N=%d S=%d nb=%d %s: %d + %d cells' % (N, S, nb, fmt, S ** 3, cells))
-    assert sh is out
+    eng.shells(dk, fmt=code, dtype=tdt, out=out)
+    pos, part = _layout_positions(S, nc, ne)
+    # a permutation of the cells: every position taken once, the rest padding
+    assert len(np.unique(pos)) == S ** 3
+    assert np.array_equal(np.bincount(part), [3 * H ** 3, H ** 3, H ** 3, 3 * H ** 3])
+    pad = np.ones(lay.stride, bool)
+    pad[pos] = False
+    pos_d = torch.from_numpy(pos).cuda()
+    pad_d = torch.from_numpy(pad).cuda()
+    for p, r in planes:
+        assert torch.equal(p[:, pos_d], r[:, :S ** 3])
+        assert bool((p[:, pad_d] == 0).all())
+    assert torch.equal(out.as_float(), ref.as_float())
+    print('level layout N=%d S=%d nb=%d %s: stride %d (S^3 = %d)' % (N, S, nb, fmt, lay.stride,
+                                                                    S ** 3))
+
+
+def _exact_py(S, L, rinf, r2):
+    """The level rule restated: no nonzero dual vector (S/2)(a, b, c) of the
+    level's parity class with every component <= rinf and norm <= r2."""
+    h = S // 2
+    A = rinf // h
+    for a, b, c in itertools.product(range(-A, A + 1), repeat=3):
+        if (a, b, c) == (0, 0, 0):
+            continue
+        cls = [a % 2 == 0 and b % 2 == 0 and c % 2 == 0, a % 2 == b % 2 == c % 2,
+               (a + b + c) % 2 == 0, True][L]
+        if cls and h * h * (a * a + b * b + c * c) <= r2 * r2 * (1 + 1e-12):
+            return False
+    return True
 
 
 @pytest.mark.parametrize('N,nb,width', [(64, 10, 1), (128, 12, 2), (512, 32, 2)])
-def test_triangle_levels_follow_the_mode_bound(N, nb, width):
+def test_triangle_levels_follow_the_dual_lattice_rule(N, nb, width):
     from bskit_amd import bins
     box = 1000.0
     lo, hi = _bins(nb, width, box)
@@ -72,49 +123,60 @@ def test_triangle_levels_follow_the_mode_bound(N, nb, width):
     tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), 1)
     lv = eng.triangle_levels(tri)
     m = np.array([math.isqrt(int(v)) if v >= 0 else 0 for v in eng.n2hi])
-    ref = (2 * m[tri].sum(axis=1) < eng.S).astype(np.int32)
-    assert np.array_equal(lv, ref)
-    print('levels N=%d S=%d: %d of %d triangles on the decimated mesh'
-          % (N, eng.S, lv.sum(), len(tri)))
+    r = np.sqrt(np.maximum(eng.n2hi, 0).astype(float))
+    ref = []
+    for t in tri:
+        rinf, r2 = int(m[t].sum()), float(r[t].sum())
+        ref.append(next(L for L in (3, 2, 1, 0) if L == 0 or _exact_py(eng.S, L, rinf, r2)))
+    assert np.array_equal(lv, np.array(ref))
+    counts = np.bincount(lv, minlength=4)
+    print('levels N=%d S=%d: %s of %d triangles at levels 0..3' % (N, eng.S, counts.tolist(),
+                                                                  len(tri)))
     if N == 512:
-        assert 0 < lv.sum() < len(tri)
+        assert (counts > 0).all()
 
 
-def test_decimated_sums_are_exact_in_fp64():
-    """FP64 shells: for level-1 triangles 8 x (sum over the decimated copy)
-    equals the full-mesh sum to rounding; for triangles past the bound the
-    decimated sum picks up aliased terms."""
+def test_level_sums_are_exact_in_fp64():
+    """FP64 shells in the level layout: for every triangle 2^L x (sum over the
+    cells of its level L) equals the whole-mesh sum to rounding, and for each
+    level above 0 some triangle not exact there shows aliased terms (the
+    rule is not vacuous)."""
     from bskit_amd import _lib as L
     from bskit_amd import bins
-    N, nb, box = 96, 10, 1000.0
+    N, nb, box = 128, 12, 1000.0
     lo, hi = _bins(nb, 2, box)
     eng = _engine(N, box, lo, hi)
-    S, H = eng.S, eng.S // 2
     tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), 1)
     lv = eng.triangle_levels(tri)
-    assert lv.any() and not lv.all()
+    counts = np.bincount(lv, minlength=4)
+    assert (counts > 0).all(), counts
     dk = eng.forward(torch.from_numpy(_field(N, 9, np.float64)).cuda())
     sh = eng.shells(dk, fmt=L.FMT_F64, dtype=torch.float64,
-                    out=eng.alloc_shells(L.FMT_F64, nb, decimated=True))
-    off = sh.dec[0]
-    a = sh.data[:, :S ** 3]
-    d = sh.data[:, off:off + H ** 3]
+                    out=eng.alloc_shells(L.FMT_F64, nb, levels=True))
+    a = sh.level_values(0)
     full = torch.stack([(a[i] * a[j] * a[k]).sum() for i, j, k in tri]).cpu().numpy()
-    dec = 8 * torch.stack([(d[i] * d[j] * d[k]).sum() for i, j, k in tri]).cpu().numpy()
     absum = torch.stack([(a[i].abs() * a[j].abs() * a[k].abs()).sum()
                          for i, j, k in tri]).cpu().numpy()
-    err = np.abs(full - dec) / absum
-    print('fp64 decimated: level-1 max %.2e, level-0 max %.2e'
-          % (err[lv == 1].max(), err[lv == 0].max()))
-    assert err[lv == 1].max() < 1e-12
-    assert err[lv == 0].max() > 1e-6  # the bound is not vacuous here
+    for L_ in range(1, 4):
+        d = sh.level_values(L_)
+        part = 2.0 ** L_ * torch.stack([(d[i] * d[j] * d[k]).sum()
+                                        for i, j, k in tri]).cpu().numpy()
+        err = np.abs(part - full) / absum
+        # the dual of level 3 contains the duals of levels 1 and 2, so a
+        # triangle exact at 3 is exact at every level
+        exact = (lv == L_) | (lv == 3)
+        below = lv == 0 if L_ == 1 else lv < L_
+        print('fp64 level %d: exact max %.2e, below-level max %.2e'
+              % (L_, err[exact].max(), err[below].max()))
+        assert err[exact].max() < 1e-12
+        assert err[below].max() > 1e-6
 
 
 @pytest.mark.parametrize('N,nb,width,mode', [(128, 12, 2, 0), (512, 32, 2, 0), (128, 12, 2, 1),
                                              (96, 10, 2, 2)])
-def test_two_level_mfma_matches_one_level_and_fp64(N, nb, width, mode):
-    """Split-BF16 sums with the two-level plan against the one-level plan on
-    the same shells and against the FP64 plain path."""
+def test_level_mfma_matches_one_level_and_fp64(N, nb, width, mode):
+    """Split-BF16 sums with the level plan against the one-level plan on the
+    same fields and against the FP64 plain path."""
     from bskit_amd import _lib as L
     from bskit_amd import bins
     box = 1000.0
@@ -128,13 +190,13 @@ def test_two_level_mfma_matches_one_level_and_fp64(N, nb, width, mode):
     nf = mode + 1
     fs = [_field(N, 40 + i, np.float64) for i in range(nf)]
     d32 = [eng.forward(torch.from_numpy(f.astype(np.float32)).cuda()) for f in fs]
-    two = [eng.shells(d, fmt=L.FMT_BF16X2, out=eng.alloc_shells(L.FMT_BF16X2, nb, True))
+    lvl = [eng.shells(d, fmt=L.FMT_BF16X2, out=eng.alloc_shells(L.FMT_BF16X2, nb, True))
            for d in d32]
     one = [eng.shells(d, fmt=L.FMT_BF16X2, out=eng.alloc_shells(L.FMT_BF16X2, nb, False))
            for d in d32]
     plan = eng.pair_plan(tri, nb, mode, levels=True)
-    assert plan.npass_dec > 0
-    got2 = eng.triples(two, tri, path='mfma', scale=sc).cpu().numpy()
+    assert sum(plan.level_passes[1:]) > 0
+    got2 = eng.triples(lvl, tri, path='mfma', scale=sc).cpu().numpy()
     got1 = eng.triples(one, tri, path='mfma', scale=sc).cpu().numpy()
     s64 = [eng.shells(eng.forward(torch.from_numpy(f).cuda()), fmt=L.FMT_F64,
                       dtype=torch.float64) for f in fs]
@@ -145,7 +207,7 @@ def test_two_level_mfma_matches_one_level_and_fp64(N, nb, width, mode):
                          for i, j, k in tri]).cpu().numpy() * sc
     e21 = (np.abs(got2 - got1) / absum).max()
     e2r = (np.abs(got2 - ref) / absum).max()
-    print('two-level N=%d nb=%d mode %d: %d full + %d decimated passes, vs one-level %.2e, '
-          'vs fp64 %.2e' % (N, nb, mode, plan.npass_full, plan.npass_dec, e21, e2r))
+    print('levels N=%d nb=%d mode %d: passes per level %s, windows %s, vs one-level %.2e, '
+          'vs fp64 %.2e' % (N, nb, mode, plan.level_passes, plan.level_windows, e21, e2r))
     assert e21 < 5e-5
     assert e2r < 5e-5

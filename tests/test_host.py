@@ -1,5 +1,6 @@
 """Host-side logic (CPU): bins, output formats, combiner, library ABI surface."""
 import ctypes
+import itertools
 import os
 import re
 import subprocess
@@ -178,11 +179,12 @@ def test_library_loads_and_host_calls_work():
         PairPlan(np.array([[3, 0, 0]]), 3, L.MODE_AUTO, upload=False)
 
 
-def _emulate_pair_plan(plan, sets, tri, mode, nshell, dec_sets=None):
+def _emulate_pair_plan(plan, sets, tri, mode, nshell, level_sets=None):
     """FP64 numpy emulation of the MFMA kernel's tables: per item the 16x16
     tile sum_x (d_a d_b)(x) d_k(x), read back through the triangle slots.
-    Items of the decimated-mesh passes (after npass_full) sum dec_sets; the
-    slot's bit 30 scales by 8."""
+    Items of the passes of sublattice level L sum level_sets[L] (the passes
+    of each level follow those of the level before); the slot's bits 28-29
+    (the level) scale by 2^L."""
     items, prs = plan.tables()
     # the kernel re-forms pair products only at flagged items: an unflagged
     # item continues the previous item's tile in the same wave's list
@@ -197,27 +199,30 @@ def _emulate_pair_plan(plan, sets, tri, mode, nshell, dec_sets=None):
                 break
             if lst[s] >= 0:
                 assert (lst[s] & 0xffff) == (lst[s - 1] & 0xffff)
-    nb = nshell
     part = np.zeros((len(items), 16, 16))
     for g, d in enumerate(items):
         if (d >> 29) & 1:
             continue  # padding
-        dec = g // (8 * ni) >= plan.npass_full
-        f0, f1, f2 = dec_sets if dec else sets
+        lv = int(np.searchsorted(np.cumsum(plan.level_passes), g // (8 * ni), side='right'))
+        f0, f1, f2 = level_sets[lv] if level_sets is not None else sets
+        # the level's shell window: rows and blocks count from w0, row nw is zero
+        w0, nw = plan.level_windows[lv]
+        assert 0 <= w0 and w0 + nw <= nshell
         z = np.zeros_like(f0[0])
         tile = d & 0xffff
         kb = (d >> 16) & 0xff
         for row in range(16):
             pr = prs[tile * 16 + row]
             a, b = pr & 0xffff, pr >> 16
-            pa = (f0[a] if a < nb else z) * (f1[b] if b < nb else z)
+            assert a <= nw and b <= nw
+            pa = (f0[w0 + a] if a < nw else z) * (f1[w0 + b] if b < nw else z)
             for col in range(16):
                 k = 16 * kb + col
-                if k < nb:
-                    part[g, row, col] = np.sum(pa * f2[k])
+                if k < nw:
+                    part[g, row, col] = np.sum(pa * f2[w0 + k])
     flat = part.reshape(-1)
     slots = plan.slots()
-    return flat[slots & 0x3fffffff] * np.where((slots >> 30) & 1, 8.0, 1.0)
+    return flat[slots & 0x0fffffff] * 2.0 ** ((slots >> 28) & 3)
 
 
 @pytest.mark.parametrize('mode,nshell,ntri,seed', [(0, 7, 40, 1), (0, 33, 300, 2), (1, 20, 150, 3),
@@ -244,13 +249,14 @@ def test_pair_plan_tables_compute_every_triangle(mode, nshell, ntri, seed):
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize('mode,nshell,ntri,seed,frac', [(0, 33, 400, 11, 0.3), (0, 20, 150, 12, 1.0),
-                                                        (1, 20, 150, 13, 0.5),
-                                                        (2, 18, 200, 14, 0.0)])
-def test_two_level_pair_plan(mode, nshell, ntri, seed, frac):
-    """A two-level plan: level-1 triangles are read from the decimated-mesh
-    passes (their own fields here) and scaled by 8, the rest from the full
-    mesh; level 0 passes come first."""
+@pytest.mark.parametrize('mode,nshell,ntri,seed,probs', [
+    (0, 33, 400, 11, (0.1, 0.3, 0.4, 0.2)), (0, 20, 150, 12, (0, 0, 0, 1)),
+    (1, 20, 150, 13, (0.25, 0.25, 0.25, 0.25)), (2, 18, 200, 14, (1, 0, 0, 0)),
+    (0, 40, 600, 15, (0, 0.5, 0, 0.5))])
+def test_level_pair_plan(mode, nshell, ntri, seed, probs):
+    """A plan over sublattice levels: each triangle is read from the passes
+    of its level (their own fields here) and scaled by 2^level; the passes
+    of each level follow those of the level before."""
     from bskit_amd import _lib as L
     from bskit_amd.engine import PairPlan
     try:
@@ -259,24 +265,62 @@ def test_two_level_pair_plan(mode, nshell, ntri, seed, frac):
         pytest.skip('library not loadable here: %s' % e)
     rng = np.random.default_rng(seed)
     nset = {0: 1, 1: 2, 2: 3}[mode]
-    full = [rng.standard_normal((nshell, 24)) for _ in range(nset)]
-    dec = [rng.standard_normal((nshell, 6)) for _ in range(nset)]
     roles = {0: (0, 0, 0), 1: (0, 0, 1), 2: (0, 1, 2)}[mode]
+    per_level = []
+    for lv in range(4):
+        fields = [rng.standard_normal((nshell, 24 >> (lv // 2))) for _ in range(nset)]
+        per_level.append([fields[r] for r in roles])
     tri = rng.integers(0, nshell, size=(ntri, 3))
-    levels = (rng.random(ntri) < frac).astype(np.int32)
+    levels = rng.choice(4, size=ntri, p=probs).astype(np.int32)
     plan = PairPlan(tri, nshell, mode, upload=False, levels=levels)
-    assert (plan.npass_dec > 0) == bool(levels.any())
-    assert (plan.npass_full > 0) == bool((levels == 0).any())
-    sets = [full[r] for r in roles]
-    dsets = [dec[r] for r in roles]
-    got = _emulate_pair_plan(plan, sets, tri, mode, nshell, dsets)
-    ref = np.array([8 * np.sum(dsets[0][i] * dsets[1][j] * dsets[2][k]) if lv else
-                    np.sum(sets[0][i] * sets[1][j] * sets[2][k])
+    for lv in range(4):
+        assert (plan.level_passes[lv] > 0) == bool((levels == lv).any()), lv
+    got = _emulate_pair_plan(plan, per_level[0], tri, mode, nshell, per_level)
+    ref = np.array([2.0 ** lv * np.sum(per_level[lv][0][i] * per_level[lv][1][j] *
+                                       per_level[lv][2][k])
                     for (i, j, k), lv in zip(tri, levels)])
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
-    # a plan without levels, or with all zeros, has no decimated passes
+    # all triangles at level 0: the plan of bsk_pairs_create
     p0 = PairPlan(tri, nshell, mode, upload=False, levels=np.zeros(ntri, np.int32))
-    assert p0.npass_dec == 0 and np.array_equal(p0.slots() >> 30, np.zeros(ntri))
+    p1 = PairPlan(tri, nshell, mode, upload=False)
+    assert p0.level_passes == p1.level_passes and p0.level_passes[1:] == [0, 0, 0]
+    assert np.array_equal(p0.slots(), p1.slots())
+
+
+@pytest.mark.parametrize('mode', [0, 1, 2])
+def test_level_pair_plan_stages_shell_windows(mode):
+    """Each level stages only the shell range its triangles use: the window
+    is the smallest such range and the emulated sums still match."""
+    from bskit_amd import _lib as L
+    from bskit_amd.engine import PairPlan
+    try:
+        L.lib()
+    except (ImportError, OSError) as e:
+        pytest.skip('library not loadable here: %s' % e)
+    rng = np.random.default_rng(20 + mode)
+    nshell = 40
+    spans = [(25, 40), (10, 30), (0, 18), (3, 9)]  # shells used per level
+    tri, levels = [], []
+    for lv, (a, b) in enumerate(spans):
+        t = rng.integers(a, b, size=(60, 3))
+        t[0] = (a, b - 1, a)  # both ends of the span
+        tri.append(t)
+        levels += [lv] * len(t)
+    tri = np.concatenate(tri)
+    levels = np.array(levels, np.int32)
+    nset = {0: 1, 1: 2, 2: 3}[mode]
+    roles = {0: (0, 0, 0), 1: (0, 0, 1), 2: (0, 1, 2)}[mode]
+    per_level = []
+    for lv in range(4):
+        fields = [rng.standard_normal((nshell, 16)) for _ in range(nset)]
+        per_level.append([fields[r] for r in roles])
+    plan = PairPlan(tri, nshell, mode, upload=False, levels=levels)
+    assert plan.level_windows == [(a, b - a) for a, b in spans]
+    got = _emulate_pair_plan(plan, per_level[0], tri, mode, nshell, per_level)
+    ref = np.array([2.0 ** lv * np.sum(per_level[lv][0][i] * per_level[lv][1][j] *
+                                       per_level[lv][2][k])
+                    for (i, j, k), lv in zip(tri, levels)])
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
 
 
 def test_two_level_pair_plan_rejects_bad_levels():
@@ -288,7 +332,9 @@ def test_two_level_pair_plan_rejects_bad_levels():
         pytest.skip('library not loadable here: %s' % e)
     tri = np.array([[0, 1, 2], [1, 1, 1]])
     with pytest.raises(L.BskError, match='level'):
-        PairPlan(tri, 4, 0, upload=False, levels=np.array([0, 2], np.int32))
+        PairPlan(tri, 4, 0, upload=False, levels=np.array([0, 4], np.int32))
+    with pytest.raises(L.BskError, match='level'):
+        PairPlan(tri, 4, 0, upload=False, levels=np.array([-1, 0], np.int32))
 
 
 @pytest.mark.parametrize('mode,nshell', [(2, 70), (1, 100)])
@@ -408,3 +454,30 @@ def test_subbox_index_roundtrip_and_cuts():
     assert np.allclose(sm.attrs['BoxSize'], 50.) and list(sm.attrs['Nmesh']) == [4, 4, 4]
     sl = subbox.field_subbox_slice(f, [(50., 100.), (0., 50.), (50., 100.)], BoxSize=100.)
     assert np.array_equal(sl.array, sm.array)
+
+
+@pytest.mark.parametrize('S', [8, 12])
+def test_sublattice_duals_by_brute_force(S):
+    """The dual lattices behind the sublattice levels, by brute force on a
+    small mesh: the frequencies q (mod S) at which sum_{x in H} e^{2 pi i q.x/S}
+    does not vanish are exactly (S/2)(a, b, c) with a, b, c all even (level 0,
+    the whole mesh), all of one parity (1: x+y+z even), of even sum (2: x, y,
+    z of equal parity) or any (3: all even)."""
+    i = np.arange(S)
+    x, y, z = np.meshgrid(i, i, i, indexing='ij')
+    cells = {0: np.ones_like(x, bool), 1: (x + y + z) % 2 == 0,
+             2: (x % 2 == y % 2) & (y % 2 == z % 2),
+             3: (x % 2 == 0) & (y % 2 == 0) & (z % 2 == 0)}
+    h = S // 2
+    for L, mask in cells.items():
+        assert mask.sum() * 2 ** L == S ** 3
+        # the sum over H of e^{2 pi i q.x / S} for every q: an FFT of the mask
+        f = np.fft.fftn(mask.astype(float))
+        dual = {tuple(q) for q in np.argwhere(np.abs(f) > 1e-6 * mask.sum())}
+        rule = set()
+        for a, b, c in itertools.product(range(2), repeat=3):
+            cls = [a % 2 == 0 and b % 2 == 0 and c % 2 == 0, a % 2 == b % 2 == c % 2,
+                   (a + b + c) % 2 == 0, True][L]
+            if cls:
+                rule.add(((h * a) % S, (h * b) % S, (h * c) % S))
+        assert dual == rule, (L, sorted(dual), sorted(rule))
