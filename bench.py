@@ -404,11 +404,14 @@ def main():
             plan = eng.pair_plan(wl['tri'], nsh, 0)
             return roofline('k_triple_zrows', ms, d2_bytes, plan.npass, flops_fused, plan)
         plan = eng.pair_plan(wl['tri'], nsh, 0, levels=levels)
-        # split-BF16 hi+lo of every shell read once per level summed over
-        cells = (sum(eng.layout.ncell[k] for k in range(L.LEVELS) if plan.level_passes[k])
-                 if levels else ncell)
-        return roofline('k_triple_pairs', ms, nsh * 4 * float(cells), plan.npass, flops_sep,
-                        plan)
+        # split-BF16 hi+lo of the shells each segment stages, read once
+        if levels:
+            parts = eng.layout.part_cells()
+            alg = sum(4.0 * wn * sum(parts[q] for q in range(4) if pm >> q & 1)
+                      for pm, w0, wn, npass, ni in plan.segments)
+        else:
+            alg = nsh * 4 * ncell
+        return roofline('k_triple_pairs', ms, alg, plan.npass, flops_sep, plan)
 
     roof = path_roofline(main_path, float(np.mean(timer.pair_durations('kernel'))))
     if alt is not None:

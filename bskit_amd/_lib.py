@@ -59,13 +59,12 @@ _SIGS = {
                           _vp, _vp, _vp],
     'bsk_pairs_create': [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _int_p],
     'bsk_pairs_create_levels': [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _int_p, _int_p],
-    'bsk_pairs_level_passes': [_vp, _int_p],
-    'bsk_pairs_level_windows': [_vp, _int_p, _int_p],
+    'bsk_pairs_segments': [_vp, _int_p, _int_p, _int_p, _int_p, _int_p, _int_p],
     'bsk_pairs_upload': [_vp],
     'bsk_pairs_destroy': [_vp],
     'bsk_pairs_info': [_vp, _int_p, _int_p, _int_p, _int_p, _int_p],
     'bsk_pairs_slots': [_vp, _int_p],
-    'bsk_pairs_tables': [_vp, _int_p, _int_p],
+    'bsk_pairs_tables': [_vp, _int_p, _int_p, _int_p],
     'bsk_triples_mfma_workspace': [_vp, _c_int, _c_size_p],
     'bsk_triples_mfma': [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll, _c_ll, _c_int, _c_double, _vp,
                          _vp, _vp],

@@ -617,6 +617,12 @@ int bsk_triples_tiled(int dtype, const void* fields, long long shell_stride, int
 }  // extern "C"
 
 // ------------------------------------------------------------ pair plans
+// Parts of the level layout (fft_passes.h ShellLevels) in row order: P0 =
+// C0 minus E, P1 = E, P2 = O, P3 = C1 minus O; the levels whose cells
+// include each part (bit L): every level includes E.
+constexpr int kParts = 4;
+constexpr unsigned kPartLevels[kParts] = {0x3, 0xf, 0x5, 0x1};
+
 struct bsk_pairs {
   int mode = 0;
   int nshell = 0;
@@ -624,17 +630,24 @@ struct bsk_pairs {
   int ntri = 0;
   int npair = 0, ntile = 0, nitem = 0;
   int nkb = 0;
-  int ni = 1;          // items per wave and launch (compiled count)
-  int npass = 1;
-  int lpass[BSK_LEVELS] = {1, 0, 0, 0};  // passes per sublattice level, in level order
-  // shell window (first shell, count) each level's passes stage
-  int lwin[BSK_LEVELS][2] = {{0, 1}, {0, 1}, {0, 1}, {0, 1}};
-  int item_total = 0;  // npass * waves * ni
+  int ni = 1;          // largest items per wave of a launch (compiled count)
+  int npass = 1;       // launches in all
+  // launch segments, in launch order: runs of consecutive parts with the same
+  // levels (the first holds E and every item, its launches store the
+  // partials, the rest add to them); per segment the parts (bit p), the
+  // staged shell window (first, count) and the launches
+  int nseg = 1;
+  int seg_parts[kParts] = {0xf, 0, 0, 0};
+  int seg_win[kParts][2] = {{0, 1}, {0, 1}, {0, 1}, {0, 1}};
+  int seg_pass[kParts] = {1, 0, 0, 0};
+  int seg_ni[kParts] = {1, 1, 1, 1};  // items per wave of the segment's launches
+  int item_total = 0;  // distinct items: partial positions per workgroup
   std::vector<int> slot;
-  std::vector<int> h_items, h_pairs;
-  int* d_items = nullptr;  // [npass][waves * ni]
-  int* d_pairs = nullptr;  // [ntile + 1][16]
-  int* d_slot = nullptr;   // [ntri]
+  std::vector<int> h_items, h_outpos, h_pairs;
+  int* d_items = nullptr;   // per launch [waves * ni of its segment]
+  int* d_outpos = nullptr;  // the same shape
+  int* d_pairs = nullptr;   // [ntile + 1][16]
+  int* d_slot = nullptr;    // [ntri]
 };
 
 namespace {
@@ -788,37 +801,30 @@ int upload(int** dst, const std::vector<int>& v) {
   return BSK_OK;
 }
 
-// One level's share of a pair plan: role choice, tiles, units (a tile with
-// at most `cap` of its kb blocks; each unit forms the tile's pair products
-// once) and their (pass, wave) bins.
+// One level's share of a pair plan: role choice, tiles and units (a tile
+// with at most `cap` of its kb blocks; each unit forms the tile's pair
+// products once), in absolute shell indices.
 struct LevelPlan {
   std::vector<int> tri;  // global triangle indices of this level
   std::vector<Role> roles;
   PairTiles tiles;
   std::vector<int> unit_tile;
   std::vector<unsigned> unit_mask;
-  std::vector<std::vector<int>> bins;  // [npass * waves] unit lists
-  int npass = 0, maxload = 0;
-  int lo = 0, nsh = 1, nkb = 1;  // shell window (shell indices of the plan are relative to lo)
+  int lo = 0, hi = 0;  // shells used: [lo, hi)
 };
 
-LevelPlan plan_level(int mode, int cap, const std::vector<int>& idx, const int* tri) {
+LevelPlan plan_level(int mode, int nkb, int cap, const std::vector<int>& idx, const int* tri) {
   LevelPlan lp;
   lp.tri = idx;
-  // shell window of the level: its passes stage only shells [lo, lo + nsh)
-  int lo = INT_MAX, hi = -1;
-  for (int t : idx)
-    for (int e = 0; e < 3; ++e) {
-      lo = std::min(lo, tri[3 * (size_t)t + e]);
-      hi = std::max(hi, tri[3 * (size_t)t + e]);
-    }
-  lp.lo = idx.empty() ? 0 : lo;
-  lp.nsh = idx.empty() ? 1 : hi + 1 - lo;
-  lp.nkb = (lp.nsh + 15) / 16;
-  const int nkb = lp.nkb;
   std::vector<int> sub((size_t)3 * idx.size());
+  lp.lo = INT_MAX;
   for (size_t t = 0; t < idx.size(); ++t)
-    for (int e = 0; e < 3; ++e) sub[3 * t + e] = tri[3 * (size_t)idx[t] + e] - lp.lo;
+    for (int e = 0; e < 3; ++e) {
+      sub[3 * t + e] = tri[3 * (size_t)idx[t] + e];
+      lp.lo = std::min(lp.lo, sub[3 * t + e]);
+      lp.hi = std::max(lp.hi, sub[3 * t + e] + 1);
+    }
+  if (idx.empty()) lp.lo = 0;
   const int ntri = (int)idx.size();
   // role assignment: the fixed roles of <AAB>/<ABC>; for auto spectra the
   // choice of the pair sides that needs the fewest items
@@ -836,7 +842,7 @@ LevelPlan plan_level(int mode, int cap, const std::vector<int>& idx, const int* 
       }
     }
   }
-  const int W = kPairWaves, ntile = (int)lp.tiles.tile_mask.size();
+  const int ntile = (int)lp.tiles.tile_mask.size();
   for (int ti = 0; ti < ntile; ++ti) {
     unsigned m = 0;
     int n = 0;
@@ -855,23 +861,33 @@ LevelPlan plan_level(int mode, int cap, const std::vector<int>& idx, const int* 
       lp.unit_mask.push_back(m);
     }
   }
-  const int nunit = (int)lp.unit_tile.size();
-  if (nunit == 0) return lp;  // no triangles: no passes
-  // units to (pass, wave) bins, costliest first (forming a tile's products
-  // weighs three items), each to the cheapest bin with room for its items
-  auto ucost = [&](int u) { return 3 + __builtin_popcount(lp.unit_mask[u]); };
+  return lp;
+}
+
+// Units (tile, kb mask) to (pass, wave) bins, costliest first (forming a
+// tile's products weighs three items), each to the cheapest bin with room for
+// its items; the fewest passes for which every unit fits.
+std::vector<std::vector<int>> bin_units(const std::vector<unsigned>& mask, int cap, int* npass,
+                                        int* maxload) {
+  const int W = kPairWaves, nunit = (int)mask.size();
+  int nitem = 0;
+  for (unsigned m : mask) {
+    nitem += __builtin_popcount(m);
+    cap = std::max(cap, __builtin_popcount(m));  // every unit fits an empty bin
+  }
+  auto ucost = [&](int u) { return 3 + __builtin_popcount(mask[u]); };
   std::vector<int> order(nunit);
   for (int i = 0; i < nunit; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ucost(x) > ucost(y); });
-  for (lp.npass = std::max(1, (lp.tiles.nitem + W * cap - 1) / (W * cap));; ++lp.npass) {
-    // ends: every unit fits an empty bin (popcount <= cap)
-    lp.bins.assign((size_t)lp.npass * W, {});
-    std::vector<int> load((size_t)lp.npass * W, 0), cost((size_t)lp.npass * W, 0);
+  std::vector<std::vector<int>> bins;
+  for (*npass = std::max(1, (nitem + W * cap - 1) / (W * cap));; ++*npass) {
+    bins.assign((size_t)*npass * W, {});
+    std::vector<int> load((size_t)*npass * W, 0), cost((size_t)*npass * W, 0);
     bool ok = true;
     for (int u : order) {
-      const int n = __builtin_popcount(lp.unit_mask[u]);
+      const int n = __builtin_popcount(mask[u]);
       int b = -1;
-      for (int q = 0; q < (int)lp.bins.size(); ++q)
+      for (int q = 0; q < (int)bins.size(); ++q)
         if (load[q] + n <= cap && (b < 0 || cost[q] < cost[b])) b = q;
       if (b < 0) {
         ok = false;
@@ -879,16 +895,17 @@ LevelPlan plan_level(int mode, int cap, const std::vector<int>& idx, const int* 
       }
       load[b] += n;
       cost[b] += ucost(u);
-      lp.bins[b].push_back(u);
+      bins[b].push_back(u);
     }
     if (ok) break;
   }
-  for (const auto& b : lp.bins) {
+  *maxload = 1;
+  for (const auto& b : bins) {
     int l = 0;
-    for (int u : b) l += __builtin_popcount(lp.unit_mask[u]);
-    lp.maxload = std::max(lp.maxload, l);
+    for (int u : b) l += __builtin_popcount(mask[u]);
+    *maxload = std::max(*maxload, l);
   }
-  return lp;
+  return bins;
 }
 
 int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri,
@@ -912,83 +929,134 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
   const int sets = mode == BSK_MODE_AUTO ? 1 : (mode == BSK_MODE_AAB ? 2 : 3);
   const int nkb = (nshell + 15) / 16;
   int nsh[3] = {nshell, nshell, nshell};
-  if (!bsk_pair_prefetch(nshell) || bsk_pair_lds_bytes(sets, nsh, nkb) > 160 * 1024 ||
-      nkb > 16) {
+  if (!bsk_pair_prefetch(nshell) || bsk_pair_lds_bytes(sets, nsh, nkb + 1) > 160 * 1024 ||
+      nkb > 16 || nshell >= 0xffff) {
     bsk_set_error("%s: %d shells exceed the MFMA kernel's LDS staging "
                   "(128 shells, 96 for <ABC>)", fn, nshell);
     return BSK_EUNSUPPORTED;
   }
-  const int W = kPairWaves, cap = bsk_pair_item_cap(sets, nshell);
+  // at most 4 items per wave per launch keeps two workgroups per CU (more
+  // items of a segment take more launches over its cells); units (a tile's
+  // items formed from one set of pair products) hold at most that many
+  const int W = kPairWaves, cap = std::min(4, bsk_pair_item_cap(sets, nshell));
   std::vector<int> idx[BSK_LEVELS];
   for (int t = 0; t < ntri; ++t) idx[level ? level[t] : 0].push_back(t);
+  // per level: tiles and units in absolute shell indices
   LevelPlan lp[BSK_LEVELS];
-  int q_win[BSK_LEVELS][2] = {{0, nshell}, {0, nshell}, {0, nshell}, {0, nshell}};
   int tile0[BSK_LEVELS + 1] = {0};
-  int lpass[BSK_LEVELS], pass0[BSK_LEVELS], maxload = 1;
+  unsigned present = 0;
   for (int L = 0; L < BSK_LEVELS; ++L) {
-    lp[L] = plan_level(mode, cap, idx[L], tri);
-    q_win[L][0] = lp[L].lo;
-    q_win[L][1] = lp[L].nsh;
+    lp[L] = plan_level(mode, nkb, cap, idx[L], tri);
     tile0[L + 1] = tile0[L] + (int)lp[L].tiles.tile_mask.size();
-    lpass[L] = lp[L].npass;
-    maxload = std::max(maxload, lp[L].maxload);
-  }
-  if (!ntri) lpass[0] = 1;  // an empty list still gets one pass of padding items
-  int npass = 0;
-  for (int L = 0; L < BSK_LEVELS; ++L) {
-    pass0[L] = npass;
-    npass += lpass[L];
+    if (!idx[L].empty()) present |= 1u << L;
   }
   const int ntile = tile0[BSK_LEVELS];
   if (ntile >= 0xffff) {
     bsk_set_error("%s: %d pair tiles exceed 65534; split the triangle list", fn, ntile);
     return BSK_EUNSUPPORTED;
   }
-  const int ni = bsk_pair_items_round(maxload);
-  if ((long long)npass * W * ni * 256 >= (1LL << 28)) {  // slot field below the level bits
-    bsk_set_error("%s: %lld items exceed the slot range; split the triangle list", fn,
-                  (long long)npass * W * ni);
+  // partial positions: one per (level tile, kb) item
+  std::vector<int> item_of((size_t)ntile * nkb, -1);
+  int nitem = 0;
+  for (int L = 0; L < BSK_LEVELS; ++L)
+    for (int ti = 0; ti < (int)lp[L].tiles.tile_mask.size(); ++ti)
+      for (int kb = 0; kb < nkb; ++kb)
+        if (lp[L].tiles.tile_mask[ti] >> kb & 1u) item_of[(size_t)(tile0[L] + ti) * nkb + kb] = nitem++;
+  if ((long long)std::max(nitem, 1) * 256 >= (1LL << 28)) {  // slot field below the level bits
+    bsk_set_error("%s: %d items exceed the slot range; split the triangle list", fn, nitem);
     return BSK_EUNSUPPORTED;
   }
-  // item descriptors: tile | kb << 16 | padding << 29 | (first item of a
-  // unit: new pair products) << 31; the passes of each level in level order
-  std::vector<int> items((size_t)npass * W * ni);
-  std::vector<int> item_of((size_t)ntile * nkb, -1);  // global item of (tile, kb)
-  for (int L = 0; L < BSK_LEVELS; ++L) {
-    for (size_t b = 0; b < lp[L].bins.size(); ++b) {
-      const int g0 = (pass0[L] * W + (int)b) * ni;
+  // segments: runs of consecutive parts with the same present levels, the
+  // one holding E (every level) first
+  struct Seg {
+    int parts;
+    unsigned lv;
+  };
+  std::vector<Seg> segs;
+  for (int p = 0; p < kParts; ++p) {
+    const unsigned lv = kPartLevels[p] & present;
+    if (!lv) continue;
+    if (!segs.empty() && segs.back().lv == lv && (segs.back().parts >> (p - 1) & 1))
+      segs.back().parts |= 1 << p;
+    else
+      segs.push_back({1 << p, lv});
+  }
+  if (segs.empty()) segs.push_back({0xf, 1u});  // empty list: one pass of padding
+  std::stable_partition(segs.begin(), segs.end(), [](const Seg& g) { return (g.parts >> 1) & 1; });
+  // per segment: the units of its levels binned into passes
+  struct SegPlan {
+    int lo, nsh, npass, ni;
+    std::vector<int> unit_tile;  // global tile
+    std::vector<unsigned> unit_mask;
+    std::vector<std::vector<int>> bins;
+  };
+  std::vector<SegPlan> sp(segs.size());
+  int maxni = 1;
+  size_t nentry = 0;
+  for (size_t g = 0; g < segs.size(); ++g) {
+    int lo = INT_MAX, hi = 0;
+    for (int L = 0; L < BSK_LEVELS; ++L) {
+      if (!(segs[g].lv >> L & 1) || idx[L].empty()) continue;
+      lo = std::min(lo, lp[L].lo);
+      hi = std::max(hi, lp[L].hi);
+      for (size_t u = 0; u < lp[L].unit_tile.size(); ++u) {
+        sp[g].unit_tile.push_back(tile0[L] + lp[L].unit_tile[u]);
+        sp[g].unit_mask.push_back(lp[L].unit_mask[u]);
+      }
+    }
+    if (lo == INT_MAX) lo = 0, hi = 1;
+    sp[g].lo = lo;
+    sp[g].nsh = hi - lo;
+    int ml = 1;
+    if (sp[g].unit_mask.empty()) {
+      sp[g].npass = 1;
+      sp[g].bins.assign(W, {});
+    } else {
+      sp[g].bins = bin_units(sp[g].unit_mask, cap, &sp[g].npass, &ml);
+    }
+    sp[g].ni = bsk_pair_items_round(ml);
+    maxni = std::max(maxni, sp[g].ni);
+    nentry += (size_t)sp[g].npass * W * sp[g].ni;
+  }
+  int npass = 0;
+  for (const auto& g : sp) npass += g.npass;
+  // launch lists: descriptors tile | kb << 16 | padding << 29 | (first item
+  // of a unit: new pair products) << 31, kb counted from the segment's first
+  // staged block; partial positions (-1: padding); each launch holds waves x
+  // the segment's items per wave
+  std::vector<int> items(nentry), outpos(nentry, -1);
+  const int pad = ntile | (1 << 29);
+  size_t seg0 = 0;
+  for (const auto& g : sp) {
+    const int kb0 = g.lo / 16, ni = g.ni;
+    for (size_t b = 0; b < g.bins.size(); ++b) {
+      const size_t e0 = seg0 + b * ni;
       int s = 0;
-      for (int u : lp[L].bins[b]) {
-        const int ti = tile0[L] + lp[L].unit_tile[u];
+      for (int u : g.bins[b]) {
+        const int ti = g.unit_tile[u];
         bool first = true;
         for (int kb = 0; kb < nkb; ++kb) {
-          if (!(lp[L].unit_mask[u] >> kb & 1u)) continue;
-          items[g0 + s] = ti | (kb << 16) | (first ? (int)0x80000000u : 0);
-          item_of[(size_t)ti * nkb + kb] = g0 + s;
+          if (!(g.unit_mask[u] >> kb & 1u)) continue;
+          items[e0 + s] = ti | ((kb - kb0) << 16) | (first ? (int)0x80000000u : 0);
+          outpos[e0 + s] = item_of[(size_t)ti * nkb + kb];
           first = false;
           ++s;
         }
       }
-      for (; s < ni; ++s)  // padding: skipped by the kernel (the zero tile, block 0)
-        items[g0 + s] = ntile | (1 << 29) | (s == 0 ? (int)0x80000000u : 0);
+      for (; s < ni; ++s)  // padding: skipped by the kernel
+        items[e0 + s] = pad | (s == 0 ? (int)0x80000000u : 0);
     }
+    seg0 += (size_t)g.npass * W * ni;
   }
-  if (!ntri)
-    for (int b = 0; b < W; ++b)
-      for (int s = 0; s < ni; ++s)
-        items[b * ni + s] = ntile | (1 << 29) | (s == 0 ? (int)0x80000000u : 0);
-  // pair rows (padding rows point at the zero row of each set)
-  std::vector<int> prs((size_t)(ntile + 1) * 16, nshell | (nshell << 16));
+  // pair rows in absolute shell indices, 0xffff for padding rows (the
+  // kernel's zero row); the last tile is all padding
+  std::vector<int> prs((size_t)(ntile + 1) * 16, 0xffff | (0xffff << 16));
   bsk_pairs* q = new bsk_pairs();
   q->slot.resize(ntri);
-  int npair = 0, nitem = 0;
+  int npair = 0;
   for (int L = 0; L < BSK_LEVELS; ++L) {
     const PairTiles& tl = lp[L].tiles;
     const int nt = (int)tl.tile_mask.size();
-    const int zr = lp[L].nsh | (lp[L].nsh << 16);  // the window's zero rows
-    for (int ti = 0; ti < nt; ++ti)
-      std::fill(prs.begin() + (size_t)(tile0[L] + ti) * 16,
-                prs.begin() + (size_t)(tile0[L] + ti + 1) * 16, zr);
     std::vector<std::pair<int, int>> where(tl.pairs.size());  // (global tile, row)
     for (int ti = 0; ti < nt; ++ti) {
       const int p0 = tl.tile_of_pair_start[ti];
@@ -1017,7 +1085,6 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
       q->slot[lp[L].tri[t]] = g * 256 + row * 16 + (c % 16) + (L << 28);
     }
     npair += (int)tl.pairs.size();
-    nitem += tl.nitem;
   }
   q->mode = mode;
   q->nshell = nshell;
@@ -1027,15 +1094,19 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
   q->ntile = ntile;
   q->nitem = nitem;
   q->nkb = nkb;
-  q->ni = ni;
+  q->ni = maxni;
   q->npass = npass;
-  std::copy(lpass, lpass + BSK_LEVELS, q->lpass);
-  for (int L = 0; L < BSK_LEVELS; ++L) {
-    q->lwin[L][0] = ntri ? q_win[L][0] : 0;
-    q->lwin[L][1] = ntri ? q_win[L][1] : nshell;
+  q->nseg = (int)sp.size();
+  for (int g = 0; g < q->nseg; ++g) {
+    q->seg_parts[g] = segs[g].parts;
+    q->seg_win[g][0] = sp[g].lo;
+    q->seg_win[g][1] = sp[g].nsh;
+    q->seg_pass[g] = sp[g].npass;
+    q->seg_ni[g] = sp[g].ni;
   }
-  q->item_total = npass * W * ni;
+  q->item_total = std::max(nitem, 1);
   q->h_items.swap(items);
+  q->h_outpos.swap(outpos);
   q->h_pairs.swap(prs);
   *out = q;
   return BSK_OK;
@@ -1058,24 +1129,21 @@ int bsk_pairs_create_levels(bsk_pairs** out, int mode, int nshell, int ntri, con
   return pairs_create(out, mode, nshell, ntri, tri, level, "bsk_pairs_create_levels");
 }
 
-int bsk_pairs_level_windows(const bsk_pairs* q, int* first, int* count) {
-  if (!q || !first || !count) {
-    bsk_set_error("bsk_pairs_level_windows: invalid argument");
+int bsk_pairs_segments(const bsk_pairs* q, int* nseg, int* parts, int* first, int* count,
+                       int* npass, int* items_per_wave) {
+  if (!q || !nseg || !parts || !first || !count || !npass || !items_per_wave) {
+    bsk_set_error("bsk_pairs_segments: invalid argument");
     return BSK_EINVAL;
   }
-  for (int L = 0; L < BSK_LEVELS; ++L) {
-    first[L] = q->lwin[L][0];
-    count[L] = q->lwin[L][1];
+  *nseg = q->nseg;
+  for (int g = 0; g < kParts; ++g) {
+    const bool in = g < q->nseg;
+    parts[g] = in ? q->seg_parts[g] : 0;
+    first[g] = in ? q->seg_win[g][0] : 0;
+    count[g] = in ? q->seg_win[g][1] : 0;
+    npass[g] = in ? q->seg_pass[g] : 0;
+    items_per_wave[g] = in ? q->seg_ni[g] : 0;
   }
-  return BSK_OK;
-}
-
-int bsk_pairs_level_passes(const bsk_pairs* q, int* npass) {
-  if (!q || !npass) {
-    bsk_set_error("bsk_pairs_level_passes: invalid argument");
-    return BSK_EINVAL;
-  }
-  std::copy(q->lpass, q->lpass + BSK_LEVELS, npass);
   return BSK_OK;
 }
 
@@ -1086,13 +1154,15 @@ int bsk_pairs_upload(bsk_pairs* q) {
   }
   if (q->d_items) return BSK_OK;
   int rc = upload(&q->d_items, q->h_items);
+  if (!rc) rc = upload(&q->d_outpos, q->h_outpos);
   if (!rc) rc = upload(&q->d_pairs, q->h_pairs);
   if (!rc) rc = upload(&q->d_slot, q->slot);
   if (rc) {
     (void)hipFree(q->d_items);
+    (void)hipFree(q->d_outpos);
     (void)hipFree(q->d_pairs);
     (void)hipFree(q->d_slot);
-    q->d_items = q->d_pairs = q->d_slot = nullptr;
+    q->d_items = q->d_outpos = q->d_pairs = q->d_slot = nullptr;
   }
   return rc;
 }
@@ -1100,6 +1170,7 @@ int bsk_pairs_upload(bsk_pairs* q) {
 int bsk_pairs_destroy(bsk_pairs* q) {
   if (!q) return BSK_OK;
   (void)hipFree(q->d_items);
+  (void)hipFree(q->d_outpos);
   (void)hipFree(q->d_pairs);
   (void)hipFree(q->d_slot);
   delete q;
@@ -1129,12 +1200,13 @@ int bsk_pairs_slots(const bsk_pairs* q, int* slot_host) {
   return BSK_OK;
 }
 
-int bsk_pairs_tables(const bsk_pairs* q, int* items, int* pairs) {
+int bsk_pairs_tables(const bsk_pairs* q, int* items, int* outpos, int* pairs) {
   if (!q) {
     bsk_set_error("bsk_pairs_tables: null plan");
     return BSK_EINVAL;
   }
   if (items) std::copy(q->h_items.begin(), q->h_items.end(), items);
+  if (outpos) std::copy(q->h_outpos.begin(), q->h_outpos.end(), outpos);
   if (pairs) std::copy(q->h_pairs.begin(), q->h_pairs.end(), pairs);
   return BSK_OK;
 }
@@ -1159,26 +1231,33 @@ int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
     bsk_set_error("%s: invalid argument", fn);
     return BSK_EINVAL;
   }
-  long long off[BSK_LEVELS] = {0}, cells[BSK_LEVELS] = {ncell};
-  for (int L = 1; L < BSK_LEVELS; ++L) {
-    if (!q->lpass[L]) continue;
-    if (!level_off || !level_ncell) {
+  // cell range of each layout part: P1 = level 3, P0 = the rest of level 1
+  // before it, P2 = the rest of level 2 after it, P3 = the rest of level 0
+  long long pbeg[kParts], pend[kParts];
+  if (level_off && level_ncell) {
+    const long long* o = level_off;
+    const long long* n = level_ncell;
+    pbeg[0] = o[1], pend[0] = o[3];
+    pbeg[1] = o[3], pend[1] = o[3] + n[3];
+    pbeg[2] = o[3] + n[3], pend[2] = o[2] + n[2];
+    pbeg[3] = o[2] + n[2], pend[3] = o[0] + n[0];
+    bool ok = o[0] == o[1] && o[2] == o[3] && o[0] <= o[3] && o[0] + n[0] <= shell_stride;
+    for (int p = 0; p < kParts; ++p)
+      ok = ok && pbeg[p] <= pend[p] && pbeg[p] % kPairChunk == 0 && pend[p] % kPairChunk == 0;
+    if (!ok) {
+      bsk_set_error("%s: level ranges do not nest as the level layout's (bsk_plan_levels)", fn);
+      return BSK_EINVAL;
+    }
+  } else {
+    if (q->nseg != 1 || q->seg_parts[0] != 0xf) {
       bsk_set_error("%s: the plan has sublattice-level triangles; use bsk_triples_mfma_levels "
                     "with the bsk_plan_levels layout", fn);
       return BSK_EINVAL;
     }
+    // one segment of all parts: the whole C-order range
+    std::fill(pbeg, pbeg + kParts, 0LL);
+    std::fill(pend, pend + kParts, ncell);
   }
-  if (level_off && level_ncell)
-    for (int L = 0; L < BSK_LEVELS; ++L) {
-      off[L] = level_off[L];
-      cells[L] = level_ncell[L];
-      if (q->lpass[L] && (cells[L] <= 0 || cells[L] % kPairChunk || off[L] < 0 ||
-                          shell_stride < off[L] + cells[L])) {
-        bsk_set_error("%s: level %d range [%lld, +%lld) does not fit the shell stride %lld "
-                      "in 128-cell chunks", fn, L, off[L], cells[L], shell_stride);
-        return BSK_EINVAL;
-      }
-    }
   if (q->ntri == 0) return BSK_OK;
   if (!q->d_items) {
     bsk_set_error("%s: pair plan not uploaded (bsk_pairs_upload)", fn);
@@ -1194,23 +1273,32 @@ int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
   a.partial = reinterpret_cast<float*>(work);
   a.item_total = q->item_total;
   hipStream_t s = as_stream(stream);
-  for (int L = 0, p = 0; L < BSK_LEVELS; ++L)
-    for (int k = 0; k < q->lpass[L]; ++k, ++p) {
-      // the cells of level L (one contiguous range of every shell row) of
-      // the shells of its window
-      const long long base = off[L] + (long long)q->lwin[L][0] * shell_stride;
-      for (int r = 0; r < 3; ++r) {
-        a.hi[r] = hi[r] + base;
-        a.lo[r] = lo[r] + base;
-        a.nsh[r] = q->lwin[L][1];
-      }
-      a.nkb = (q->lwin[L][1] + 15) / 16;
-      a.ncell = cells[L];
-      a.item0 = p * kPairWaves * q->ni;
-      a.items = q->d_items + a.item0;
-      int rc = bsk_triple_pairs_launch(a, q->sets, q->ni, s);
+  size_t e0 = 0;  // first list entry of the launch
+  for (int g = 0, p = 0; g < q->nseg; ++g) {
+    // the segment's cells (its parts are consecutive) of its shell window
+    const int fp = __builtin_ctz(q->seg_parts[g]), lp = 31 - __builtin_clz(q->seg_parts[g]);
+    const long long c0 = pbeg[fp], c1 = pend[lp];
+    const int w0 = q->seg_win[g][0], wn = q->seg_win[g][1];
+    const long long base = c0 + (long long)w0 * shell_stride;
+    for (int r = 0; r < 3; ++r) {
+      a.hi[r] = hi[r] + base;
+      a.lo[r] = lo[r] + base;
+      a.nsh[r] = wn;
+    }
+    a.lo_row = w0;
+    a.koff = w0 % 16;
+    a.nkb = (a.koff + wn + 15) / 16;
+    a.ncell = c1 - c0;
+    a.accumulate = g > 0;
+    const int ni = q->seg_ni[g];
+    for (int k = 0; k < q->seg_pass[g]; ++k, ++p, e0 += (size_t)kPairWaves * ni) {
+      a.items = q->d_items + e0;
+      a.outpos = q->d_outpos + e0;
+      if (a.ncell == 0) continue;
+      int rc = bsk_triple_pairs_launch(a, q->sets, ni, s);
       if (rc) return rc;
     }
+  }
   if (!out_dev) return BSK_OK;  // partials only; bsk_triples_mfma_reduce finishes
   return bsk_reduce_mfma_launch(a.partial, nwg, q->item_total, q->ntri, q->d_slot, scale,
                                 out_dev, s);
@@ -1244,17 +1332,17 @@ namespace {
 const char* zfused_gap(const bsk_plan* p, const bsk_pairs* q) {
   if (!p || !q) return "null plan";
   if (q->mode != BSK_MODE_AUTO) return "auto spectra only";
-  for (int L = 1; L < BSK_LEVELS; ++L)
-    if (q->lpass[L]) return "the plan has sublattice-level triangles";
-  const int nsh = q->lwin[0][1];  // shells staged: the plan's window
+  if (q->nseg != 1 || q->seg_parts[0] != 0xf) return "the plan has sublattice-level triangles";
+  const int nsh = q->seg_win[0][1];  // shells staged: the plan's window
   if (nsh > kZMaxShells) return "too many shells";
-  if (q->ni < 1 || q->ni > 4) return "items per wave above 4";
+  if (q->seg_ni[0] < 1 || q->seg_ni[0] > 4) return "items per wave above 4";
   bool size = false;
 #define C(n) size |= p->S == n;
   BSK_ZFUSED_SIZES(C)
 #undef C
   if (!size) return "no kernel for this shell mesh";
-  if (bsk_zrows_lds_bytes(p->S, nsh, (nsh + 15) / 16) > 160 * 1024) return "LDS";
+  if (bsk_zrows_lds_bytes(p->S, nsh, (q->seg_win[0][0] % 16 + nsh + 15) / 16) > 160 * 1024)
+    return "LDS";
   return nullptr;
 }
 }  // namespace
@@ -1288,7 +1376,7 @@ int bsk_triples_zfused(bsk_plan* p, const bsk_pairs* q, const void* shell_work, 
   const int S = p->S;
   ZArgs a;
   // the plan's shell window [lo, lo + nsh) of the shells of the workspace
-  const int lo = q->lwin[0][0], nsh = q->lwin[0][1];
+  const int lo = q->seg_win[0][0], nsh = q->seg_win[0][1];
   a.d2 = reinterpret_cast<const cplx<float>*>(shell_work) +
          (size_t)q->nshell * g.MY * S * g.MZP +  // after D1 (shells_t layout)
          (size_t)lo * S * S * g.MZP;
@@ -1298,7 +1386,9 @@ int bsk_triples_zfused(bsk_plan* p, const bsk_pairs* q, const void* shell_work, 
   a.n2hi = p->d_hi;
   a.shell0 = shell0 + lo;
   a.nsh = nsh;
-  a.nkb = (nsh + 15) / 16;
+  a.lo_row = lo;
+  a.koff = lo % 16;
+  a.nkb = (a.koff + nsh + 15) / 16;
   a.nwg = nwg;
   a.pairs = q->d_pairs;
   a.partial = reinterpret_cast<float*>(work);
@@ -1306,10 +1396,11 @@ int bsk_triples_zfused(bsk_plan* p, const bsk_pairs* q, const void* shell_work, 
   a.tw = shell_tw(p, 0.f);
   hipStream_t s = as_stream(stream);
   for (int pass = 0; pass < q->npass; ++pass) {
-    a.item0 = pass * kPairWaves * q->ni;
-    a.items = q->d_items + a.item0;
-    const int rc = p->variant[5] ? bsk_triple_zrows_launch_v1(a, S, q->ni, s)
-                                 : bsk_triple_zrows_launch_v0(a, S, q->ni, s);
+    const size_t e0 = (size_t)pass * kPairWaves * q->seg_ni[0];
+    a.items = q->d_items + e0;
+    a.outpos = q->d_outpos + e0;
+    const int rc = p->variant[5] ? bsk_triple_zrows_launch_v1(a, S, q->seg_ni[0], s)
+                                 : bsk_triple_zrows_launch_v0(a, S, q->seg_ni[0], s);
     if (rc) return rc;
   }
   if (!out_dev) return BSK_OK;

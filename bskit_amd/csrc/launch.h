@@ -63,18 +63,21 @@ struct PairArgs {
   long long stride;       // elements between consecutive shells of a set
   int nsh[3];             // shells per role set
   int nkb;                // 16-shell blocks of the third side staged in LDS
+  int lo_row;             // shell of staged row 0 (pair rows in the table are absolute)
+  int koff;               // third-side LDS row of staged shell 0 (rows below are zero)
   long long ncell;        // cells per shell (multiple of kPairChunk)
   int nwg;                // workgroups (cell ranges)
   const int* items;       // [kPairWaves * NI] item descriptors of this launch
+  const int* outpos;      // [kPairWaves * NI] partial position of each item (< 0: padding)
+  int accumulate;         // add to the partials an earlier launch stored
   const int* pairs;       // [tiles + 1][16]: (row in set 0) | (row in set 1) << 16
   float* partial;         // [nwg][item_total][256]
-  int item0;              // first item of this launch in the partial layout
   int item_total;         // items per workgroup in the partial layout
 };
 int bsk_pair_items_round(int ni);  // compiled items-per-wave count >= ni (0: too many)
 int bsk_pair_item_cap(int sets, int nshell);  // items per wave that compile without spills
 int bsk_pair_prefetch(int nshell);  // prefetch segments per thread (0: too many shells)
-size_t bsk_pair_lds_bytes(int sets, const int nsh[3], int nkb);
+size_t bsk_pair_lds_bytes(int sets, const int nsh[3], int nkb);  // nkb covers koff + nsh[2]
 int bsk_triple_pairs_launch(const PairArgs& a, int sets, int ni, hipStream_t s);
 int bsk_reduce_mfma_launch(const float* partial, int nwg, int items_per_wg, int ntri,
                            const int* slot_dev, double scale, double* out, hipStream_t s);
@@ -91,11 +94,14 @@ struct ZArgs {
   const int* n2hi;        // shell limits (device), from shell0
   int shell0;
   int nsh, nkb;
+  int lo_row;             // shell of row 0 (pair rows in the table are absolute)
+  int koff;               // third-side LDS row of row 0
   int nwg;
   const int* items;       // [kPairWaves * NI] item descriptors of this launch
+  const int* outpos;      // [kPairWaves * NI] partial position of each item (< 0: padding)
   const int* pairs;       // [tiles + 1][16]
   float* partial;         // [nwg][item_total][256]
-  int item0, item_total;
+  int item_total;
   const cplx<float>* tw;  // S twiddles exp(-2 pi i t / S)
 };
 size_t bsk_zrows_lds_bytes(int S, int nsh, int nkb);

@@ -119,9 +119,12 @@ BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l) {
 // from set 2 (<ABC>).
 // Waves per SIMD the register allocation must allow: 4 (two workgroups per
 // CU, at most 128 VGPRs) where the item, prefetch and role-set registers fit
-// without spilling, else 2.
+// without spilling, else 2; 8 for one item and one prefetch segment (at most
+// 64 VGPRs), whose launches stage few shells per chunk and need more
+// workgroups in flight.
 template <int NI, int PV, int SETS>
 constexpr int pair_occupancy() {
+  if (NI == 1 && PV == 1 && SETS == 1) return 8;  // thin launches: 4 workgroups per CU
   constexpr int ni_max = SETS == 1 ? 4 : (SETS == 2 ? 3 : 1);
   return (NI <= ni_max && PV <= 2) ? 4 : 2;
 }
@@ -142,19 +145,26 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
 
-  // zero rows: FP32 padding rows and third-side rows beyond n2
+  // zero rows: FP32 padding rows and the third-side rows outside the staged
+  // shells [koff, koff + n2)
   for (int e = threadIdx.x; e < kLDF; e += kNT) {
     fA[n0 * kLDF + e] = 0.f;
     if (SETS == 3) fB[n1 * kLDF + e] = 0.f;
   }
-  for (int e = threadIdx.x; e < (krows - n2) * kLDH; e += kNT) {
-    hK[n2 * kLDH + e] = 0;
-    lK[n2 * kLDH + e] = 0;
+  for (int e = threadIdx.x; e < krows * kLDH; e += kNT) {
+    const int row = e / kLDH;
+    if (row < a.koff || row >= a.koff + n2) {
+      hK[e] = 0;
+      lK[e] = 0;
+    }
   }
+  uint16_t* const hKs = hK + a.koff * kLDH;  // row of staged third-side shell 0
+  uint16_t* const lKs = lK + a.koff * kLDH;
 
-  // item descriptors of this wave and the LDS rows of this lane's pair
-  int desc[NI], oij[NI];
-  setup<NI, kLDF>(desc, oij, a.items, a.pairs, wave, col);
+  // item descriptors and partial positions of this wave and the LDS rows of
+  // this lane's pair
+  int desc[NI], oij[NI], opos[NI];
+  setup<NI, kLDF>(desc, oij, opos, a.items, a.outpos, a.pairs, wave, col, a.lo_row, n0);
   State<NI> st;
   init(st);
 
@@ -169,9 +179,9 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
   }
   for (long long ch = ch0; ch < ch1; ++ch) {
     __syncthreads();  // every wave is done with the previous chunk
-    put(p0, n0, fA, SETS == 1 ? hK : nullptr, SETS == 1 ? lK : nullptr);
+    put(p0, n0, fA, SETS == 1 ? hKs : nullptr, SETS == 1 ? lKs : nullptr);
     if (SETS == 3) put(p1, n1, fB, nullptr, nullptr);
-    if (SETS >= 2) put(p2, n2, nullptr, hK, lK);
+    if (SETS >= 2) put(p2, n2, nullptr, hKs, lKs);
     __syncthreads();
     if (ch + 1 < ch1) {  // next chunk's HBM loads overlap this chunk's math
       const long long cb = (ch + 1) * kCK;
@@ -184,8 +194,8 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
       kstep<NI, kLDH>(st, desc, oij, fA, fB, hK, lK, ks * 32 + grp * 4, ks * 32 + grp * 8, col,
                       kc);
   }
-  store(st, a.partial + ((long long)blockIdx.x * a.item_total + a.item0 + wave * NI) * 256, grp,
-        col);
+  store_at(st, a.partial + (long long)blockIdx.x * a.item_total * 256, opos, a.accumulate != 0,
+           grp, col);
 }
 
 __global__ __launch_bounds__(64) void k_reduce_mfma(const float* __restrict__ partial, int nwg,

@@ -75,17 +75,22 @@ BSK_DEV void init(State<NI>& st) {
   st.since_flush = 0;
 }
 
-// Descriptors of wave `wave` and, per item, the LDS offsets (floats, both
-// < 2^16) of this lane's pair rows in FP32 rows of LDF floats.
+// Descriptors and partial positions of wave `wave` and, per item, the LDS
+// offsets (floats, both < 2^16) of this lane's pair rows in FP32 rows of LDF
+// floats: table rows are shells, staged at row shell - lo; 0xffff (padding)
+// is the zero row zrow.
 template <int NI, int LDF>
-BSK_DEV void setup(int (&desc)[NI], int (&oij)[NI], const int* items, const int* pairs, int wave,
-                   int col) {
+BSK_DEV void setup(int (&desc)[NI], int (&oij)[NI], int (&opos)[NI], const int* items,
+                   const int* outpos, const int* pairs, int wave, int col, int lo, int zrow) {
 #pragma unroll
   for (int s = 0; s < NI; ++s) {
     desc[s] = __builtin_amdgcn_readfirstlane(items[wave * NI + s]);
+    opos[s] = __builtin_amdgcn_readfirstlane(outpos[wave * NI + s]);
     const int tile = desc[s] & 0xffff;
     const int pr = pairs[tile * 16 + col];
-    oij[s] = ((pr & 0xffff) * LDF) | (((pr >> 16) * LDF) << 16);
+    const int ra = pr & 0xffff, rb = (pr >> 16) & 0xffff;
+    const int ia = ra == 0xffff ? zrow : ra - lo, ib = rb == 0xffff ? zrow : rb - lo;
+    oij[s] = (ia * LDF) | ((ib * LDF) << 16);
   }
 }
 
@@ -143,6 +148,25 @@ BSK_DEV void store(State<NI>& st, float* out, int grp, int col) {
       const double v = st.acc2[s][r] + (double)st.acc[s][r];
       out[s * 256 + (4 * grp + r) * 16 + col] = (float)v;
     }
+}
+
+// The same tiles at item positions pos[s] of the workgroup's partials
+// (out + 256 pos[s]; pos < 0: padding, nothing stored), written or added to
+// what an earlier launch over other cells stored there.
+template <int NI>
+BSK_DEV void store_at(State<NI>& st, float* out, const int (&pos)[NI], bool add, int grp,
+                      int col) {
+#pragma unroll
+  for (int s = 0; s < NI; ++s) {
+    if (pos[s] < 0) continue;  // uniform
+    float* o = out + (long long)pos[s] * 256;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double v = st.acc2[s][r] + (double)st.acc[s][r];
+      float& d = o[(4 * grp + r) * 16 + col];
+      d = add ? (float)((double)d + v) : (float)v;
+    }
+  }
 }
 
 }  // namespace pairmma

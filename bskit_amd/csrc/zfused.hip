@@ -79,8 +79,8 @@ __global__ __launch_bounds__(kZT, 4) void k_triple_zrows(ZArgs a) {
   for (int e = tid; e < S; e += kZT) tw[e] = a.tw[e];
   for (int e = tid; e < nsh; e += kZT) mzs[e] = shell_mz(a.MZ, a.n2hi[a.shell0 + e]);
 
-  int desc[NI], oij[NI];
-  setup<NI, G::LDA>(desc, oij, a.items, a.pairs, wave, col);
+  int desc[NI], oij[NI], opos[NI];
+  setup<NI, G::LDA>(desc, oij, opos, a.items, a.outpos, a.pairs, wave, col, a.lo_row, nsh);
   State<NI> st;
   init(st);
 
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kZT, 4) void k_triple_zrows(ZArgs a) {
         uint32_t h0, l0, h1, l1;
         split2(v.x, v.y, h0, l0, kc);
         split2(v.z, v.w, h1, l1, kc);
-        const int pos = b * G::LDH + permuted_pos(c);
+        const int pos = (b + a.koff) * G::LDH + permuted_pos(c);  // third-side row of shell b
         *reinterpret_cast<uint2*>(hK + pos) = make_uint2(h0, h1);
         *reinterpret_cast<uint2*>(lK + pos) = make_uint2(l0, l1);
       }
@@ -144,8 +144,7 @@ __global__ __launch_bounds__(kZT, 4) void k_triple_zrows(ZArgs a) {
       kstep<NI, G::LDH>(st, desc, oij, fA, fA, hK, lK, ks * 32 + grp * 4, ks * 32 + grp * 8,
                         col, kc);
   }
-  store(st, a.partial + ((long long)blockIdx.x * a.item_total + a.item0 + wave * NI) * 256, grp,
-        col);
+  store_at(st, a.partial + (long long)blockIdx.x * a.item_total * 256, opos, false, grp, col);
 }
 
 template <int NI, int S>

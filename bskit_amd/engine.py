@@ -97,6 +97,11 @@ class LevelLayout:
         return isinstance(other, LevelLayout) and (self.S, self.off, self.ncell, self.stride) == (
             other.S, other.off, other.ncell, other.stride)
 
+    def part_cells(self):
+        """Padded cell counts of the parts C0 minus E, E, O, C1 minus O."""
+        o, n = self.off, self.ncell
+        return (o[3] - o[1], n[3], o[2] + n[2] - o[3] - n[3], o[0] + n[0] - o[2] - n[2])
+
     def c_order_index(self, device):
         """Row position of every cell in C order (x, y, z), for inspection."""
         if self._perm is None or self._perm.device != torch.device(device):
@@ -171,12 +176,13 @@ class PairPlan:
             L.call('bsk_pairs_create_levels', ctypes.byref(h), int(mode), int(nshell), len(tri),
                    tri.ctypes.data_as(ip), levels.ctypes.data_as(ip))
         self.handle = h
-        lp = (ctypes.c_int * L.LEVELS)()
-        L.call('bsk_pairs_level_passes', h, lp)
-        self.level_passes = list(lp)  # launches per sublattice level
-        w0, wn = (ctypes.c_int * L.LEVELS)(), (ctypes.c_int * L.LEVELS)()
-        L.call('bsk_pairs_level_windows', h, w0, wn)
-        self.level_windows = list(zip(w0, wn))  # (first shell, count) staged per level
+        ns = ctypes.c_int()
+        arr = [(ctypes.c_int * 4)() for _ in range(5)]
+        L.call('bsk_pairs_segments', h, ctypes.byref(ns), *arr)
+        # launch segments: (layout parts bitmask, first shell, shells staged,
+        # launches, items per wave)
+        self.segments = [tuple(a[g] for a in arr) for g in range(ns.value)]
+        self.list_entries = sum(8 * npass * ni for _, _, _, npass, ni in self.segments)
         if upload:
             L.call('bsk_pairs_upload', h)
         self.ntri = len(tri)
@@ -185,12 +191,14 @@ class PairPlan:
         self.npair, self.ntile, self.nitem, self.items_per_wave, self.npass = [x.value for x in v]
 
     def tables(self):
-        """(items, pair rows) host tables of the launch."""
-        items = np.zeros(self.npass * 8 * self.items_per_wave, dtype=np.int32)
+        """(items, partial positions, pair rows) host tables of the launches."""
+        items = np.zeros(self.list_entries, dtype=np.int32)
+        outpos = np.zeros(self.list_entries, dtype=np.int32)
         pairs = np.zeros((self.ntile + 1) * 16, dtype=np.int32)
         ip = ctypes.POINTER(ctypes.c_int)
-        L.call('bsk_pairs_tables', self.handle, items.ctypes.data_as(ip), pairs.ctypes.data_as(ip))
-        return items, pairs
+        L.call('bsk_pairs_tables', self.handle, items.ctypes.data_as(ip),
+               outpos.ctypes.data_as(ip), pairs.ctypes.data_as(ip))
+        return items, outpos, pairs
 
     def slots(self):
         out = np.zeros(self.ntri, dtype=np.int32)

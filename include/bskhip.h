@@ -226,25 +226,28 @@ int bsk_pairs_info(const bsk_pairs* pairs, int* npair, int* ntile, int* nitem,
                    int* items_per_wave, int* npass);
 /* Plan over sublattice levels: triangle t is summed over the cells of
  * level[t] (bsk_plan_triangle_levels) and scaled by 2^level[t]; run it with
- * bsk_triples_mfma_levels.  npass[L] (BSK_LEVELS entries): launches over
- * level L. */
+ * bsk_triples_mfma_levels.  The launches run over segments of the level
+ * layout: runs of consecutive parts (P0..P3) holding the same levels of the
+ * plan, each with the items of those levels, the segment holding E (every
+ * item) first; later segments add to its partials.  bsk_pairs_segments:
+ * *nseg segments in launch order, per segment the parts (bit p), the shell
+ * window its launches stage (shells [first, first + count)), its launches
+ * and their items per wave (arrays of 4 entries).  A plan without levels (all level 0) is
+ * one segment of all parts. */
 int bsk_pairs_create_levels(bsk_pairs** pairs, int mode, int nshell, int ntri,
                             const int* tri_host, const int* level);
-int bsk_pairs_level_passes(const bsk_pairs* pairs, int* npass);
-/* Shell window of each level: its passes stage only shells [first[L],
- * first[L] + count[L]) (the smallest range holding its triangles' shells);
- * the pair rows and third-side blocks of its tiles count from first[L], and
- * row count[L] is the zero row. */
-int bsk_pairs_level_windows(const bsk_pairs* pairs, int* first, int* count);
+int bsk_pairs_segments(const bsk_pairs* pairs, int* nseg, int* parts, int* first, int* count,
+                       int* npass, int* items_per_wave);
 /* Partial-buffer slot of each triangle (item * 256 + row * 16 + column,
  * plus level << 28). */
 int bsk_pairs_slots(const bsk_pairs* pairs, int* slot_host);
-/* Host copies of the launch tables, for inspection: items
- * [npass * 8 * items_per_wave] (tile | kb << 16 | padding flag << 29 |
- * new-tile flag << 31) and
- * pair rows [(ntile + 1) * 16] (row in set 0 | row in set 1 << 16; padding
- * rows and the last, all-padding tile point at row nshell, a zero row). */
-int bsk_pairs_tables(const bsk_pairs* pairs, int* items, int* pair_rows);
+/* Host copies of the launch tables, for inspection: items and outpos
+ * (per launch 8 waves x its segment's items per wave: descriptor tile | kb << 16 | padding flag
+ * << 29 | new-tile flag << 31, kb counted from block first/16 of the
+ * launch's segment; partial position of the item, -1 for padding) and pair
+ * rows [(ntile + 1) * 16] (shell of set 0 | shell of set 1 << 16, 0xffff for
+ * padding rows; the last tile is all padding). */
+int bsk_pairs_tables(const bsk_pairs* pairs, int* items, int* outpos, int* pair_rows);
 int bsk_triples_mfma_workspace(const bsk_pairs* pairs, int nwg, size_t* bytes);
 int bsk_triples_mfma(const bsk_pairs* pairs, const uint16_t* hi_i, const uint16_t* lo_i,
                      const uint16_t* hi_j, const uint16_t* lo_j, const uint16_t* hi_k,

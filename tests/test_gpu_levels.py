@@ -195,7 +195,7 @@ def test_level_mfma_matches_one_level_and_fp64(N, nb, width, mode):
     one = [eng.shells(d, fmt=L.FMT_BF16X2, out=eng.alloc_shells(L.FMT_BF16X2, nb, False))
            for d in d32]
     plan = eng.pair_plan(tri, nb, mode, levels=True)
-    assert sum(plan.level_passes[1:]) > 0
+    assert len(plan.segments) > 1
     got2 = eng.triples(lvl, tri, path='mfma', scale=sc).cpu().numpy()
     got1 = eng.triples(one, tri, path='mfma', scale=sc).cpu().numpy()
     s64 = [eng.shells(eng.forward(torch.from_numpy(f).cuda()), fmt=L.FMT_F64,
@@ -207,7 +207,7 @@ def test_level_mfma_matches_one_level_and_fp64(N, nb, width, mode):
                          for i, j, k in tri]).cpu().numpy() * sc
     e21 = (np.abs(got2 - got1) / absum).max()
     e2r = (np.abs(got2 - ref) / absum).max()
-    print('levels N=%d nb=%d mode %d: passes per level %s, windows %s, vs one-level %.2e, '
-          'vs fp64 %.2e' % (N, nb, mode, plan.level_passes, plan.level_windows, e21, e2r))
+    print('levels N=%d nb=%d mode %d: segments (parts, first, count, launches) %s, '
+          'vs one-level %.2e, vs fp64 %.2e' % (N, nb, mode, plan.segments, e21, e2r))
     assert e21 < 5e-5
     assert e2r < 5e-5

@@ -170,7 +170,7 @@ def test_library_loads_and_host_calls_work():
     sl = plan.slots()
     assert sl[0] == sl[1]  # symmetric spectra: any side order maps to one slot
     L.call('bsk_triples_mfma_workspace', plan.handle, 10, ctypes.byref(n))
-    assert n.value == 10 * plan.npass * 8 * plan.items_per_wave * 256 * 4
+    assert n.value == 10 * max(plan.nitem, 1) * 256 * 4  # FP32 partial tiles per workgroup
     for mode, nsh in ((L.MODE_AUTO, 129), (L.MODE_ABC, 97)):
         with pytest.raises(L.BskError):
             PairPlan(np.array([[0, 0, 0]]), nsh, mode, upload=False)
@@ -179,50 +179,85 @@ def test_library_loads_and_host_calls_work():
         PairPlan(np.array([[3, 0, 0]]), 3, L.MODE_AUTO, upload=False)
 
 
-def _emulate_pair_plan(plan, sets, tri, mode, nshell, level_sets=None):
-    """FP64 numpy emulation of the MFMA kernel's tables: per item the 16x16
-    tile sum_x (d_a d_b)(x) d_k(x), read back through the triangle slots.
-    Items of the passes of sublattice level L sum level_sets[L] (the passes
-    of each level follow those of the level before); the slot's bits 28-29
-    (the level) scale by 2^L."""
-    items, prs = plan.tables()
+# parts of the level layout (P0 = C0 minus E, P1 = E, P2 = O, P3 = C1 minus O)
+# in each sublattice level
+_LEVEL_PARTS = {0: (0, 1, 2, 3), 1: (0, 1), 2: (1, 2), 3: (1,)}
+
+
+def _emulate_pair_plan(plan, sets, tri, mode, nshell, part_sets=None):
+    """FP64 numpy emulation of the MFMA kernel's tables: per launch item the
+    16x16 tile sum_x (d_a d_b)(x) d_k(x) over the cells of its segment, stored
+    at its partial position by the first segment and added by the others,
+    read back through the triangle slots (scaled by 2^level).  part_sets[p]:
+    the role fields on the cells of layout part p; without it the plan must be
+    one segment of all parts, summed over `sets`."""
+    items, outpos, prs = plan.tables()
     # the kernel re-forms pair products only at flagged items: an unflagged
     # item continues the previous item's tile in the same wave's list
     # (flag 29: padding)
-    ni = plan.items_per_wave
-    for w in range(len(items) // ni):
-        lst = items[w * ni:(w + 1) * ni]
-        assert lst[0] < 0
-        for s in range(1, ni):
-            if (lst[s] >> 29) & 1:
-                assert all((x >> 29) & 1 for x in lst[s:])  # padding closes the list
-                break
-            if lst[s] >= 0:
-                assert (lst[s] & 0xffff) == (lst[s - 1] & 0xffff)
-    part = np.zeros((len(items), 16, 16))
-    for g, d in enumerate(items):
-        if (d >> 29) & 1:
-            continue  # padding
-        lv = int(np.searchsorted(np.cumsum(plan.level_passes), g // (8 * ni), side='right'))
-        f0, f1, f2 = level_sets[lv] if level_sets is not None else sets
-        # the level's shell window: rows and blocks count from w0, row nw is zero
-        w0, nw = plan.level_windows[lv]
-        assert 0 <= w0 and w0 + nw <= nshell
+    e = 0
+    for _, _, _, npass, ni in plan.segments:
+        for w in range(8 * npass):
+            lst = items[e:e + ni]
+            e += ni
+            assert lst[0] < 0
+            for s in range(1, ni):
+                if (lst[s] >> 29) & 1:
+                    assert all((x >> 29) & 1 for x in lst[s:])  # padding closes the list
+                    break
+                if lst[s] >= 0:
+                    assert (lst[s] & 0xffff) == (lst[s - 1] & 0xffff)
+    assert e == len(items)
+    assert plan.segments[0][0] & 2  # E first
+    part = np.zeros((max(plan.nitem, 1), 16, 16))
+    written = np.zeros(len(part), bool)
+    launch, e0 = 0, 0
+    for g, (parts, w0, wn, npass, ni) in enumerate(plan.segments):
+        assert 0 <= w0 and w0 + wn <= nshell
+        if part_sets is None:
+            assert parts == 0xf and len(plan.segments) == 1
+            f0, f1, f2 = sets
+        else:
+            f0, f1, f2 = (np.concatenate([part_sets[q][r] for q in range(4) if parts >> q & 1],
+                                         axis=1) for r in range(3))
         z = np.zeros_like(f0[0])
-        tile = d & 0xffff
-        kb = (d >> 16) & 0xff
-        for row in range(16):
-            pr = prs[tile * 16 + row]
-            a, b = pr & 0xffff, pr >> 16
-            assert a <= nw and b <= nw
-            pa = (f0[w0 + a] if a < nw else z) * (f1[w0 + b] if b < nw else z)
-            for col in range(16):
-                k = 16 * kb + col
-                if k < nw:
-                    part[g, row, col] = np.sum(pa * f2[w0 + k])
+        for _ in range(npass):
+            for e in range(e0, e0 + 8 * ni):
+                d, pos = items[e], outpos[e]
+                if (d >> 29) & 1:
+                    assert pos == -1
+                    continue  # padding
+                tile = d & 0xffff
+                kb = ((d >> 16) & 0xff) + w0 // 16
+                tsum = np.zeros((16, 16))
+                for row in range(16):
+                    pr = prs[tile * 16 + row]
+                    a, b = pr & 0xffff, (pr >> 16) & 0xffff
+                    for v in (a, b):
+                        assert v == 0xffff or w0 <= v < w0 + wn  # staged in this segment
+                    pa = (f0[a] if a != 0xffff else z) * (f1[b] if b != 0xffff else z)
+                    for col in range(16):
+                        k = 16 * kb + col
+                        if w0 <= k < w0 + wn:  # other third-side rows are zero
+                            tsum[row, col] = np.sum(pa * f2[k])
+                if g == 0:
+                    assert not written[pos]
+                    part[pos], written[pos] = tsum, True
+                else:
+                    assert written[pos]
+                    part[pos] += tsum
+            launch += 1
+            e0 += 8 * ni
+    assert launch == plan.npass
     flat = part.reshape(-1)
     slots = plan.slots()
     return flat[slots & 0x0fffffff] * 2.0 ** ((slots >> 28) & 3)
+
+
+def _level_reference(part_sets, tri, levels):
+    return np.array([2.0 ** lv * sum(np.sum(part_sets[p][0][i] * part_sets[p][1][j] *
+                                            part_sets[p][2][k]) for p in _LEVEL_PARTS[lv])
+                     for (i, j, k), lv in zip(tri, levels)])
 
 
 @pytest.mark.parametrize('mode,nshell,ntri,seed', [(0, 7, 40, 1), (0, 33, 300, 2), (1, 20, 150, 3),
@@ -252,11 +287,11 @@ def test_pair_plan_tables_compute_every_triangle(mode, nshell, ntri, seed):
 @pytest.mark.parametrize('mode,nshell,ntri,seed,probs', [
     (0, 33, 400, 11, (0.1, 0.3, 0.4, 0.2)), (0, 20, 150, 12, (0, 0, 0, 1)),
     (1, 20, 150, 13, (0.25, 0.25, 0.25, 0.25)), (2, 18, 200, 14, (1, 0, 0, 0)),
-    (0, 40, 600, 15, (0, 0.5, 0, 0.5))])
+    (0, 40, 600, 15, (0, 0.5, 0, 0.5)), (0, 24, 300, 16, (0.5, 0, 0.5, 0))])
 def test_level_pair_plan(mode, nshell, ntri, seed, probs):
-    """A plan over sublattice levels: each triangle is read from the passes
-    of its level (their own fields here) and scaled by 2^level; the passes
-    of each level follow those of the level before."""
+    """A plan over sublattice levels: each triangle's item is summed over
+    the layout parts of its level (their own fields here) through the
+    segment launches and scaled by 2^level."""
     from bskit_amd import _lib as L
     from bskit_amd.engine import PairPlan
     try:
@@ -266,31 +301,37 @@ def test_level_pair_plan(mode, nshell, ntri, seed, probs):
     rng = np.random.default_rng(seed)
     nset = {0: 1, 1: 2, 2: 3}[mode]
     roles = {0: (0, 0, 0), 1: (0, 0, 1), 2: (0, 1, 2)}[mode]
-    per_level = []
-    for lv in range(4):
-        fields = [rng.standard_normal((nshell, 24 >> (lv // 2))) for _ in range(nset)]
-        per_level.append([fields[r] for r in roles])
+    part_sets = []
+    for p in range(4):
+        fields = [rng.standard_normal((nshell, (12, 4, 4, 12)[p])) for _ in range(nset)]
+        part_sets.append([fields[r] for r in roles])
     tri = rng.integers(0, nshell, size=(ntri, 3))
     levels = rng.choice(4, size=ntri, p=probs).astype(np.int32)
     plan = PairPlan(tri, nshell, mode, upload=False, levels=levels)
-    for lv in range(4):
-        assert (plan.level_passes[lv] > 0) == bool((levels == lv).any()), lv
-    got = _emulate_pair_plan(plan, per_level[0], tri, mode, nshell, per_level)
-    ref = np.array([2.0 ** lv * np.sum(per_level[lv][0][i] * per_level[lv][1][j] *
-                                       per_level[lv][2][k])
-                    for (i, j, k), lv in zip(tri, levels)])
+    # segments: maximal runs of parts with the same levels present
+    present = set(levels.tolist())
+    covered = 0
+    for parts, w0, wn, npass, ni in plan.segments:
+        assert ni <= 4
+        lv = [{L_ for L_ in present if p in _LEVEL_PARTS[L_]} for p in range(4) if parts >> p & 1]
+        assert all(x == lv[0] for x in lv) and npass > 0
+        assert not covered & parts
+        covered |= parts
+    got = _emulate_pair_plan(plan, None, tri, mode, nshell, part_sets)
+    ref = _level_reference(part_sets, tri, levels)
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
-    # all triangles at level 0: the plan of bsk_pairs_create
+    # all triangles at level 0: the plan of bsk_pairs_create, one segment
     p0 = PairPlan(tri, nshell, mode, upload=False, levels=np.zeros(ntri, np.int32))
     p1 = PairPlan(tri, nshell, mode, upload=False)
-    assert p0.level_passes == p1.level_passes and p0.level_passes[1:] == [0, 0, 0]
-    assert np.array_equal(p0.slots(), p1.slots())
+    assert len(p1.segments) == 1 and p1.segments[0][0] == 0xf
+    assert p0.segments == p1.segments and np.array_equal(p0.slots(), p1.slots())
 
 
 @pytest.mark.parametrize('mode', [0, 1, 2])
 def test_level_pair_plan_stages_shell_windows(mode):
-    """Each level stages only the shell range its triangles use: the window
-    is the smallest such range and the emulated sums still match."""
+    """Each segment stages only the shell range its levels' triangles use
+    (windows not aligned to 16: third-side rows offset in LDS) and the
+    emulated sums still match."""
     from bskit_amd import _lib as L
     from bskit_amd.engine import PairPlan
     try:
@@ -310,16 +351,16 @@ def test_level_pair_plan_stages_shell_windows(mode):
     levels = np.array(levels, np.int32)
     nset = {0: 1, 1: 2, 2: 3}[mode]
     roles = {0: (0, 0, 0), 1: (0, 0, 1), 2: (0, 1, 2)}[mode]
-    per_level = []
-    for lv in range(4):
-        fields = [rng.standard_normal((nshell, 16)) for _ in range(nset)]
-        per_level.append([fields[r] for r in roles])
+    part_sets = []
+    for p in range(4):
+        fields = [rng.standard_normal((nshell, 8)) for _ in range(nset)]
+        part_sets.append([fields[r] for r in roles])
     plan = PairPlan(tri, nshell, mode, upload=False, levels=levels)
-    assert plan.level_windows == [(a, b - a) for a, b in spans]
-    got = _emulate_pair_plan(plan, per_level[0], tri, mode, nshell, per_level)
-    ref = np.array([2.0 ** lv * np.sum(per_level[lv][0][i] * per_level[lv][1][j] *
-                                       per_level[lv][2][k])
-                    for (i, j, k), lv in zip(tri, levels)])
+    for parts, w0, wn, npass, ni in plan.segments:
+        lv = [L_ for L_ in range(4) if any(parts >> p & 1 for p in _LEVEL_PARTS[L_])]
+        assert (w0, w0 + wn) == (min(spans[L_][0] for L_ in lv), max(spans[L_][1] for L_ in lv))
+    got = _emulate_pair_plan(plan, None, tri, mode, nshell, part_sets)
+    ref = _level_reference(part_sets, tri, levels)
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
 
 
@@ -350,7 +391,7 @@ def test_pair_plan_splits_wide_tiles_over_waves(mode, nshell):
         pytest.skip('library not loadable here: %s' % e)
     tri = np.array([(i, j, k) for i in range(5) for j in range(4) for k in range(0, nshell, 3)])
     plan = PairPlan(tri, nshell, mode, upload=False)
-    items, _ = plan.tables()
+    items, _, _ = plan.tables()
     ni = plan.items_per_wave
     waves = {}
     for g, d in enumerate(items):
