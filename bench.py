@@ -120,6 +120,9 @@ def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
     out = torch.empty(len(wl['tri']), dtype=torch.float64, device='cuda')
     dtype = torch.float64 if fmt == L.FMT_F64 else torch.float32
     field = wl['field'].to(dtype)
+    # level layout: write only the parts of each shell the sums read
+    parts = (eng.shell_parts(wl['tri']) if path == 'mfma' and shells is not None and
+             shells.layout is not None else None)
 
     def step(t):
         if t is not None:
@@ -132,7 +135,7 @@ def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
             eng.bispectrum_zfused(dk, wl['tri'], scale=scale, out=out, kernel_events=kev,
                                   after_shells=(lambda: t.mark('shells')) if t else None)
         else:
-            eng.shells(dk, fmt=fmt, dtype=dtype, out=shells)
+            eng.shells(dk, fmt=fmt, dtype=dtype, out=shells, parts=parts)
             if t is not None:
                 t.mark('shells')
             eng.triples([shells], wl['tri'], path=path, scale=scale, out=out, kernel_events=kev)

@@ -49,6 +49,9 @@ _SIGS = {
     'bsk_shells_workspace': [_vp, _c_int, _c_int, _c_int, _c_size_p],
     'bsk_shells': [_vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp,
                    _vp, _c_ll, _c_ll, _vp, _vp],
+    'bsk_shells_levels': [_vp, _c_int, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp, _c_ll, _vp,
+                          _vp, _vp],
+    'bsk_pairs_shell_parts': [_vp, ctypes.POINTER(ctypes.c_ubyte)],
     'bsk_power_spectrum_workspace': [_vp, _c_size_p],
     'bsk_power_spectrum': [_vp, _c_int, _vp, _vp, _vp, _vp],
     'bsk_triples_plain_workspace': [_c_int, _c_int, _c_size_p],

@@ -36,12 +36,15 @@ class _Pair:
                    for _ in range(2)]
         self.sets = [eng.alloc_shells(fmt, eng.nb) for _ in range(2)]
         self.out = torch.empty(len(tri), dtype=torch.float64, device=eng.device)
+        # level layout: write only the parts of each shell the sums read
+        self.parts = (eng.shell_parts(tri, 2) if path == PATH_MFMA and
+                      self.sets[0].layout is not None else None)
 
     def enqueue(self, work):
         eng = self.eng
         for f, dk, sh in zip(self.fields, self.dk, self.sets):
             eng.forward(f, out=dk, work=work)
-            eng.shells(dk, fmt=self.fmt, out=sh, work=work)
+            eng.shells(dk, fmt=self.fmt, out=sh, work=work, parts=self.parts)
         eng.triples(self.sets, self.tri, path=self.path, scale=self.scale, out=self.out,
                     work=work)
 

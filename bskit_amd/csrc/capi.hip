@@ -144,7 +144,8 @@ int forward_t(bsk_plan* p, const T* field, cplx<T>* dk, void* work, hipStream_t 
 template <typename T>
 int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int x0, int xs,
              int stages, int fmt, void* out, void* out_lo, long long stride, long long cell0,
-             void* work, hipStream_t s, const cplx<T>* tw) {
+             void* work, hipStream_t s, const cplx<T>* tw,
+             const unsigned char* parts = nullptr) {
   const int N = p->S;
   const Geom g = p->g;
   cplx<T>* d1 = reinterpret_cast<cplx<T>*>(work);
@@ -165,7 +166,8 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
   char* o = reinterpret_cast<char*>(out) + (size_t)shell0 * stride * esz;
   char* ol = out_lo ? reinterpret_cast<char*>(out_lo) + (size_t)shell0 * stride * esz : nullptr;
   return bsk_inv_z<T>(N, p->variant[5], d2, o, ol, tw, g.MZ, g.MZP, (long long)xs * N, cell0,
-                      stride, fmt, nsh, p->d_hi, shell0, (stages & BSK_STAGE_LEVELS) ? 1 : 0, s);
+                      stride, fmt, nsh, p->d_hi, shell0, (stages & BSK_STAGE_LEVELS) ? 1 : 0,
+                      parts, s);
 }
 
 const cplx<float>* shell_tw(const bsk_plan* p, float) { return p->tws_f ? p->tws_f : p->tw_f; }
@@ -523,6 +525,34 @@ int bsk_shells(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int
   }
   bsk_set_error("bsk_shells: dtype must be BSK_DT_F32 or BSK_DT_F64");
   return BSK_EINVAL;
+}
+
+int bsk_shells_levels(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int nshell,
+                      int fmt, void* out, void* out_lo, long long shell_stride,
+                      const unsigned char* parts, void* work, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (dtype != BSK_DT_F32 || (fmt != BSK_FMT_F32 && fmt != BSK_FMT_BF16X2)) {
+    if (dtype != BSK_DT_F64 || fmt != BSK_FMT_F64) {
+      bsk_set_error("bsk_shells_levels: FP32 shells as F32 or BF16X2, FP64 shells as F64");
+      return BSK_EINVAL;
+    }
+  }
+  if (nshell <= 0 || shell0 < 0 || shell0 + nshell > p->nbin || nshell > 65535 || !work ||
+      !out || (fmt == BSK_FMT_BF16X2 && !out_lo) || (kind == BSK_KIND_DELTA && !dk) ||
+      kind < BSK_KIND_DELTA || kind > BSK_KIND_KMAG || p->S % 4 ||
+      shell_stride < level_layout(p->S).stride) {
+    bsk_set_error("bsk_shells_levels: invalid argument (S a multiple of 4, shell stride of at "
+                  "least the bsk_plan_levels stride)");
+    return BSK_EINVAL;
+  }
+  const int stages = BSK_STAGE_X | BSK_STAGE_YZ | BSK_STAGE_LEVELS;
+  if (dtype == BSK_DT_F32)
+    return shells_t<float>(p, kind, (const cplx<float>*)dk, shell0, nshell, 0, p->S, stages, fmt,
+                           out, out_lo, shell_stride, 0, work, as_stream(stream),
+                           shell_tw(p, 0.f), parts);
+  return shells_t<double>(p, kind, (const cplx<double>*)dk, shell0, nshell, 0, p->S, stages, fmt,
+                          out, nullptr, shell_stride, 0, work, as_stream(stream),
+                          shell_tw(p, 0.0), parts);
 }
 
 // ---------------------------------------------------------------- triples
@@ -1144,6 +1174,18 @@ int bsk_pairs_segments(const bsk_pairs* q, int* nseg, int* parts, int* first, in
     npass[g] = in ? q->seg_pass[g] : 0;
     items_per_wave[g] = in ? q->seg_ni[g] : 0;
   }
+  return BSK_OK;
+}
+
+int bsk_pairs_shell_parts(const bsk_pairs* q, unsigned char* parts) {
+  if (!q || !parts) {
+    bsk_set_error("bsk_pairs_shell_parts: invalid argument");
+    return BSK_EINVAL;
+  }
+  std::fill(parts, parts + q->nshell, (unsigned char)0);
+  for (int g = 0; g < q->nseg; ++g)
+    for (int b = q->seg_win[g][0]; b < q->seg_win[g][0] + q->seg_win[g][1]; ++b)
+      parts[b] |= (unsigned char)q->seg_parts[g];
   return BSK_OK;
 }
 

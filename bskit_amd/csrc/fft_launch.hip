@@ -177,25 +177,25 @@ namespace {
 template <typename T, int N, int V>
 void inv_z_n(const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ, int MZP,
              long long nrows, long long cell0, long long stride, int fmt, int nsh,
-             const int* n2hi, int shell0, int lv, hipStream_t s) {
+             const int* n2hi, int shell0, int lv, const unsigned char* parts, hipStream_t s) {
   constexpr int NH = N / 2, P = pz<T, NH>();
   hipLaunchKernelGGL((k_inv_z<T, NH, V, P, kW>), dim3(cdiv(nrows, P), nsh), dim3(kW), 0, s, in,
-                     out, out_lo, tw, MZ, MZP, nrows, cell0, stride, fmt, n2hi, shell0, lv);
+                     out, out_lo, tw, MZ, MZP, nrows, cell0, stride, fmt, n2hi, shell0, lv, parts);
 }
 }  // namespace
 template <typename T>
 int bsk_inv_z(int N, int V, const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
               int MZP, long long nrows, long long cell0, long long stride, int fmt, int nsh,
-              const int* n2hi, int shell0, int lv, hipStream_t s) {
-#define C(n) BSK_VCASE(n, (inv_z_n<T, n, 0>(in, out, out_lo, tw, MZ, MZP, nrows, cell0, stride, fmt, nsh, n2hi, shell0, lv, s)), \
-                       (inv_z_n<T, n, 1>(in, out, out_lo, tw, MZ, MZP, nrows, cell0, stride, fmt, nsh, n2hi, shell0, lv, s)))
+              const int* n2hi, int shell0, int lv, const unsigned char* parts, hipStream_t s) {
+#define C(n) BSK_VCASE(n, (inv_z_n<T, n, 0>(in, out, out_lo, tw, MZ, MZP, nrows, cell0, stride, fmt, nsh, n2hi, shell0, lv, parts, s)), \
+                       (inv_z_n<T, n, 1>(in, out, out_lo, tw, MZ, MZP, nrows, cell0, stride, fmt, nsh, n2hi, shell0, lv, parts, s)))
   BSK_SWITCH(T, N, V, C)
 #undef C
 }
 template int bsk_inv_z<float>(int, int, const cplx<float>*, void*, void*, const cplx<float>*, int,
                               int, long long, long long, long long, int, int, const int*, int,
-                              int, hipStream_t);
+                              int, const unsigned char*, hipStream_t);
 template int bsk_inv_z<double>(int, int, const cplx<double>*, void*, void*, const cplx<double>*,
                                int, int, long long, long long, long long, int, int, const int*, int,
-                               int, hipStream_t);
+                               int, const unsigned char*, hipStream_t);
 #endif

@@ -474,13 +474,15 @@ struct ShellLevels {
 //
 // lv != 0 (whole mesh: nrows = N^2, cell0 = 0): each shell is written in the
 // sublattice level layout instead of C order (ShellLevels: a permutation of
-// the cells into four parts, zero-padded).
+// the cells into four parts, zero-padded), only the parts p with bit p of
+// parts[shell] set (all parts without the array).
 template <typename T, int NH, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, void* out,
                                              void* out_lo, const cplx<T>* __restrict__ twg,
                                              int MZ, int MZP, long long nrows, long long cell0,
                                              long long shell_stride, int fmt,
-                                             const int* __restrict__ n2hi, int shell0, int lv) {
+                                             const int* __restrict__ n2hi, int shell0, int lv,
+                                             const unsigned char* __restrict__ parts) {
   constexpr int N = 2 * NH;
   constexpr int LD = pencil_ld<NH>();
   constexpr int KH = NH / 2 + 1;  // k = 0..NH/2 (each with its mirror NH - k)
@@ -502,9 +504,11 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   // uniform base of the level layout: this shell's row
   char* const shi = static_cast<char*>(out) + bl * shell_stride * esz;
   char* const slo = out_lo ? static_cast<char*>(out_lo) + bl * shell_stride * esz : nullptr;
-  if (lv && blockIdx.x == 0)  // zero padding of the four parts
+  const int pm = parts ? parts[bl] : 0xf;  // parts written of this shell (level layout)
+  if (lv && blockIdx.x == 0)  // zero padding of the parts written
 #pragma unroll
     for (int part = 0; part < 4; ++part) {
+      if (!(pm >> part & 1)) continue;
       const long long end = part + 1 < 4 ? Lay::OFF[part + 1] : Lay::OFF[3] + Lay::NC;
       for (long long e = Lay::OFF[part] + Lay::CNT[part] + tid; e < end; e += W)
         store_zero<T>(fmt, shi, slo, (unsigned)e);
@@ -556,14 +560,20 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
       const unsigned pos = (unsigned)(((x >> 1) * NH + (y >> 1)) * NH + 2 * q);
 #pragma unroll
       for (int zp = 0; zp < 2; ++zp) {
+        int part;
         unsigned off;
-        if (((x + y + zp) & 1) == 0)  // C0: E if all even, else P0
-          off = (xy == 0 && zp == 0) ? (unsigned)Lay::OFF[1]
-                                     : (unsigned)(Lay::OFF[0] + (long long)(xy - 1) * Lay::E);
-        else  // C1: O if all odd, else P3
-          off = (xy == 3 && zp == 1) ? (unsigned)Lay::OFF[2]
-                                     : (unsigned)(Lay::OFF[3] + (long long)xy * Lay::E);
-        store_pair<T>(fmt, shi, slo, off + pos, zp ? b[0].y : b[0].x, zp ? b[1].y : b[1].x, sc);
+        if (((x + y + zp) & 1) == 0) {  // C0: E if all even, else P0
+          part = (xy == 0 && zp == 0) ? 1 : 0;
+          off = part ? (unsigned)Lay::OFF[1]
+                     : (unsigned)(Lay::OFF[0] + (long long)(xy - 1) * Lay::E);
+        } else {  // C1: O if all odd, else P3
+          part = (xy == 3 && zp == 1) ? 2 : 3;
+          off = part == 2 ? (unsigned)Lay::OFF[2]
+                          : (unsigned)(Lay::OFF[3] + (long long)xy * Lay::E);
+        }
+        if (pm >> part & 1)
+          store_pair<T>(fmt, shi, slo, off + pos, zp ? b[0].y : b[0].x, zp ? b[1].y : b[1].x,
+                        sc);
       }
     }
   }

@@ -32,7 +32,8 @@ int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, 
 template <typename T>
 int bsk_inv_z(int N, int V, const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
               int MZP, long long nrows, long long cell0, long long stride, int fmt, int nsh,
-              const int* n2hi, int shell0, int lv, hipStream_t s);
+              const int* n2hi, int shell0, int lv, const unsigned char* parts,
+              hipStream_t s);
 
 // Triple products ------------------------------------------------------------
 // Reference-structured FP32/FP64 path: one sweep over all cells per triangle.

@@ -119,6 +119,14 @@ class LevelLayout:
         return self._perm
 
 
+class ShellParts:
+    """Layout parts (bit p) to write per shell: device bytes and a host copy."""
+
+    def __init__(self, host, device):
+        self.host = np.ascontiguousarray(host, dtype=np.uint8)
+        self.dev = torch.from_numpy(self.host.copy()).to(device)
+
+
 class ShellSet:
     """nb shell fields of one input field in one storage format: rows of
     ``stride`` elements holding the S^3 cells in C order, or (``layout``) in
@@ -132,6 +140,7 @@ class ShellSet:
         self.stride = data.shape[1]
         self.ncell = self.stride if ncell is None else int(ncell)
         self.layout = layout
+        self.parts = None  # level layout: parts written per shell (None: all)
 
     @property
     def sum_cells(self):
@@ -199,6 +208,13 @@ class PairPlan:
         L.call('bsk_pairs_tables', self.handle, items.ctypes.data_as(ip),
                outpos.ctypes.data_as(ip), pairs.ctypes.data_as(ip))
         return items, outpos, pairs
+
+    def shell_parts(self, nshell):
+        """Layout parts (bit p) in which the plan's launches stage each shell."""
+        out = np.zeros(nshell, dtype=np.uint8)
+        L.call('bsk_pairs_shell_parts', self.handle,
+               out.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)))
+        return out
 
     def slots(self):
         out = np.zeros(self.ntri, dtype=np.int32)
@@ -334,8 +350,13 @@ class GridEngine:
 
     # ---------------------------------------------------------------- shells
     def shells(self, dk=None, kind=L.KIND_DELTA, fmt=L.FMT_F32, dtype=torch.float32,
-               shell0=0, nshell=None, out=None, stream=None, work=None):
-        """All (or a range of) shell fields of one spectrum on the full S^3 mesh."""
+               shell0=0, nshell=None, out=None, stream=None, work=None, parts=None):
+        """All (or a range of) shell fields of one spectrum on the full S^3 mesh.
+
+        parts (level layout only): the ShellParts of shell_parts(tri), the
+        layout parts each shell must hold for those triangles; MFMA sums over
+        these shells then accept only triangle lists whose plans read no other
+        part."""
         nshell = self.nb - shell0 if nshell is None else nshell
         dt = _dt(dtype)
         if kind == L.KIND_DELTA:
@@ -347,11 +368,36 @@ class GridEngine:
         nbytes = L.size_out()
         L.call('bsk_shells_workspace', self._plan, dt, nshell, self.S, ctypes.byref(nbytes))
         w = (work or self.work).get('shell', nbytes.value)
+        if parts is not None:
+            if out.layout is None:
+                raise ValueError('part masks need shells in the level layout')
+            pm = parts.dev[shell0:shell0 + nshell]
+            L.call('bsk_shells_levels', self._plan, dt, kind, L.ptr(dk), shell0, nshell, fmt,
+                   L.ptr(out.data), L.ptr(out.lo), out.stride, L.ptr(pm), L.ptr(w),
+                   stream_handle(stream))
+            if out.parts is None:
+                out.parts = np.full(out.nb, 0xf, np.uint8)
+            out.parts[shell0:shell0 + nshell] = parts.host[shell0:shell0 + nshell]
+            return out
         stages = L.STAGE_X | L.STAGE_YZ | (L.STAGE_LEVELS if out.layout else 0)
         L.call('bsk_shells', self._plan, dt, kind, L.ptr(dk), shell0, nshell, 0, self.S,
                stages, fmt, L.ptr(out.data), L.ptr(out.lo), out.stride, 0,
                L.ptr(w), stream_handle(stream))
+        if out.parts is not None:
+            out.parts[shell0:shell0 + nshell] = 0xf
         return out
+
+    def shell_parts(self, tri, nfield=1):
+        """ShellParts: the layout parts each shell must hold for the MFMA sums
+        of tri with nfield fields (cached); pass it to shells(..., parts=)."""
+        tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
+        key = (tri.tobytes(), 'parts', int(nfield))
+        t = self._tri_cache.get(key)
+        if t is None:
+            plan = self.pair_plan(tri, self.nb, _MODE[nfield], levels=True)
+            t = ShellParts(plan.shell_parts(self.nb), self.device)
+            self._tri_cache[key] = t
+        return t
 
     def alloc_shells(self, fmt, nb, levels=None):
         """Shell storage for nb shells: C order, or the sublattice level
@@ -511,6 +557,12 @@ class GridEngine:
             if any(x.layout != lay for x in sets):
                 raise ValueError('shell sets of one call need the same layout')
             plan = self.pair_plan(tri, sa.nb, mode, levels=lay is not None)
+            if lay is not None and any(x.parts is not None for x in sets):
+                need = plan.shell_parts(sa.nb)
+                for x in sets:
+                    if x.parts is not None and np.any(need & ~x.parts):
+                        raise ValueError('the shells were written for another triangle list: '
+                                         'parts this plan reads are missing')
             if lay is None:
                 off = (ctypes.c_longlong * L.LEVELS)(0, 0, 0, 0)
                 cells = (ctypes.c_longlong * L.LEVELS)(ncell, 0, 0, 0)

@@ -111,6 +111,8 @@ class Server:
         if job.path == PATH_MFMA and len(tri):
             mode = {1: L.MODE_AUTO, 2: L.MODE_AAB}[job.nfield]
             eng.pair_plan(tri, eng.nb, mode, levels=eng.uses_levels())
+            if eng.uses_levels():
+                eng.shell_parts(tri, job.nfield)
         eng.triangles_device(tri)
         job.footprint = self.footprint(eng, job.nfield)
         if job.footprint > self.budget:
@@ -129,7 +131,10 @@ class Server:
             sets = []
             for f in job.fields[:job.nfield]:
                 dk = eng.forward(f, work=w)
-                sets.append(eng.shells(dk, fmt=fmt, out=eng.alloc_shells(fmt, eng.nb), work=w))
+                sh = eng.alloc_shells(fmt, eng.nb)
+                parts = (eng.shell_parts(job.tri, job.nfield)
+                         if job.path == PATH_MFMA and sh.layout is not None else None)
+                sets.append(eng.shells(dk, fmt=fmt, out=sh, work=w, parts=parts))
             out = eng.triples(sets, job.tri, path=job.path, scale=job.box ** 6 / float(eng.ncell),
                               work=w)
             host = torch.empty(len(job.tri), dtype=torch.float64, pin_memory=True)

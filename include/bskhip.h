@@ -166,6 +166,16 @@ int bsk_shells(bsk_plan* plan, int dtype, int kind, const void* dk, int shell0, 
                int xslab, int stages, int fmt, void* out, void* out_lo, long long shell_stride,
                long long out_cell0, void* work, void* stream);
 
+/* Shells [shell0, shell0 + nshell) over the whole shell mesh in the level
+ * layout (stages X, YZ and LEVELS of bsk_shells), writing of shell b only
+ * the layout parts p with bit p of parts[b - shell0] set (device array of
+ * nshell bytes; NULL: every part).  With the masks of bsk_pairs_shell_parts
+ * for the triangle list summed next, the parts no launch of its plan reads
+ * are not written.  No reference counterpart (see bsk_plan_levels). */
+int bsk_shells_levels(bsk_plan* plan, int dtype, int kind, const void* dk, int shell0, int nshell,
+                      int fmt, void* out, void* out_lo, long long shell_stride,
+                      const unsigned char* parts, void* work, void* stream);
+
 /* Binned power spectrum sums of the plan's bins from the compact delta(k)
  * cube (bsk_forward output): out_dev[3b + 0] = sum w |delta_k|^2,
  * out_dev[3b + 1] = sum w (modes), out_dev[3b + 2] = sum w |k| over the modes
@@ -238,6 +248,9 @@ int bsk_pairs_create_levels(bsk_pairs** pairs, int mode, int nshell, int ntri,
                             const int* tri_host, const int* level);
 int bsk_pairs_segments(const bsk_pairs* pairs, int* nseg, int* parts, int* first, int* count,
                        int* npass, int* items_per_wave);
+/* The layout parts (bit p) in which some launch of the plan stages shell
+ * b: parts[b] for the nshell shells of the plan. */
+int bsk_pairs_shell_parts(const bsk_pairs* pairs, unsigned char* parts);
 /* Partial-buffer slot of each triangle (item * 256 + row * 16 + column,
  * plus level << 28). */
 int bsk_pairs_slots(const bsk_pairs* pairs, int* slot_host);

@@ -211,3 +211,53 @@ def test_level_mfma_matches_one_level_and_fp64(N, nb, width, mode):
           'vs one-level %.2e, vs fp64 %.2e' % (N, nb, mode, plan.segments, e21, e2r))
     assert e21 < 5e-5
     assert e2r < 5e-5
+
+
+@pytest.mark.parametrize('nfield', [1, 2])
+def test_part_masked_shells_give_the_same_sums(nfield):
+    """Shells written only in the layout parts the plan reads: the MFMA sums
+    are bit-identical to those over fully written shells, the parts left out
+    keep what the buffer held, and a plan that reads more is refused."""
+    from bskit_amd import _lib as L
+    from bskit_amd import bins
+    N, nb, box = 512, 32, 1000.0
+    lo, hi = _bins(nb, 2, box)
+    eng = _engine(N, box, lo, hi)
+    tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), nfield).astype(np.int32)
+    parts = eng.shell_parts(tri, nfield)
+    assert parts.host.shape == (nb,) and (parts.host & 2).all()  # E: every shell
+    assert (parts.host != 0xf).any()  # something is left out
+    fs = [_field(N, 60 + i, np.float32) for i in range(nfield)]
+    d32 = [eng.forward(torch.from_numpy(f).cuda()) for f in fs]
+    full = [eng.shells(d, fmt=L.FMT_BF16X2, out=eng.alloc_shells(L.FMT_BF16X2, nb, True))
+            for d in d32]
+    masked = []
+    for d in d32:
+        sh = eng.alloc_shells(L.FMT_BF16X2, nb, True)
+        sh.data.fill_(0x1234)
+        sh.lo.fill_(0x1234)
+        masked.append(eng.shells(d, fmt=L.FMT_BF16X2, out=sh, parts=parts))
+    sc = 1.0 / eng.ncell
+    a = eng.triples(full, tri, path='mfma', scale=sc).cpu().numpy()
+    b = eng.triples(masked, tri, path='mfma', scale=sc).cpu().numpy()
+    assert np.array_equal(a, b)
+    # parts not written keep the fill; written parts equal the full shells
+    lay = eng.layout
+    pc = lay.part_cells()
+    starts = np.cumsum((0,) + pc[:3])
+    for s_full, s_mask in zip(full, masked):
+        for sh_i in range(nb):
+            for p in range(4):
+                seg = slice(starts[p], starts[p] + pc[p])
+                if parts.host[sh_i] >> p & 1:
+                    assert torch.equal(s_mask.data[sh_i, seg], s_full.data[sh_i, seg])
+                else:
+                    assert bool((s_mask.data[sh_i, seg] == 0x1234).all())
+    # shells written for a smaller list are refused for this one
+    small = eng.shell_parts(np.array([[0, 0, 1]], np.int32), nfield)
+    assert small.host[0] == 2 and small.host[2:].max() == 0  # E of shells 0, 1 only
+    thin = [eng.shells(d, fmt=L.FMT_BF16X2, out=eng.alloc_shells(L.FMT_BF16X2, nb, True),
+                       parts=small) for d in d32]
+    with pytest.raises(ValueError, match='another triangle list'):
+        eng.triples(thin, tri, path='mfma', scale=sc)
+    print('part masks nfield=%d: %s' % (nfield, np.unique(parts.host).tolist()))
