@@ -411,7 +411,7 @@ def main():
         if levels:
             parts = eng.layout.part_cells()
             alg = sum(4.0 * wn * sum(parts[q] for q in range(4) if pm >> q & 1)
-                      for pm, w0, wn, npass, ni in plan.segments)
+                      for pm, w0, wn, *_ in plan.segments)
         else:
             alg = nsh * 4 * ncell
         return roofline('k_triple_pairs', ms, alg, plan.npass, flops_sep, plan)

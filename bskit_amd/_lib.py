@@ -62,7 +62,7 @@ _SIGS = {
                           _vp, _vp, _vp],
     'bsk_pairs_create': [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _int_p],
     'bsk_pairs_create_levels': [ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _int_p, _int_p],
-    'bsk_pairs_segments': [_vp, _int_p, _int_p, _int_p, _int_p, _int_p, _int_p],
+    'bsk_pairs_segments': [_vp, _int_p, _int_p, _int_p, _int_p, _int_p, _int_p, _int_p],
     'bsk_pairs_upload': [_vp],
     'bsk_pairs_destroy': [_vp],
     'bsk_pairs_info': [_vp, _int_p, _int_p, _int_p, _int_p, _int_p],

@@ -671,6 +671,7 @@ struct bsk_pairs {
   int seg_win[kParts][2] = {{0, 1}, {0, 1}, {0, 1}, {0, 1}};
   int seg_pass[kParts] = {1, 0, 0, 0};
   int seg_ni[kParts] = {1, 1, 1, 1};  // items per wave of the segment's launches
+  int seg_rep[kParts] = {1, 1, 1, 1};  // replicas of its wave lists (each runs 1/rep of the K steps)
   int item_total = 0;  // distinct items: partial positions per workgroup
   std::vector<int> slot;
   std::vector<int> h_items, h_outpos, h_pairs;
@@ -897,9 +898,9 @@ LevelPlan plan_level(int mode, int nkb, int cap, const std::vector<int>& idx, co
 // Units (tile, kb mask) to (pass, wave) bins, costliest first (forming a
 // tile's products weighs three items), each to the cheapest bin with room for
 // its items; the fewest passes for which every unit fits.
-std::vector<std::vector<int>> bin_units(const std::vector<unsigned>& mask, int cap, int* npass,
-                                        int* maxload) {
-  const int W = kPairWaves, nunit = (int)mask.size();
+std::vector<std::vector<int>> bin_units(const std::vector<unsigned>& mask, int cap, int W,
+                                        int* npass, int* maxload) {
+  const int nunit = (int)mask.size();
   int nitem = 0;
   for (unsigned m : mask) {
     nitem += __builtin_popcount(m);
@@ -1015,7 +1016,7 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
   std::stable_partition(segs.begin(), segs.end(), [](const Seg& g) { return (g.parts >> 1) & 1; });
   // per segment: the units of its levels binned into passes
   struct SegPlan {
-    int lo, nsh, npass, ni;
+    int lo, nsh, npass, ni, rep = 1;
     std::vector<int> unit_tile;  // global tile
     std::vector<unsigned> unit_mask;
     std::vector<std::vector<int>> bins;
@@ -1042,7 +1043,21 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
       sp[g].npass = 1;
       sp[g].bins.assign(W, {});
     } else {
-      sp[g].bins = bin_units(sp[g].unit_mask, cap, &sp[g].npass, &ml);
+      sp[g].bins = bin_units(sp[g].unit_mask, cap, W, &sp[g].npass, &ml);
+      // few items (a plan over levels): lists on W / rep waves, replicated
+      // rep times, each replica running every rep-th K step
+      if (level)
+        for (int rep : {4, 2}) {
+          int np = 0, m2 = 1;
+          auto b2 = bin_units(sp[g].unit_mask, cap, W / rep, &np, &m2);
+          if (np == 1 && sp[g].npass == 1) {
+            sp[g].bins.clear();
+            for (int r = 0; r < rep; ++r) sp[g].bins.insert(sp[g].bins.end(), b2.begin(), b2.end());
+            sp[g].rep = rep;
+            ml = m2;
+            break;
+          }
+        }
     }
     sp[g].ni = bsk_pair_items_round(ml);
     maxni = std::max(maxni, sp[g].ni);
@@ -1133,6 +1148,7 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
     q->seg_win[g][1] = sp[g].nsh;
     q->seg_pass[g] = sp[g].npass;
     q->seg_ni[g] = sp[g].ni;
+    q->seg_rep[g] = sp[g].rep;
   }
   q->item_total = std::max(nitem, 1);
   q->h_items.swap(items);
@@ -1160,8 +1176,8 @@ int bsk_pairs_create_levels(bsk_pairs** out, int mode, int nshell, int ntri, con
 }
 
 int bsk_pairs_segments(const bsk_pairs* q, int* nseg, int* parts, int* first, int* count,
-                       int* npass, int* items_per_wave) {
-  if (!q || !nseg || !parts || !first || !count || !npass || !items_per_wave) {
+                       int* npass, int* items_per_wave, int* replicas) {
+  if (!q || !nseg || !parts || !first || !count || !npass || !items_per_wave || !replicas) {
     bsk_set_error("bsk_pairs_segments: invalid argument");
     return BSK_EINVAL;
   }
@@ -1173,6 +1189,7 @@ int bsk_pairs_segments(const bsk_pairs* q, int* nseg, int* parts, int* first, in
     count[g] = in ? q->seg_win[g][1] : 0;
     npass[g] = in ? q->seg_pass[g] : 0;
     items_per_wave[g] = in ? q->seg_ni[g] : 0;
+    replicas[g] = in ? q->seg_rep[g] : 0;
   }
   return BSK_OK;
 }
@@ -1332,6 +1349,7 @@ int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
     a.nkb = (a.koff + wn + 15) / 16;
     a.ncell = c1 - c0;
     a.accumulate = g > 0;
+    a.rep = q->seg_rep[g];
     const int ni = q->seg_ni[g];
     for (int k = 0; k < q->seg_pass[g]; ++k, ++p, e0 += (size_t)kPairWaves * ni) {
       a.items = q->d_items + e0;
@@ -1378,6 +1396,7 @@ const char* zfused_gap(const bsk_plan* p, const bsk_pairs* q) {
   const int nsh = q->seg_win[0][1];  // shells staged: the plan's window
   if (nsh > kZMaxShells) return "too many shells";
   if (q->seg_ni[0] < 1 || q->seg_ni[0] > 4) return "items per wave above 4";
+  if (q->seg_rep[0] != 1) return "replicated wave lists";
   bool size = false;
 #define C(n) size |= p->S == n;
   BSK_ZFUSED_SIZES(C)

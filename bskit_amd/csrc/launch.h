@@ -71,6 +71,7 @@ struct PairArgs {
   const int* items;       // [kPairWaves * NI] item descriptors of this launch
   const int* outpos;      // [kPairWaves * NI] partial position of each item (< 0: padding)
   int accumulate;         // add to the partials an earlier launch stored
+  int rep;                // replicas of the wave lists (1, 2, 4), splitting the K steps
   const int* pairs;       // [tiles + 1][16]: (row in set 0) | (row in set 1) << 16
   float* partial;         // [nwg][item_total][256]
   int item_total;         // items per workgroup in the partial layout

@@ -28,6 +28,8 @@
 //    sums are spatially coherent and large against the cancelling total).
 // Per-workgroup tiles are written as FP32 partials [wg][item][16][16]; a
 // second kernel sums them over workgroups in a fixed order in FP64.
+#include <algorithm>
+
 #include "pair_mma.h"
 
 namespace {
@@ -142,6 +144,9 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
 
   const PairConsts kc = pair_consts();
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform
+  // rep replicas of the item lists (waves w and w + kW/rep hold the same
+  // list); replica rgrp runs the K steps ks with ks % rep == rgrp
+  const int rep = a.rep, rgrp = wave / (kW / rep);
   const int lane = threadIdx.x & 63;
   const int col = lane & 15, grp = lane >> 4;
 
@@ -191,11 +196,36 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
     }
 #pragma unroll  // constant LDS offsets per K step
     for (int ks = 0; ks < kKSTEPS; ++ks)
-      kstep<NI, kLDH>(st, desc, oij, fA, fB, hK, lK, ks * 32 + grp * 4, ks * 32 + grp * 8, col,
-                      kc);
+      if (ks % rep == rgrp)  // uniform: this replica's K steps
+        kstep<NI, kLDH>(st, desc, oij, fA, fB, hK, lK, ks * 32 + grp * 4, ks * 32 + grp * 8, col,
+                        kc);
   }
-  store_at(st, a.partial + (long long)blockIdx.x * a.item_total * 256, opos, a.accumulate != 0,
-           grp, col);
+  float* const out = a.partial + (long long)blockIdx.x * a.item_total * 256;
+  if (rep == 1) {
+    store_at(st, out, opos, a.accumulate != 0, grp, col);
+    return;
+  }
+  // replicas of a wave's list ran other K steps: the first adds theirs
+  // (through LDS, FP64) and stores
+  const int wpr = kW / rep, wl = wave % wpr;  // waves per replica, wave within it
+  double* x = reinterpret_cast<double*>(smem);
+  __syncthreads();  // every wave is done with the staged chunk
+  if (rgrp > 0)
+#pragma unroll
+    for (int s = 0; s < NI; ++s)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        x[(((rgrp - 1) * wpr + wl) * NI + s) * 256 + r * 64 + lane] =
+            st.acc2[s][r] + (double)st.acc[s][r];
+  __syncthreads();
+  if (rgrp == 0) {
+    for (int g = 1; g < rep; ++g)
+#pragma unroll
+      for (int s = 0; s < NI; ++s)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) st.acc2[s][r] += x[(((g - 1) * wpr + wl) * NI + s) * 256 + r * 64 + lane];
+    store_at(st, out, opos, a.accumulate != 0, grp, col);
+  }
 }
 
 __global__ __launch_bounds__(64) void k_reduce_mfma(const float* __restrict__ partial, int nwg,
@@ -290,7 +320,13 @@ int bsk_triple_pairs_launch(const PairArgs& a, int sets, int ni, hipStream_t s) 
   int nmax = a.nsh[0];
   for (int r = 1; r < 3; ++r) nmax = a.nsh[r] > nmax ? a.nsh[r] : nmax;
   const int pv = bsk_pair_prefetch(nmax);
-  const size_t lds = bsk_pair_lds_bytes(sets, a.nsh, a.nkb);
+  if (a.rep != 1 && a.rep != 2 && a.rep != 4) {
+    bsk_set_error("mfma triple: %d replicas (1, 2 or 4)", a.rep);
+    return BSK_EINVAL;
+  }
+  // LDS: the staging, and the replicas' tiles (FP64) at the end
+  const size_t lds = std::max(bsk_pair_lds_bytes(sets, a.nsh, a.nkb),
+                              (size_t)(a.rep - 1) * (kW / a.rep) * ni * 256 * 8);
   if (!pv || lds > 160 * 1024) {
     bsk_set_error("mfma triple: %d shells exceed the kernel's staging (LDS %zu bytes)", nmax,
                   lds);

@@ -186,12 +186,12 @@ class PairPlan:
                    tri.ctypes.data_as(ip), levels.ctypes.data_as(ip))
         self.handle = h
         ns = ctypes.c_int()
-        arr = [(ctypes.c_int * 4)() for _ in range(5)]
+        arr = [(ctypes.c_int * 4)() for _ in range(6)]
         L.call('bsk_pairs_segments', h, ctypes.byref(ns), *arr)
         # launch segments: (layout parts bitmask, first shell, shells staged,
-        # launches, items per wave)
+        # launches, items per wave, replicas of the wave lists)
         self.segments = [tuple(a[g] for a in arr) for g in range(ns.value)]
-        self.list_entries = sum(8 * npass * ni for _, _, _, npass, ni in self.segments)
+        self.list_entries = sum(8 * g[3] * g[4] for g in self.segments)
         if upload:
             L.call('bsk_pairs_upload', h)
         self.ntri = len(tri)

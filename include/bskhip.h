@@ -241,13 +241,16 @@ int bsk_pairs_info(const bsk_pairs* pairs, int* npair, int* ntile, int* nitem,
  * plan, each with the items of those levels, the segment holding E (every
  * item) first; later segments add to its partials.  bsk_pairs_segments:
  * *nseg segments in launch order, per segment the parts (bit p), the shell
- * window its launches stage (shells [first, first + count)), its launches
- * and their items per wave (arrays of 4 entries).  A plan without levels (all level 0) is
+ * window its launches stage (shells [first, first + count)), its launches,
+ * their items per wave and the replicas of their wave lists (a segment with
+ * few items puts its lists on 8/replicas waves, repeated, each copy running
+ * every replicas-th K step; the first copy adds the others' sums and
+ * stores) (arrays of 4 entries).  A plan without levels (all level 0) is
  * one segment of all parts. */
 int bsk_pairs_create_levels(bsk_pairs** pairs, int mode, int nshell, int ntri,
                             const int* tri_host, const int* level);
 int bsk_pairs_segments(const bsk_pairs* pairs, int* nseg, int* parts, int* first, int* count,
-                       int* npass, int* items_per_wave);
+                       int* npass, int* items_per_wave, int* replicas);
 /* The layout parts (bit p) in which some launch of the plan stages shell
  * b: parts[b] for the nshell shells of the plan. */
 int bsk_pairs_shell_parts(const bsk_pairs* pairs, unsigned char* parts);

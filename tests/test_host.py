@@ -196,7 +196,7 @@ def _emulate_pair_plan(plan, sets, tri, mode, nshell, part_sets=None):
     # item continues the previous item's tile in the same wave's list
     # (flag 29: padding)
     e = 0
-    for _, _, _, npass, ni in plan.segments:
+    for _, _, _, npass, ni, rep in plan.segments:
         for w in range(8 * npass):
             lst = items[e:e + ni]
             e += ni
@@ -212,7 +212,8 @@ def _emulate_pair_plan(plan, sets, tri, mode, nshell, part_sets=None):
     part = np.zeros((max(plan.nitem, 1), 16, 16))
     written = np.zeros(len(part), bool)
     launch, e0 = 0, 0
-    for g, (parts, w0, wn, npass, ni) in enumerate(plan.segments):
+    for g, (parts, w0, wn, npass, ni, rep) in enumerate(plan.segments):
+        assert rep in (1, 2, 4) and (rep == 1 or npass == 1)
         assert 0 <= w0 and w0 + wn <= nshell
         if part_sets is None:
             assert parts == 0xf and len(plan.segments) == 1
@@ -222,7 +223,10 @@ def _emulate_pair_plan(plan, sets, tri, mode, nshell, part_sets=None):
                                          axis=1) for r in range(3))
         z = np.zeros_like(f0[0])
         for _ in range(npass):
-            for e in range(e0, e0 + 8 * ni):
+            # replicas: waves w and w + 8/rep hold the same list; one counts
+            lists = items[e0:e0 + 8 * ni].reshape(8, ni)
+            assert all(np.array_equal(lists[w], lists[w % (8 // rep)]) for w in range(8))
+            for e in range(e0, e0 + (8 // rep) * ni):
                 d, pos = items[e], outpos[e]
                 if (d >> 29) & 1:
                     assert pos == -1
@@ -311,7 +315,7 @@ def test_level_pair_plan(mode, nshell, ntri, seed, probs):
     # segments: maximal runs of parts with the same levels present
     present = set(levels.tolist())
     covered = 0
-    for parts, w0, wn, npass, ni in plan.segments:
+    for parts, w0, wn, npass, ni, rep in plan.segments:
         assert ni <= 4
         lv = [{L_ for L_ in present if p in _LEVEL_PARTS[L_]} for p in range(4) if parts >> p & 1]
         assert all(x == lv[0] for x in lv) and npass > 0
@@ -320,11 +324,13 @@ def test_level_pair_plan(mode, nshell, ntri, seed, probs):
     got = _emulate_pair_plan(plan, None, tri, mode, nshell, part_sets)
     ref = _level_reference(part_sets, tri, levels)
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
-    # all triangles at level 0: the plan of bsk_pairs_create, one segment
+    # all triangles at level 0: one segment of all parts, the items of
+    # bsk_pairs_create (which never replicates its wave lists: the C-order and
+    # fused entry points run it)
     p0 = PairPlan(tri, nshell, mode, upload=False, levels=np.zeros(ntri, np.int32))
     p1 = PairPlan(tri, nshell, mode, upload=False)
-    assert len(p1.segments) == 1 and p1.segments[0][0] == 0xf
-    assert p0.segments == p1.segments and np.array_equal(p0.slots(), p1.slots())
+    assert len(p1.segments) == 1 and p1.segments[0][0] == 0xf and p1.segments[0][5] == 1
+    assert p0.segments[0][:4] == p1.segments[0][:4] and np.array_equal(p0.slots(), p1.slots())
 
 
 @pytest.mark.parametrize('mode', [0, 1, 2])
@@ -356,7 +362,7 @@ def test_level_pair_plan_stages_shell_windows(mode):
         fields = [rng.standard_normal((nshell, 8)) for _ in range(nset)]
         part_sets.append([fields[r] for r in roles])
     plan = PairPlan(tri, nshell, mode, upload=False, levels=levels)
-    for parts, w0, wn, npass, ni in plan.segments:
+    for parts, w0, wn, npass, ni, rep in plan.segments:
         lv = [L_ for L_ in range(4) if any(parts >> p & 1 for p in _LEVEL_PARTS[L_])]
         assert (w0, w0 + wn) == (min(spans[L_][0] for L_ in lv), max(spans[L_][1] for L_ in lv))
     got = _emulate_pair_plan(plan, None, tri, mode, nshell, part_sets)
