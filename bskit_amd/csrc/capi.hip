@@ -1275,9 +1275,17 @@ int bsk_triples_mfma_workspace(const bsk_pairs* q, int nwg, size_t* bytes) {
     bsk_set_error("bsk_triples_mfma_workspace: invalid argument");
     return BSK_EINVAL;
   }
-  *bytes = sizeof(float) * 256 * (size_t)q->item_total * nwg;
+  // per-workgroup FP32 partials, then the reduction's FP64 range sums
+  *bytes = sizeof(float) * 256 * (size_t)q->item_total * nwg + bsk_reduce_mfma_bytes(q->item_total);
   return BSK_OK;
 }
+
+namespace {
+double* reduce_scratch(void* work, const bsk_pairs* q, int nwg) {
+  return reinterpret_cast<double*>(static_cast<char*>(work) +
+                                   sizeof(float) * 256 * (size_t)q->item_total * nwg);
+}
+}  // namespace
 
 namespace {
 int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
@@ -1361,7 +1369,7 @@ int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
   }
   if (!out_dev) return BSK_OK;  // partials only; bsk_triples_mfma_reduce finishes
   return bsk_reduce_mfma_launch(a.partial, nwg, q->item_total, q->ntri, q->d_slot, scale,
-                                out_dev, s);
+                                out_dev, reduce_scratch(work, q, nwg), s);
 }
 }  // namespace
 
@@ -1466,17 +1474,18 @@ int bsk_triples_zfused(bsk_plan* p, const bsk_pairs* q, const void* shell_work, 
   }
   if (!out_dev) return BSK_OK;
   return bsk_reduce_mfma_launch(a.partial, nwg, q->item_total, q->ntri, q->d_slot, scale,
-                                out_dev, s);
+                                out_dev, reduce_scratch(work, q, nwg), s);
 }
 
 int bsk_triples_mfma_reduce(const bsk_pairs* q, int nwg, double scale, double* out_dev,
-                            const void* work, void* stream) {
+                            void* work, void* stream) {
   if (!q || nwg <= 0 || !out_dev || !work || !q->d_slot) {
     bsk_set_error("bsk_triples_mfma_reduce: invalid argument (plan uploaded?)");
     return BSK_EINVAL;
   }
   return bsk_reduce_mfma_launch(reinterpret_cast<const float*>(work), nwg, q->item_total,
-                                q->ntri, q->d_slot, scale, out_dev, as_stream(stream));
+                                q->ntri, q->d_slot, scale, out_dev, reduce_scratch(work, q, nwg),
+                                as_stream(stream));
 }
 
 }  // extern "C"

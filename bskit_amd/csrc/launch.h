@@ -81,8 +81,10 @@ int bsk_pair_item_cap(int sets, int nshell);  // items per wave that compile wit
 int bsk_pair_prefetch(int nshell);  // prefetch segments per thread (0: too many shells)
 size_t bsk_pair_lds_bytes(int sets, const int nsh[3], int nkb);  // nkb covers koff + nsh[2]
 int bsk_triple_pairs_launch(const PairArgs& a, int sets, int ni, hipStream_t s);
+size_t bsk_reduce_mfma_bytes(int items_per_wg);  // scratch of the reduction (FP64 range sums)
 int bsk_reduce_mfma_launch(const float* partial, int nwg, int items_per_wg, int ntri,
-                           const int* slot_dev, double scale, double* out, hipStream_t s);
+                           const int* slot_dev, double scale, double* out, double* sums,
+                           hipStream_t s);
 
 // Fused z c2r + split-BF16 triple sums (zfused.hip): shell meshes with a
 // compiled kernel, at most kZMaxShells shells.

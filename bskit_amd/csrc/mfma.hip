@@ -228,19 +228,40 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
   }
 }
 
-__global__ __launch_bounds__(64) void k_reduce_mfma(const float* __restrict__ partial, int nwg,
-                                                    long long wg_stride, const int* __restrict__ slot,
+// Reduction of the per-workgroup FP32 partials in two coalesced steps, in
+// a fixed order (deterministic): k_reduce_slots sums, per partial slot, the
+// workgroups of each of kRedSplit ranges in FP64; k_reduce_tri adds the
+// ranges of each triangle's slot and applies the scale and the level factor.
+constexpr int kRedSplit = 16;
+
+__global__ __launch_bounds__(256) void k_reduce_slots(const float* __restrict__ partial, int nwg,
+                                                      int nslot, double* __restrict__ sums) {
+  const int e = blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
+  if (e >= nslot) return;
+  const int w0 = (int)((long long)nwg * c / kRedSplit), w1 = (int)((long long)nwg * (c + 1) / kRedSplit);
+  // four interleaved chains (four loads in flight), added in a fixed order
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  int w = w0;
+  for (; w + 4 <= w1; w += 4)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s[k] += (double)partial[(long long)(w + k) * nslot + e];
+  for (; w < w1; ++w) s[0] += (double)partial[(long long)w * nslot + e];
+  sums[(long long)c * nslot + e] = (s[0] + s[1]) + (s[2] + s[3]);
+}
+
+__global__ __launch_bounds__(256) void k_reduce_tri(const double* __restrict__ sums, int nslot,
+                                                    int ntri, const int* __restrict__ slot,
                                                     double scale, double* __restrict__ out) {
-  const int t = blockIdx.x;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= ntri) return;
   const int v = slot[t];
-  const long long e = v & 0x0fffffff;
+  const int e = v & 0x0fffffff;
   // bits 28-29: sublattice level L, whose cells are 2^-L of the mesh
   const double f = (double)(1 << ((v >> 28) & 3));
   double s = 0.0;
-  for (int w = threadIdx.x; w < nwg; w += 64) s += (double)partial[w * wg_stride + e];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (threadIdx.x == 0) out[t] = s * scale * f;
+  for (int c = 0; c < kRedSplit; ++c) s += sums[(long long)c * nslot + e];
+  out[t] = s * scale * f;
 }
 
 template <int NI, int PV, int SETS>
@@ -339,11 +360,20 @@ int bsk_triple_pairs_launch(const PairArgs& a, int sets, int ni, hipStream_t s) 
   return BSK_EINVAL;
 }
 
+size_t bsk_reduce_mfma_bytes(int items_per_wg) {
+  return sizeof(double) * kRedSplit * 256 * (size_t)items_per_wg;
+}
+
 int bsk_reduce_mfma_launch(const float* partial, int nwg, int items_per_wg, int ntri,
-                           const int* slot, double scale, double* out, hipStream_t s) {
+                           const int* slot, double scale, double* out, double* sums,
+                           hipStream_t s) {
   if (ntri <= 0) return BSK_OK;
-  hipLaunchKernelGGL(k_reduce_mfma, dim3(ntri), dim3(64), 0, s, partial, nwg,
-                     (long long)items_per_wg * 256, slot, scale, out);
+  const int nslot = items_per_wg * 256;
+  hipLaunchKernelGGL(k_reduce_slots, dim3((nslot + 255) / 256, kRedSplit), dim3(256), 0, s,
+                     partial, nwg, nslot, sums);
+  BSK_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_reduce_tri, dim3((ntri + 255) / 256), dim3(256), 0, s, sums, nslot, ntri,
+                     slot, scale, out);
   BSK_LAUNCH_CHECK();
   return BSK_OK;
 }

@@ -280,9 +280,10 @@ int bsk_triples_mfma_levels(const bsk_pairs* pairs, const uint16_t* hi_i, const 
                             double scale, double* out_dev, void* work, void* stream);
 /* out_dev == NULL in bsk_triples_mfma leaves the per-workgroup partials in
  * work; bsk_triples_mfma_reduce then sums them in FP64 into out_dev (the
- * same result as one call with out_dev). */
+ * same result as one call with out_dev; it uses the tail of work, after the
+ * partials, for its range sums). */
 int bsk_triples_mfma_reduce(const bsk_pairs* pairs, int nwg, double scale, double* out_dev,
-                            const void* work, void* stream);
+                            void* work, void* stream);
 /* The same split-BF16 sums for an auto spectrum with the z c2r of the shells
  * done inside the triple-sum kernel: shell_work is the bsk_shells workspace
  * after stages BSK_STAGE_X | BSK_STAGE_Y of shells [shell0, shell0 +

@@ -170,7 +170,9 @@ def test_library_loads_and_host_calls_work():
     sl = plan.slots()
     assert sl[0] == sl[1]  # symmetric spectra: any side order maps to one slot
     L.call('bsk_triples_mfma_workspace', plan.handle, 10, ctypes.byref(n))
-    assert n.value == 10 * max(plan.nitem, 1) * 256 * 4  # FP32 partial tiles per workgroup
+    # FP32 partial tiles per workgroup, then the reduction's FP64 sums over 16
+    # workgroup ranges per slot
+    assert n.value == 10 * max(plan.nitem, 1) * 256 * 4 + 16 * max(plan.nitem, 1) * 256 * 8
     for mode, nsh in ((L.MODE_AUTO, 129), (L.MODE_ABC, 97)):
         with pytest.raises(L.BskError):
             PairPlan(np.array([[0, 0, 0]]), nsh, mode, upload=False)
