@@ -514,6 +514,26 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
         store_zero<T>(fmt, shi, slo, (unsigned)e);
     }
   const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
+  // Pencil p transforms row row0 + prow(p), p < np.  All nr rows, or, for a
+  // shell whose parts hold only E and O, the rows with x + y even (x + y odd
+  // feeds neither; y = row mod 2 as N is even): n1 of them in the x plane of
+  // row0 from r1, the rest in the next plane from r2 (at most one plane
+  // boundary per workgroup as N >= P), gathered into the first pencils.
+  const bool gather = lv && !(pm & 0x9) && N >= P;
+  int np = nr, n1 = nr, r1 = 0, r2 = 0, step = 1;
+  if (gather) {
+    const long long x0 = row0 / N, b = (x0 + 1) * N, end = row0 + nr;
+    const long long e1 = b < end ? b : end;
+    const int f1 = (int)((x0 + row0) & 1), f2 = (int)((x0 + 1 + b) & 1);
+    n1 = e1 - row0 > f1 ? (int)((e1 - row0 - f1 + 1) / 2) : 0;
+    const int n2 = end - b > f2 ? (int)((end - b - f2 + 1) / 2) : 0;
+    r1 = f1;
+    r2 = (int)(b - row0) + f2;
+    np = n1 + n2;
+    step = 2;
+    if (np == 0) return;  // uniform
+  }
+  auto prow = [&](int p) { return p < n1 ? r1 + step * p : r2 + step * (p - n1); };
   // (k, NH - k) pairs: lane k of rows r0, r0 + RK, ...; up to 8 rows (16
   // loads) in flight per batch
   const int k = tid & (KQ - 1), r0 = tid / KQ, j = NH - k;
@@ -526,8 +546,8 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
 #pragma unroll
     for (int q = 0; q < B; ++q) {
       const int p = r0 + (q0 + q) * RK;
-      const bool ok = q0 + q < NPK && kact && p < nr;
-      const unsigned r = (unsigned)(p * MZP);
+      const bool ok = q0 + q < NPK && kact && p < np;
+      const unsigned r = (unsigned)(prow(p) * MZP);
       xk[q] = (ok && k < mzb) ? get_u32(src, r + k) : cplx<T>{(T)0, (T)0};
       xj[q] = (ok && j < mzb) ? get_u32(src, r + j) : cplx<T>{(T)0, (T)0};
     }
@@ -536,7 +556,12 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
       if (q0 + q < NPK && kact) c2r_pair<T>(buf + (r0 + (q0 + q) * RK) * LD, k, NH, xk[q], xj[q], wk);
   }
   __syncthreads();
-  fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
+  if (gather)
+    fft_lds_n<T, NH, true, P, W, LD, N>(
+        buf, tw, np, (unsigned)((0x100000000ull + (unsigned)np - 1) / (unsigned)np), tid,
+        typename Plan<NH, V>::seq{});
+  else
+    fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
   // two complex values (four reals) per lane and store
   const SplitConsts sc = split_consts();
   const long long obase = (bl * shell_stride + cell0 + row0 * N) * esz;
@@ -547,7 +572,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
 #pragma unroll
     for (int i = 0; i < NPQ; ++i) {
       const int p = rq + i * RQ;
-      if (p >= nr) continue;
+      if (p >= np) continue;
       const cplx<T>* b = buf + p * LD + 2 * q;
       if (!lv) {
         store_quad<T>(fmt, ohi, olo, (unsigned)(p * N + 4 * q), b[0], b[1], sc);
@@ -555,7 +580,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
       }
       // z = 4q, 4q + 2 (b[0].x, b[1].x) and z = 4q + 1, 4q + 3 (b[0].y, b[1].y):
       // the same z/2 = 2q, 2q + 1 in the part of each z parity
-      const int row = (int)row0 + p, x = row / N, y = row - x * N;
+      const int row = (int)row0 + prow(p), x = row / N, y = row - x * N;
       const int xy = (x & 1) * 2 + (y & 1);  // 0: both even, 3: both odd
       const unsigned pos = (unsigned)(((x >> 1) * NH + (y >> 1)) * NH + 2 * q);
 #pragma unroll
