@@ -11,10 +11,15 @@ equilateral (32), isosceles m=2 and squeezed with |k1| = k_f (a 33rd shell
 
 One step ("one grid") = forward FFT of the HBM-resident painted field +
 33 shell fields + all T triangle sums + normalisation by the cached N_tri.
-Grid info (N_tri, k-means) is computed once (FP64, exact counts) and
-reported separately.  Two paths are timed: split-BF16 MFMA (the value) and
-the reference-structured FP32 sweep per triangle; both are compared with
-the FP64 path for the relative B error.
+The value is the throughput with --inflight grids (default 3) in flight on
+as many HIP streams (consecutive grids alternate between the streams, each
+with its own buffers, so one grid's FFT passes overlap another's triple
+sums); the same path is then timed one grid at a time (stage times, the
+roofline's kernel time, `one_grid_at_a_time`).  Grid info (N_tri, k-means)
+is computed once (FP64, exact counts) and reported separately.  Two paths
+are timed: split-BF16 MFMA (the value) and the reference-structured FP32
+sweep per triangle; both are compared with the FP64 path for the relative
+B error.
 
 Latency (the metric's p50/p99): a serving run of seeded fast-path jobs
 (N uniform over {128,192,256,384,512,768}, nb uniform over [8, 40], auto or
@@ -112,46 +117,64 @@ class StageTimer:
         return [a.elapsed_time(b) for a, b in self.ev.get(name, [])]
 
 
-def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None):
+def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None, stages=True,
+             inflight=1):
+    """Time `steps` grids of one path.  With inflight > 1 consecutive grids
+    go to `inflight` HIP streams in turn, each with its own spectrum, shells,
+    output and workspace, so one grid's FFT passes overlap another's triple
+    sums; every grid still runs forward + shells + all triangle sums +
+    normalisation.  timer: the kernel events of the triple sums are recorded
+    in every step, the stage marks only with stages=True."""
     from bskit_amd import _lib as L
+    from bskit_amd.engine import Workspace
     # L^6/N^3 sum over the N^3 mesh == L^6/S^3 sum over the S^3 shell mesh
     scale = wl['L'] ** 6 / float(eng.ncell)
-    shells = eng.alloc_shells(fmt, eng.nb) if path != 'zfused' else None
-    out = torch.empty(len(wl['tri']), dtype=torch.float64, device='cuda')
     dtype = torch.float64 if fmt == L.FMT_F64 else torch.float32
+    ctype = torch.complex128 if fmt == L.FMT_F64 else torch.complex64
     field = wl['field'].to(dtype)
     # level layout: write only the parts of each shell the sums read
-    parts = (eng.shell_parts(wl['tri']) if path == 'mfma' and shells is not None and
-             shells.layout is not None else None)
+    use_parts = path == 'mfma' and eng.uses_levels(fmt)
+    parts = eng.shell_parts(wl['tri']) if use_parts else None
+    streams = [torch.cuda.current_stream()] + [torch.cuda.Stream() for _ in range(inflight - 1)]
+    ctx = []
+    for s in streams:
+        with torch.cuda.stream(s):
+            ctx.append(dict(
+                shells=eng.alloc_shells(fmt, eng.nb) if path != 'zfused' else None,
+                out=torch.empty(len(wl['tri']), dtype=torch.float64, device='cuda'),
+                dk=torch.empty((eng.MY, eng.MX, eng.MZ), dtype=ctype, device='cuda'),
+                work=eng.work if s is streams[0] else Workspace(eng.device)))
+    torch.cuda.synchronize()
 
-    def step(t):
-        if t is not None:
-            t.mark('start')
-        dk = eng.forward(field)
-        if t is not None:
-            t.mark('fwd')
-        kev = t.pair('kernel') if (t is not None and path in ('mfma', 'zfused')) else None
-        if path == 'zfused':  # shells x + y passes, then z c2r fused into the sums
-            eng.bispectrum_zfused(dk, wl['tri'], scale=scale, out=out, kernel_events=kev,
-                                  after_shells=(lambda: t.mark('shells')) if t else None)
-        else:
-            eng.shells(dk, fmt=fmt, dtype=dtype, out=shells, parts=parts)
-            if t is not None:
-                t.mark('shells')
-            eng.triples([shells], wl['tri'], path=path, scale=scale, out=out, kernel_events=kev)
-        if t is not None:
-            t.mark('triples')
-        b = out / ntri_dev
-        if t is not None:
-            t.mark('end')
+    def step(i, t):
+        s, c = streams[i % inflight], ctx[i % inflight]
+        mark = (lambda name: t.mark(name)) if (t is not None and stages) else (lambda name: None)
+        with torch.cuda.stream(s):
+            mark('start')
+            dk = eng.forward(field, stream=s, work=c['work'], out=c['dk'])
+            mark('fwd')
+            kev = t.pair('kernel') if (t is not None and path in ('mfma', 'zfused')) else None
+            if path == 'zfused':  # shells x + y passes, then z c2r fused into the sums
+                eng.bispectrum_zfused(dk, wl['tri'], scale=scale, out=c['out'], stream=s,
+                                      work=c['work'], kernel_events=kev,
+                                      after_shells=lambda: mark('shells'))
+            else:
+                eng.shells(dk, fmt=fmt, dtype=dtype, out=c['shells'], parts=parts, stream=s,
+                           work=c['work'])
+                mark('shells')
+                eng.triples([c['shells']], wl['tri'], path=path, scale=scale, out=c['out'],
+                            stream=s, work=c['work'], kernel_events=kev)
+            mark('triples')
+            b = c['out'] / ntri_dev
+            mark('end')
         return b
 
-    for _ in range(warmup):
-        step(None)
+    for i in range(warmup):
+        step(i, None)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        b = step(timer)
+    for i in range(steps):
+        b = step(i, timer)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     return b, elapsed
@@ -241,9 +264,11 @@ def run_batch(args, torch, np):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=200)
+    ap.add_argument('--warmup', type=int, default=20)
     ap.add_argument('--plain-steps', type=int, default=2)
+    ap.add_argument('--inflight', type=int, default=3,
+                    help='grids in flight on as many HIP streams in the timed run')
     ap.add_argument('--nmesh', type=int, default=512)
     ap.add_argument('--nbin', type=int, default=32)
     ap.add_argument('--npart-side', type=int, default=256)
@@ -316,12 +341,23 @@ def main():
     # mesh + the pair kernel
     zf_ok = not args.no_zfused and eng.zfused_supported(wl['tri'])
     main_path = 'zfused' if zf_ok and eng.prefer_zfused else 'mfma'
-    timer = StageTimer(torch)
+    # headline: args.inflight grids in flight on as many streams, no events;
+    # then the same path one grid at a time with stage marks and HIP events
+    # around the pair-kernel launches of every step (stage times, the
+    # one-at-a-time rate and the roofline's kernel time)
     b_mfma, el_mfma = run_path(eng, wl, main_path, L.FMT_BF16X2, args.steps, args.warmup, torch,
-                               ntri_dev, timer)
+                               ntri_dev, inflight=args.inflight)
     ms_step = el_mfma / args.steps * 1e3
     grids_mfma = args.steps / el_mfma
-    log('[bench] %s: %.3f ms/grid, %.2f grids/s' % (main_path, ms_step, grids_mfma))
+    log('[bench] %s, %d in flight: %.3f ms/grid, %.2f grids/s'
+        % (main_path, args.inflight, ms_step, grids_mfma))
+    timer = StageTimer(torch)
+    b_seq, el_seq = run_path(eng, wl, main_path, L.FMT_BF16X2, args.steps, args.warmup, torch,
+                             ntri_dev, timer)
+    sequential = {'grids_per_s': args.steps / el_seq, 'ms_per_grid': el_seq / args.steps * 1e3,
+                  'same_sums': bool(torch.equal(torch.nan_to_num(b_seq), torch.nan_to_num(b_mfma)))}
+    log('[bench] %s, one at a time: %.3f ms/grid, %.2f grids/s'
+        % (main_path, sequential['ms_per_grid'], sequential['grids_per_s']))
     stage_ms = {
         'forward': float(np.mean(timer.durations('start', 'fwd'))),
         'shells': float(np.mean(timer.durations('fwd', 'shells'))),
@@ -475,9 +511,15 @@ def main():
                                             'over the sparsest exact sublattice (triangles at '
                                             '1, 1/2, 1/4, 1/8 of the cells: %s)'
                                             % '/'.join(str(int(c)) for c in lcount))),
-                   'parallelism': 'single GPU'},
+                   'grids_in_flight': args.inflight,
+                   'parallelism': ('single GPU, %d grids in flight on %d HIP streams'
+                                   % (args.inflight, args.inflight) if args.inflight > 1
+                                   else 'single GPU')},
         'fp32_plain': res_plain,
-        'speedup_vs_fp32_plain': (grids_mfma / res_plain['grids_per_s']) if res_plain else None,
+        # one grid at a time on both sides
+        'speedup_vs_fp32_plain': ((sequential['grids_per_s'] / res_plain['grids_per_s'])
+                                  if res_plain else None),
+        'one_grid_at_a_time': sequential,
         'stage_ms': stage_ms,
         'other_path': alt,
         'stage_GBps': stage_gbs,
