@@ -261,3 +261,46 @@ def test_part_masked_shells_give_the_same_sums(nfield):
     with pytest.raises(ValueError, match='another triangle list'):
         eng.triples(thin, tri, path='mfma', scale=sc)
     print('part masks nfield=%d: %s' % (nfield, np.unique(parts.host).tolist()))
+
+
+def test_grids_in_flight_on_streams_equal_one_at_a_time():
+    """Grids issued on three HIP streams in turn, each stream with its own
+    shells, output and workspace (the throughput run of bench.py), give the
+    sums of the same grids run one at a time, bit for bit, over two rounds
+    that reuse every stream's buffers."""
+    from bskit_amd import _lib as L
+    from bskit_amd import bins
+    from bskit_amd.engine import Workspace
+    N, nb, box = 128, 12, 1000.0
+    lo, hi = _bins(nb, 2, box)
+    eng = _engine(N, box, lo, hi)
+    tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), 1).astype(np.int32)
+    parts = eng.shell_parts(tri)
+    fields = [torch.from_numpy(_field(N, 80 + i, np.float32)).cuda() for i in range(5)]
+    sc = 1.0 / eng.ncell
+
+    def grid(f, stream=None, work=None, sh=None, out=None):
+        dk = eng.forward(f, stream=stream, work=work)
+        sh = eng.shells(dk, fmt=L.FMT_BF16X2, out=sh, parts=parts, stream=stream, work=work)
+        return eng.triples([sh], tri, scale=sc, out=out, stream=stream, work=work)
+
+    ref = [grid(f).cpu().numpy() for f in fields]
+    ns = 3
+    streams = [torch.cuda.Stream() for _ in range(ns)]
+    ctx = []
+    for s in streams:
+        with torch.cuda.stream(s):
+            ctx.append(dict(work=Workspace(eng.device),
+                            sh=eng.alloc_shells(L.FMT_BF16X2, eng.nb),
+                            out=torch.empty(len(tri), dtype=torch.float64, device='cuda')))
+    torch.cuda.synchronize()
+    for _ in range(2):
+        got = []
+        for i, f in enumerate(fields):
+            s, c = streams[i % ns], ctx[i % ns]
+            with torch.cuda.stream(s):
+                o = grid(f, stream=s, work=c['work'], sh=c['sh'], out=c['out'])
+                got.append(o.clone())  # on s, before the stream's next grid reuses out
+        torch.cuda.synchronize()
+        for i, g in enumerate(got):
+            assert np.array_equal(g.cpu().numpy(), ref[i]), i
