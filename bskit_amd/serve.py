@@ -65,7 +65,15 @@ def _bins_key(kLU):
 class Server:
     """Admission + continuous batching of Jobs on one device."""
 
-    def __init__(self, device=None, hbm_budget=None, nstreams=4, poll_s=2e-5):
+    def __init__(self, device=None, hbm_budget=None, nstreams=4, poll_s=2e-5, assign='work'):
+        """assign: the stream an arriving job is issued to -- 'work', the one
+        with the least outstanding work (the footprints of its jobs in flight,
+        which grow with the mesh and the shell set), so a small job does not
+        queue behind a large one where another stream is nearly idle; or
+        'count', the one with the fewest jobs in flight."""
+        if assign not in ('work', 'count'):
+            raise ValueError("assign must be 'work' or 'count'")
+        self.assign = assign
         self.device = require_device(device)
         free, _ = torch.cuda.mem_get_info(self.device)
         self.budget = int(hbm_budget) if hbm_budget else int(min(256e9, 0.85 * free))
@@ -161,6 +169,7 @@ class Server:
         pending = collections.deque(sorted(jobs, key=lambda j: (j.arrival, j.jid)))
         inflight = []
         per_stream = [0] * len(self.streams)
+        work = [0] * len(self.streams)  # footprints in flight per stream
         used = 0
         peak = 0
         t0 = time.perf_counter()
@@ -173,6 +182,7 @@ class Server:
                     j.done = now
                     used -= j.footprint
                     per_stream[j.stream] -= 1
+                    work[j.stream] -= j.footprint
                 else:
                     keep.append((j, ev))
             inflight = keep
@@ -181,10 +191,12 @@ class Server:
                 if inflight and used + j.footprint > self.budget:
                     break
                 pending.popleft()
-                si = int(np.argmin(per_stream))
+                load = work if self.assign == 'work' else per_stream
+                si = min(range(len(self.streams)), key=lambda i: (load[i], per_stream[i], i))
                 j.issued = time.perf_counter() - t0
                 ev = self._issue(j, si)
                 per_stream[si] += 1
+                work[si] += j.footprint
                 used += j.footprint
                 peak = max(peak, used)
                 inflight.append((j, ev))

@@ -530,3 +530,12 @@ def test_sublattice_duals_by_brute_force(S):
             if cls:
                 rule.add(((h * a) % S, (h * b) % S, (h * c) % S))
         assert dual == rule, (L, sorted(dual), sorted(rule))
+
+
+def test_server_rejects_unknown_stream_assignment():
+    """The serving loop's stream choice is 'work' (least outstanding
+    footprint) or 'count' (fewest jobs in flight); anything else is refused
+    before the device is touched."""
+    from bskit_amd import serve
+    with pytest.raises(ValueError):
+        serve.Server(assign='random')
