@@ -12,12 +12,12 @@ equilateral (32), isosceles m=2 and squeezed with |k1| = k_f (a 33rd shell
 One step ("one grid") = forward FFT of the HBM-resident painted field +
 33 shell fields + all T triangle sums + normalisation by the cached N_tri.
 The value is the throughput with --inflight grids (default 3) in flight on
-as many HIP streams (consecutive grids alternate between the streams, each
-with its own buffers, so one grid's FFT passes overlap another's triple
-sums; the pair kernel's workgroup count for this mode is picked by timing
-256/512/768/1024 in flight); the same path is then timed one grid at a time
-with the count tuned alone (stage times, the roofline's kernel time,
-`one_grid_at_a_time`).  Grid info (N_tri, k-means)
+as many HIP streams through bskit_amd.pipeline.GridPipeline (consecutive
+grids alternate between the streams, each with its own buffers, so one
+grid's FFT passes overlap another's triple sums; the pair kernel's
+workgroup count for this mode is picked by timing 256/512/768/1024 in
+flight); the same path is then timed one grid at a time with the count
+tuned alone (stage times, the roofline's kernel time, `one_grid_at_a_time`).  Grid info (N_tri, k-means)
 is computed once (FP64, exact counts) and reported separately.  Two paths
 are timed: split-BF16 MFMA (the value) and the reference-structured FP32
 sweep per triangle; both are compared with the FP64 path for the relative
@@ -266,8 +266,8 @@ def run_batch(args, torch, np):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=200)
-    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--steps', type=int, default=500)
+    ap.add_argument('--warmup', type=int, default=50)
     ap.add_argument('--plain-steps', type=int, default=2)
     ap.add_argument('--inflight', type=int, default=3,
                     help='grids in flight on as many HIP streams in the timed run')
@@ -343,37 +343,41 @@ def main():
     # mesh + the pair kernel
     zf_ok = not args.no_zfused and eng.zfused_supported(wl['tri'])
     main_path = 'zfused' if zf_ok and eng.prefer_zfused else 'mfma'
-    # the pair kernel's workgroup count for grids in flight: fewer workgroups
-    # leave CUs to the other streams' FFT passes, so the count tuned for one
-    # grid alone is not the best one here; each candidate is timed over 100
-    # grids in flight and the fastest kept (the one-grid-at-a-time run below
-    # uses the count tuned alone)
-    nwg_alone = eng.tuned_nwg
+    # headline: args.inflight grids in flight on as many streams, through
+    # bskit_amd.pipeline.GridPipeline (the library's throughput mode), no
+    # events; its pair-kernel workgroup count is tuned in that mode (fewer
+    # workgroups leave CUs to the other streams' FFT passes).  Then the same
+    # path one grid at a time with stage marks and HIP events around the
+    # pair-kernel launches of every step (stage times, the one-at-a-time rate
+    # and the roofline's kernel time).
     nwg_in_flight = None
-    if args.inflight > 1 and main_path == 'mfma' and not args.no_tune:
-        rates = {}
-        for nwg in (256, 512, 768, 1024):
-            eng.tuned_nwg = nwg
-            _, el = run_path(eng, wl, main_path, L.FMT_BF16X2, 100, 10, torch, ntri_dev,
-                             inflight=args.inflight)
-            rates[nwg] = 100 / el
-        nwg_in_flight = max(rates, key=rates.get)
-        eng.tuned_nwg = nwg_in_flight
-        log('[bench] pair-kernel workgroups in flight: %s -> %d'
-            % ({k: round(v, 1) for k, v in rates.items()}, nwg_in_flight))
-        if tuned is not None:
-            tuned['mfma_nwg_in_flight'] = nwg_in_flight
-    # headline: args.inflight grids in flight on as many streams, no events;
-    # then the same path one grid at a time with stage marks and HIP events
-    # around the pair-kernel launches of every step (stage times, the
-    # one-at-a-time rate and the roofline's kernel time)
-    b_mfma, el_mfma = run_path(eng, wl, main_path, L.FMT_BF16X2, args.steps, args.warmup, torch,
-                               ntri_dev, inflight=args.inflight)
+    if args.inflight > 1 and main_path == 'mfma':
+        from bskit_amd.pipeline import GridPipeline
+        pipe = GridPipeline(eng, wl['tri'], inflight=args.inflight,
+                            scale=wl['L'] ** 6 / float(eng.ncell), norm=ntri_dev)
+        if not args.no_tune:
+            rates = pipe.tune(wl['field'])
+            nwg_in_flight = pipe.nwg
+            log('[bench] pair-kernel workgroups in flight: %s -> %d'
+                % ({k: round(v, 1) for k, v in rates.items()}, nwg_in_flight))
+            if tuned is not None:
+                tuned['mfma_nwg_in_flight'] = nwg_in_flight
+        for _ in range(args.warmup):
+            pipe.submit(wl['field'])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b_mfma = pipe.submit(wl['field'])
+        torch.cuda.synchronize()
+        el_mfma = time.perf_counter() - t0
+        del pipe
+    else:
+        b_mfma, el_mfma = run_path(eng, wl, main_path, L.FMT_BF16X2, args.steps, args.warmup,
+                                   torch, ntri_dev, inflight=args.inflight)
     ms_step = el_mfma / args.steps * 1e3
     grids_mfma = args.steps / el_mfma
     log('[bench] %s, %d in flight: %.3f ms/grid, %.2f grids/s'
         % (main_path, args.inflight, ms_step, grids_mfma))
-    eng.tuned_nwg = nwg_alone
     timer = StageTimer(torch)
     b_seq, el_seq = run_path(eng, wl, main_path, L.FMT_BF16X2, args.steps, args.warmup, torch,
                              ntri_dev, timer)

@@ -12,6 +12,7 @@ from .main import (FFTBispectrum, bk_binned, bk_FFT_full, bk_FFT_grid_info,  # n
                    bk_FFT_unnormalized_value, compute_bk_FFT_value, compute_k_means_on_grid,
                    compute_Nbin, k_field, number_field, pk_FFT, power_spectrum, PRECISIONS)
 from .mesh import ArrayMesh, MeshSource, ParticleMesh, cast_source  # noqa: F401
+from .pipeline import GridPipeline  # noqa: F401
 from .subbox import (field_subbox_pm, field_subbox_slice, measure_subboxes,  # noqa: F401
                      subbox_index_to_multiindex, subbox_multiindex_to_index)
 from . import io  # noqa: F401

@@ -304,3 +304,41 @@ def test_grids_in_flight_on_streams_equal_one_at_a_time():
         torch.cuda.synchronize()
         for i, g in enumerate(got):
             assert np.array_equal(g.cpu().numpy(), ref[i]), i
+
+
+def test_grid_pipeline_equals_one_at_a_time():
+    """bskit_amd.pipeline.GridPipeline (three grids in flight, normalised by
+    a device tensor) returns every grid's sums bit for bit as the engine's
+    one-at-a-time path divided by the same tensor; tune() keeps one of its
+    candidate workgroup counts, with which the sums again equal the
+    one-at-a-time path run with that count."""
+    from bskit_amd import _lib as L
+    from bskit_amd import bins
+    from bskit_amd.pipeline import GridPipeline
+    N, nb, box = 128, 12, 1000.0
+    lo, hi = _bins(nb, 2, box)
+    eng = _engine(N, box, lo, hi)
+    tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), 1).astype(np.int32)
+    fields = [torch.from_numpy(_field(N, 90 + i, np.float32)).cuda() for i in range(4)]
+    sc = 1.0 / eng.ncell
+    norm = torch.arange(1, len(tri) + 1, dtype=torch.float64, device='cuda')
+
+    def one(f, nwg=None):
+        sh = eng.shells(eng.forward(f), fmt=L.FMT_BF16X2,
+                        out=eng.alloc_shells(L.FMT_BF16X2, eng.nb))
+        return (eng.triples([sh], tri, scale=sc, nwg=nwg) / norm).cpu().numpy()
+
+    ref = [one(f) for f in fields]
+    pipe = GridPipeline(eng, tri, inflight=3, scale=sc, norm=norm)
+    for _ in range(2):
+        outs = [pipe.submit(f) for f in fields]
+        pipe.synchronize()
+        for i, o in enumerate(outs):
+            assert np.array_equal(o.cpu().numpy(), ref[i]), i
+    rates = pipe.tune(fields[0], candidates=(32, 64), grids=3, warmup=1)
+    assert set(rates) == {32, 64} and pipe.nwg in rates
+    out = pipe.submit(fields[1])
+    pipe.synchronize()
+    assert np.array_equal(out.cpu().numpy(), one(fields[1], nwg=pipe.nwg))
+    with pytest.raises(ValueError):
+        GridPipeline(eng, tri, inflight=0)
