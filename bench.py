@@ -381,8 +381,12 @@ def main():
     timer = StageTimer(torch)
     b_seq, el_seq = run_path(eng, wl, main_path, L.FMT_BF16X2, args.steps, args.warmup, torch,
                              ntri_dev, timer)
+    # the two runs may use different pair-kernel workgroup counts, which
+    # split the cells into other FP32 chains: the sums then differ by rounding
+    ok = torch.isfinite(b_seq) & torch.isfinite(b_mfma)
     sequential = {'grids_per_s': args.steps / el_seq, 'ms_per_grid': el_seq / args.steps * 1e3,
-                  'same_sums': bool(torch.equal(torch.nan_to_num(b_seq), torch.nan_to_num(b_mfma)))}
+                  'max_diff_vs_in_flight': float((b_seq[ok] - b_mfma[ok]).abs().max()
+                                                 / b_seq[ok].abs().max())}
     log('[bench] %s, one at a time: %.3f ms/grid, %.2f grids/s'
         % (main_path, sequential['ms_per_grid'], sequential['grids_per_s']))
     stage_ms = {
