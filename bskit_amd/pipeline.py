@@ -25,23 +25,29 @@ from .engine import PATH_MFMA, Workspace
 class GridPipeline:
     """Split-BF16 MFMA triple sums of grids with up to `inflight` in flight.
 
-    eng: a GridEngine; tri: the (T, 3) triangle shell indices (auto
-    spectra); scale: the factor of every sum (L^6 / S^3 for B_unnorm on the
-    shell mesh); norm: optional float64 device tensor of T values each
-    grid's sums are divided by (the cached N_tri); nwg: the pair kernel's
-    workgroup count (default: the engine's; tune() picks one for this mode).
+    eng: a GridEngine; tri: the (T, 3) triangle shell indices; nfield: 1
+    for auto spectra <AAA>, 2 for cross spectra <AAB> (sides A, A, B; each
+    grid is then two fields); scale: the factor of every sum (L^6 / S^3 for
+    B_unnorm on the shell mesh); norm: optional float64 device tensor of T
+    values each grid's sums are divided by (the cached N_tri); nwg: the pair
+    kernel's workgroup count (default: the engine's; tune() picks one for
+    this mode).
     """
 
-    def __init__(self, eng, tri, inflight=3, scale=1.0, norm=None, nwg=None):
+    def __init__(self, eng, tri, inflight=3, scale=1.0, norm=None, nwg=None, nfield=1):
         if inflight < 1:
             raise ValueError('inflight must be >= 1')
+        if nfield not in (1, 2):
+            raise ValueError('nfield must be 1 (auto) or 2 (cross <AAB>)')
         self.eng = eng
         self.tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
+        self.nfield = nfield
         self.scale = float(scale)
         self.norm = norm
         self.nwg = nwg
         self.fmt = L.FMT_BF16X2
-        self.parts = eng.shell_parts(self.tri) if eng.uses_levels(self.fmt) else None
+        self.parts = (eng.shell_parts(self.tri, nfield) if eng.uses_levels(self.fmt)
+                      else None)
         self.streams = [torch.cuda.Stream(device=eng.device) for _ in range(inflight)]
         self._bufs = []
         for s in self.streams:
@@ -50,27 +56,30 @@ class GridPipeline:
                     work=Workspace(eng.device),
                     dk=torch.empty((eng.MY, eng.MX, eng.MZ), dtype=torch.complex64,
                                    device=eng.device),
-                    shells=eng.alloc_shells(self.fmt, eng.nb)))
+                    shells=[eng.alloc_shells(self.fmt, eng.nb) for _ in range(nfield)]))
         self._next = 0
 
     @property
     def inflight(self):
         return len(self.streams)
 
-    def submit(self, field):
-        """Enqueue one FP32 N^3 grid on the next stream; returns its float64
-        device tensor of T sums, ready once that stream has run it (wait on
-        the stream or call synchronize())."""
+    def submit(self, *fields):
+        """Enqueue one grid (nfield FP32 N^3 fields: A, or A and B) on the
+        next stream; returns its float64 device tensor of T sums, ready once
+        that stream has run it (wait on the stream or call synchronize())."""
+        if len(fields) != self.nfield:
+            raise ValueError('%d field(s) per grid' % self.nfield)
         i = self._next
         self._next = (i + 1) % len(self.streams)
         s, b = self.streams[i], self._bufs[i]
         eng = self.eng
         with torch.cuda.stream(s):
-            dk = eng.forward(field, stream=s, work=b['work'], out=b['dk'])
-            eng.shells(dk, fmt=self.fmt, out=b['shells'], parts=self.parts, stream=s,
-                       work=b['work'])
+            for f, sh in zip(fields, b['shells']):
+                dk = eng.forward(f, stream=s, work=b['work'], out=b['dk'])
+                eng.shells(dk, fmt=self.fmt, out=sh, parts=self.parts, stream=s,
+                           work=b['work'])
             out = torch.empty(len(self.tri), dtype=torch.float64, device=eng.device)
-            eng.triples([b['shells']], self.tri, path=PATH_MFMA, scale=self.scale, out=out,
+            eng.triples(b['shells'], self.tri, path=PATH_MFMA, scale=self.scale, out=out,
                         stream=s, nwg=self.nwg, work=b['work'])
             if self.norm is not None:
                 out = out / self.norm
@@ -80,7 +89,7 @@ class GridPipeline:
         for s in self.streams:
             s.synchronize()
 
-    def tune(self, field, candidates=(256, 512, 768, 1024), grids=100, warmup=10):
+    def tune(self, *fields, candidates=(256, 512, 768, 1024), grids=100, warmup=10):
         """Time `grids` grids in flight per workgroup count of the pair kernel
         and keep the fastest: fewer workgroups leave CUs to the other streams'
         FFT passes, so the best count here differs from the one tuned for a
@@ -89,11 +98,11 @@ class GridPipeline:
         for nwg in candidates:
             self.nwg = int(nwg)
             for _ in range(warmup):
-                self.submit(field)
+                self.submit(*fields)
             self.synchronize()
             t0 = time.perf_counter()
             for _ in range(grids):
-                self.submit(field)
+                self.submit(*fields)
             self.synchronize()
             rates[int(nwg)] = grids / (time.perf_counter() - t0)
         self.nwg = max(rates, key=rates.get)

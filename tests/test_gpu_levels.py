@@ -342,3 +342,36 @@ def test_grid_pipeline_equals_one_at_a_time():
     assert np.array_equal(out.cpu().numpy(), one(fields[1], nwg=pipe.nwg))
     with pytest.raises(ValueError):
         GridPipeline(eng, tri, inflight=0)
+
+
+def test_grid_pipeline_cross_equals_one_at_a_time():
+    """GridPipeline with nfield=2 (<AAB>: each grid is two fields) returns the
+    sums of the engine's one-at-a-time cross path bit for bit, and refuses a
+    grid given with the wrong number of fields."""
+    from bskit_amd import _lib as L
+    from bskit_amd import bins
+    from bskit_amd.pipeline import GridPipeline
+    N, nb, box = 128, 10, 1000.0
+    lo, hi = _bins(nb, 2, box)
+    eng = _engine(N, box, lo, hi)
+    tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), 2).astype(np.int32)
+    pairs = [(torch.from_numpy(_field(N, 100 + 2 * i, np.float32)).cuda(),
+              torch.from_numpy(_field(N, 101 + 2 * i, np.float32)).cuda()) for i in range(4)]
+    sc = 1.0 / eng.ncell
+
+    def one(a, b):
+        sets = [eng.shells(eng.forward(f), fmt=L.FMT_BF16X2,
+                           out=eng.alloc_shells(L.FMT_BF16X2, eng.nb)) for f in (a, b)]
+        return eng.triples(sets, tri, scale=sc).cpu().numpy()
+
+    ref = [one(a, b) for a, b in pairs]
+    pipe = GridPipeline(eng, tri, inflight=3, scale=sc, nfield=2)
+    for _ in range(2):
+        outs = [pipe.submit(a, b) for a, b in pairs]
+        pipe.synchronize()
+        for i, o in enumerate(outs):
+            assert np.array_equal(o.cpu().numpy(), ref[i]), i
+    with pytest.raises(ValueError):
+        pipe.submit(pairs[0][0])
+    with pytest.raises(ValueError):
+        GridPipeline(eng, tri, nfield=3)
