@@ -112,6 +112,16 @@ BSK_DEV V get_u32(const V* base, unsigned e) {
                                      e * (unsigned)sizeof(V));
 }
 
+// The same load, non-temporal: for data read once (the input field), so
+// it does not displace cache lines that other kernels in flight reuse.
+template <typename T>
+BSK_DEV cplx<T> get_nt(const cplx<T>* base, unsigned e) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(
+      reinterpret_cast<const char*>(base) + e * (unsigned)sizeof(cplx<T>)));
+  return {v.x, v.y};
+}
+
 template <typename V>
 BSK_DEV void set_u32(V* base, unsigned e, V v) {
   *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + e * (unsigned)sizeof(V)) = v;
@@ -140,7 +150,7 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
   const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
   stage_in<T, P * NH, W>(
       [&](int e) {
-        return e < nr * NH ? get_u32(src, (unsigned)e) : cplx<T>{(T)0, (T)0};
+        return e < nr * NH ? get_nt(src, (unsigned)e) : cplx<T>{(T)0, (T)0};
       },
       [&](int e, cplx<T> v) {
         const int p = e / NH;
