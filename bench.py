@@ -462,7 +462,13 @@ def main():
                 'kernel_ms': grid_ms / launches, 'launches_per_grid': launches,
                 'algorithmic_bytes': alg_bytes / launches,
                 'useful_tflops': flops / t_k / 1e12, 'arith_intensity_flop_per_byte': ai,
-                'pairs': plan.npair, 'tiles': plan.ntile, 'items': plan.nitem}
+                'pairs': plan.npair, 'tiles': plan.ntile, 'items': plan.nitem,
+                # measured with counters: per K step a new tile costs each lane
+                # 8 products and 4 bf16 splits, an item 3 MFMAs, so the vector
+                # ALU binds before HBM or the matrix cores
+                'limiter': ('VALU: forming the split-bf16 pair products (%d tiles for %d '
+                            'items)' % (plan.ntile, plan.nitem)
+                            if kernel == 'k_triple_pairs' else None)}
 
     # the fused kernel reads each shell's inverse y output once: kz < mz_b
     # complex values of every (x, y) row
