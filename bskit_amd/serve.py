@@ -65,15 +65,21 @@ def _bins_key(kLU):
 class Server:
     """Admission + continuous batching of Jobs on one device."""
 
-    def __init__(self, device=None, hbm_budget=None, nstreams=4, poll_s=2e-5, assign='work'):
+    def __init__(self, device=None, hbm_budget=None, nstreams=4, poll_s=2e-5, assign='work',
+                 depth=None):
         """assign: the stream an arriving job is issued to -- 'work', the one
         with the least outstanding work (the footprints of its jobs in flight,
         which grow with the mesh and the shell set), so a small job does not
         queue behind a large one where another stream is nearly idle; or
-        'count', the one with the fewest jobs in flight."""
+        'count', the one with the fewest jobs in flight.  depth: at most this
+        many jobs in flight per stream (None: no limit); arrivals beyond it wait
+        in the host queue and go to the first stream with room."""
         if assign not in ('work', 'count'):
             raise ValueError("assign must be 'work' or 'count'")
+        if depth is not None and depth < 1:
+            raise ValueError('depth must be >= 1 or None')
         self.assign = assign
+        self.depth = depth
         self.device = require_device(device)
         free, _ = torch.cuda.mem_get_info(self.device)
         self.budget = int(hbm_budget) if hbm_budget else int(min(256e9, 0.85 * free))
@@ -190,9 +196,13 @@ class Server:
                 j = pending[0]
                 if inflight and used + j.footprint > self.budget:
                     break
-                pending.popleft()
                 load = work if self.assign == 'work' else per_stream
-                si = min(range(len(self.streams)), key=lambda i: (load[i], per_stream[i], i))
+                room = [i for i in range(len(self.streams))
+                        if self.depth is None or per_stream[i] < self.depth]
+                if not room:
+                    break
+                pending.popleft()
+                si = min(room, key=lambda i: (load[i], per_stream[i], i))
                 j.issued = time.perf_counter() - t0
                 ev = self._issue(j, si)
                 per_stream[si] += 1

@@ -532,10 +532,12 @@ def test_sublattice_duals_by_brute_force(S):
         assert dual == rule, (L, sorted(dual), sorted(rule))
 
 
-def test_server_rejects_unknown_stream_assignment():
+def test_server_rejects_unknown_stream_assignment_and_depth():
     """The serving loop's stream choice is 'work' (least outstanding
     footprint) or 'count' (fewest jobs in flight); anything else is refused
     before the device is touched."""
     from bskit_amd import serve
     with pytest.raises(ValueError):
         serve.Server(assign='random')
+    with pytest.raises(ValueError):
+        serve.Server(depth=0)
