@@ -17,11 +17,11 @@ CSRC = os.path.join(_HERE, 'csrc')
 OK, EINVAL, ENOMEM, EUNSUPPORTED, EHIP = 0, -1, -2, -3, -4
 DT_F32, DT_F64 = 0, 1
 FMT_F32, FMT_BF16X2, FMT_F64 = 0, 1, 2
-KIND_DELTA, KIND_UNIT, KIND_KMAG = 0, 1, 2
+KIND_DELTA, KIND_UNIT, KIND_KMAG, KIND_KPOW = 0, 1, 2, 3
 MODE_AUTO, MODE_AAB, MODE_ABC = 0, 1, 2
 STAGE_X, STAGE_YZ, STAGE_Y, STAGE_LEVELS = 1, 2, 4, 8
 LEVELS = 4
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 _c_int, _c_ll, _c_double, _vp = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 _c_size_p = ctypes.POINTER(ctypes.c_size_t)
@@ -44,6 +44,7 @@ _SIGS = {
     'bsk_plan_triangle_levels': [_vp, _c_int, _int_p, _int_p],
     'bsk_plan_set_variant': [_vp, _c_int, _c_int],
     'bsk_plan_set_compensation': [_vp, _c_int],
+    'bsk_plan_set_kpower': [_vp, _c_double],
     'bsk_forward_workspace': [_vp, _c_int, _c_size_p],
     'bsk_forward': [_vp, _c_int, _vp, _vp, _vp, _vp],
     'bsk_shells_workspace': [_vp, _c_int, _c_int, _c_int, _c_size_p],
@@ -77,7 +78,8 @@ _SIGS = {
                                 _vp, _vp, _vp],
     'bsk_triples_zfused_supported': [_vp, _vp, ctypes.POINTER(_c_int)],
     'bsk_triples_zfused': [_vp, _vp, _vp, _c_int, _c_int, _c_double, _vp, _vp, _vp],
-    'bsk_cic_paint': [_c_int, _c_double, _c_ll, _vp, _vp, _c_int, _vp],
+    'bsk_cic_workspace': [_c_int, _c_ll, _vp],
+    'bsk_cic_paint': [_c_int, _c_double, _c_ll, _vp, _vp, _c_int, _vp, _vp],
     'bsk_direct_triangles': [_c_int, _c_int, _vp, _c_int, _vp, _c_int, _c_int, _c_int, _vp, _vp,
                              _vp, _c_int, _c_int, _c_int, _c_int, _vp, _vp],
 }

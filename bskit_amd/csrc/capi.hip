@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <climits>
+#include <cmath>
 #include <queue>
 #include <vector>
 
@@ -40,6 +41,16 @@ struct bsk_plan {
   cplx<double>* tw_d = nullptr;
   cplx<float>* tws_f = nullptr;  // exp(-2 pi i t / S) when S != N
   cplx<double>* tws_d = nullptr;
+  // CIC window compensation tables, one per window mesh size used on this
+  // plan (kept until the plan is destroyed, so a launch in flight never
+  // reads a freed table); comp_f / comp_d point at the selected one
+  struct Comp {
+    int window;
+    float* f;
+    double* d;
+  };
+  std::vector<Comp> comps;
+  double kpower = 1.0;  // exponent p of the BSK_KIND_KPOW shells
   float* comp_f = nullptr;
   double* comp_d = nullptr;
 };
@@ -153,7 +164,8 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
   const T kf = (T)(2.0 * M_PI / p->box);
   int rc = BSK_OK;
   if (stages & BSK_STAGE_X) {
-    rc = bsk_inv_x<T>(N, p->variant[3], dk, d1, tw, g, p->d_lo, p->d_hi, shell0, nsh, kind, kf, s);
+    rc = bsk_inv_x<T>(N, p->variant[3], dk, d1, tw, g, p->d_lo, p->d_hi, shell0, nsh, kind, kf,
+                      (T)p->kpower, s);
     if (rc) return rc;
   }
   if (stages & BSK_STAGE_Y)  // y only: D2 stays in the workspace
@@ -306,8 +318,10 @@ int bsk_plan_destroy(bsk_plan* p) {
   (void)hipFree(p->tw_d);
   (void)hipFree(p->tws_f);
   (void)hipFree(p->tws_d);
-  (void)hipFree(p->comp_f);
-  (void)hipFree(p->comp_d);
+  for (auto& c : p->comps) {
+    (void)hipFree(c.f);
+    (void)hipFree(c.d);
+  }
   delete p;
   return BSK_OK;
 }
@@ -422,11 +436,15 @@ int bsk_plan_set_variant(bsk_plan* p, int pass, int variant) {
 
 int bsk_plan_set_compensation(bsk_plan* p, int nmesh_window) {
   if (int rc = check_plan(p)) return rc;
-  (void)hipFree(p->comp_f);
-  (void)hipFree(p->comp_d);
   p->comp_f = nullptr;
   p->comp_d = nullptr;
   if (nmesh_window <= 0) return BSK_OK;
+  for (const auto& c : p->comps)
+    if (c.window == nmesh_window) {
+      p->comp_f = c.f;
+      p->comp_d = c.d;
+      return BSK_OK;
+    }
   const int N = p->N;
   std::vector<float> cf(N + 1);
   std::vector<double> cd(N + 1);
@@ -435,10 +453,28 @@ int bsk_plan_set_compensation(bsk_plan* p, int nmesh_window) {
     cd[n + N / 2] = 1.0 / sqrt(1.0 - 2.0 / 3.0 * sn * sn);
     cf[n + N / 2] = (float)cd[n + N / 2];
   }
-  BSK_HIP_CHECK(hipMalloc(&p->comp_f, sizeof(float) * (N + 1)));
-  BSK_HIP_CHECK(hipMalloc(&p->comp_d, sizeof(double) * (N + 1)));
-  BSK_HIP_CHECK(hipMemcpy(p->comp_f, cf.data(), sizeof(float) * (N + 1), hipMemcpyHostToDevice));
-  BSK_HIP_CHECK(hipMemcpy(p->comp_d, cd.data(), sizeof(double) * (N + 1), hipMemcpyHostToDevice));
+  bsk_plan::Comp c{nmesh_window, nullptr, nullptr};
+  BSK_HIP_CHECK(hipMalloc(&c.f, sizeof(float) * (N + 1)));
+  if (hipMalloc(&c.d, sizeof(double) * (N + 1)) != hipSuccess) {
+    (void)hipFree(c.f);
+    bsk_set_error("bsk_plan_set_compensation: out of device memory");
+    return BSK_ENOMEM;
+  }
+  p->comps.push_back(c);
+  BSK_HIP_CHECK(hipMemcpy(c.f, cf.data(), sizeof(float) * (N + 1), hipMemcpyHostToDevice));
+  BSK_HIP_CHECK(hipMemcpy(c.d, cd.data(), sizeof(double) * (N + 1), hipMemcpyHostToDevice));
+  p->comp_f = c.f;
+  p->comp_d = c.d;
+  return BSK_OK;
+}
+
+int bsk_plan_set_kpower(bsk_plan* p, double power) {
+  if (int rc = check_plan(p)) return rc;
+  if (!std::isfinite(power)) {
+    bsk_set_error("bsk_plan_set_kpower: the power must be finite");
+    return BSK_EINVAL;
+  }
+  p->kpower = power;
   return BSK_OK;
 }
 
@@ -492,7 +528,7 @@ int bsk_shells(bsk_plan* p, int dtype, int kind, const void* dk, int shell0, int
       x0 + xslab > p->S || !work || stages < 1 || stages > 15 ||
       ((stages & BSK_STAGE_Y) && (stages & BSK_STAGE_YZ)) ||
       ((stages & BSK_STAGE_X) && kind == BSK_KIND_DELTA && !dk) ||
-      (kind < BSK_KIND_DELTA || kind > BSK_KIND_KMAG) ||
+      (kind < BSK_KIND_DELTA || kind > BSK_KIND_KPOW) ||
       (yz && (!out || out_cell0 < 0 || shell_stride < out_cell0 + slab_cells))) {
     bsk_set_error("bsk_shells: invalid argument");
     return BSK_EINVAL;
@@ -539,7 +575,7 @@ int bsk_shells_levels(bsk_plan* p, int dtype, int kind, const void* dk, int shel
   }
   if (nshell <= 0 || shell0 < 0 || shell0 + nshell > p->nbin || nshell > 65535 || !work ||
       !out || (fmt == BSK_FMT_BF16X2 && !out_lo) || (kind == BSK_KIND_DELTA && !dk) ||
-      kind < BSK_KIND_DELTA || kind > BSK_KIND_KMAG || p->S % 4 ||
+      kind < BSK_KIND_DELTA || kind > BSK_KIND_KPOW || p->S % 4 ||
       shell_stride < level_layout(p->S).stride) {
     bsk_set_error("bsk_shells_levels: invalid argument (S a multiple of 4, shell stride of at "
                   "least the bsk_plan_levels stride)");

@@ -128,24 +128,26 @@ template int bsk_fwd_x<double>(int, int, const cplx<double>*, cplx<double>*, con
 namespace {
 template <typename T, int N, int V>
 void inv_x_n(const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g, const int* lo,
-             const int* hi, int shell0, int nsh, int kind, T kf, hipStream_t s) {
+             const int* hi, int shell0, int nsh, int kind, T kf, T kp, hipStream_t s) {
   constexpr int P = pxy<T, N>();
   hipLaunchKernelGGL((k_inv_x_mask<T, N, V, P, kW>), dim3(cdiv(g.MZ, P), g.MY, nsh), dim3(kW),
-                     0, s, dk, out, tw, g, lo, hi, shell0, kind, kf);
+                     0, s, dk, out, tw, g, lo, hi, shell0, kind, kf, kp);
 }
 }  // namespace
 template <typename T>
 int bsk_inv_x(int N, int V, const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g,
-              const int* lo, const int* hi, int shell0, int nsh, int kind, T kf, hipStream_t s) {
-#define C(n) BSK_VCASE(n, (inv_x_n<T, n, 0>(dk, out, tw, g, lo, hi, shell0, nsh, kind, kf, s)), \
-                       (inv_x_n<T, n, 1>(dk, out, tw, g, lo, hi, shell0, nsh, kind, kf, s)))
+              const int* lo, const int* hi, int shell0, int nsh, int kind, T kf, T kp,
+              hipStream_t s) {
+#define C(n) BSK_VCASE(n, (inv_x_n<T, n, 0>(dk, out, tw, g, lo, hi, shell0, nsh, kind, kf, kp, s)), \
+                       (inv_x_n<T, n, 1>(dk, out, tw, g, lo, hi, shell0, nsh, kind, kf, kp, s)))
   BSK_SWITCH(T, N, V, C)
 #undef C
 }
 template int bsk_inv_x<float>(int, int, const cplx<float>*, cplx<float>*, const cplx<float>*, Geom,
-                              const int*, const int*, int, int, int, float, hipStream_t);
+                              const int*, const int*, int, int, int, float, float, hipStream_t);
 template int bsk_inv_x<double>(int, int, const cplx<double>*, cplx<double>*, const cplx<double>*,
-                               Geom, const int*, const int*, int, int, int, double, hipStream_t);
+                               Geom, const int*, const int*, int, int, int, double, double,
+                               hipStream_t);
 #endif
 
 #if BSK_PASS == 4

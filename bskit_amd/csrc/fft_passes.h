@@ -247,7 +247,7 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
 }
 
 // ------------------------------------------------- S1: masked inverse x FFT
-// grid (ceil(MZ/P), MY, nshell).  kind: delta / unit / |k|.  Workgroups whose
+// grid (ceil(MZ/P), MY, nshell).  kind: delta / unit / |k| / |k|^kp.  Workgroups whose
 // pencils all lie outside the shell's disk ny^2 + kz^2 <= n2hi write nothing.
 template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk,
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
                                                   const cplx<T>* __restrict__ twg, Geom g,
                                                   const int* __restrict__ n2lo,
                                                   const int* __restrict__ n2hi, int shell0,
-                                                  int kind, T kf) {
+                                                  int kind, T kf, T kp) {
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
@@ -281,8 +281,10 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
               v = get_u32(src, (unsigned)(rx * g.MZ + kz));
             else if (kind == BSK_KIND_UNIT)
               v.x = (T)1;
-            else
+            else if (kind == BSK_KIND_KMAG)
               v.x = kf * sqrt((T)n2);
+            else
+              v.x = pow(kf * sqrt((T)n2), kp);
           }
         }
         return v;

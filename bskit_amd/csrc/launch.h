@@ -25,7 +25,8 @@ int bsk_fwd_x(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, 
               const T* comp, hipStream_t s);
 template <typename T>
 int bsk_inv_x(int N, int V, const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g,
-              const int* lo, const int* hi, int shell0, int nsh, int kind, T kf, hipStream_t s);
+              const int* lo, const int* hi, int shell0, int nsh, int kind, T kf, T kp,
+              hipStream_t s);
 template <typename T>
 int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0,
               int xs, int nsh, const int* n2hi, int shell0, hipStream_t s);

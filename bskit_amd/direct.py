@@ -73,11 +73,7 @@ def direct_bispectrum(mesh, bin0, bin1, bin2, periodic=False, verbose=0, second_
     cubes = []
     for mm in meshes:
         f = mm.device_field(dev).to(torch.float64)
-        if mm.compensation_window:
-            eng.set_compensation(mm.compensation_window)
-        else:
-            eng.set_compensation(0)
-        cubes.append(eng.forward(f))
+        cubes.append(eng.forward(f, window=int(mm.compensation_window)))
     if len(cubes) == 1:
         ca = cb = cc = cubes[0]
     elif len(cubes) == 2:

@@ -318,6 +318,10 @@ class GridEngine:
         from . import tune
         return tune.tune(self, tri=tri, field=field, force=force, log=log)
 
+    def set_kpower(self, power):
+        """Exponent p of KIND_KPOW shells (|k|^p on the shell)."""
+        L.call('bsk_plan_set_kpower', self._plan, float(power))
+
     def set_variant(self, fft_pass, variant):
         L.call('bsk_plan_set_variant', self._plan, int(fft_pass), int(variant))
 
@@ -326,8 +330,15 @@ class GridEngine:
         return 2.0 * math.pi / self.box
 
     # --------------------------------------------------------------- forward
-    def forward(self, field, stream=None, work=None, out=None):
-        """delta(k) on the compact cube [MY, MX, MZ] (complex, 1/N^3 scaled)."""
+    def forward(self, field, stream=None, work=None, out=None, window=None):
+        """delta(k) on the compact cube [MY, MX, MZ] (complex, 1/N^3 scaled).
+
+        window: mesh size of the CIC window deconvolved in this transform
+        (0: none; None: the engine's current setting).  It is read when the
+        transform is enqueued, so fields with different windows share the
+        engine."""
+        if window is not None and int(window) != self.cic_window:
+            self.set_compensation(window)
         if field.device.type != 'cuda':
             raise ValueError('field must be a device tensor')
         if tuple(field.shape) != (self.N,) * 3:
@@ -701,14 +712,22 @@ class GridEngine:
         return nb * stride * per
 
 
-def cic_paint(positions, nmesh, box_size, to_contrast=True, out=None, stream=None):
-    """CIC-paint (npart, 3) float32 device positions onto an nmesh^3 mesh."""
+def cic_paint(positions, nmesh, box_size, to_contrast=True, out=None, stream=None, work=None):
+    """CIC-paint (npart, 3) float32 device positions onto an nmesh^3 mesh
+    (deterministic: fixed-point integer sums per node).  work: an optional
+    Workspace whose 'cic' buffer is reused between calls."""
     require_device()
     pos = positions.contiguous()
     if pos.dtype != torch.float32 or pos.dim() != 2 or pos.shape[1] != 3:
         raise ValueError('positions must be a (npart, 3) float32 tensor')
     if out is None:
         out = torch.empty((nmesh,) * 3, dtype=torch.float32, device=pos.device)
+    nbytes = L.size_out()
+    L.call('bsk_cic_workspace', int(nmesh), pos.shape[0], ctypes.byref(nbytes))
+    if work is None:
+        w = torch.empty(max(int(nbytes.value), 256), dtype=torch.uint8, device=pos.device)
+    else:
+        w = work.get('cic', nbytes.value)
     L.call('bsk_cic_paint', int(nmesh), float(box_size), pos.shape[0], L.ptr(pos), L.ptr(out),
-           1 if to_contrast else 0, stream_handle(stream))
+           1 if to_contrast else 0, L.ptr(w), stream_handle(stream))
     return out

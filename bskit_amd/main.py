@@ -10,9 +10,11 @@ gfx950 kernels:
   the largest bin edge (normalised 1/N^3 like pmesh's paint(mode='complex'));
 * all nb shell fields of a bin set come from one masked inverse-FFT launch
   sequence (replacing nb view/mask/paint(mode='real') calls);
-* the triangle sums run as split-BF16 MFMA GEMMs (default for FP32 input),
-  FP32 sweeps ('fp32') or FP64 sweeps ('fp64', default for FP64 input), with
-  a fixed-order FP64 final reduction;
+* the triangle sums run in the input's precision by default, as the
+  reference's numpy sums do (main.py:1875): FP32 sweeps ('fp32') for FP32
+  input, FP64 sweeps ('fp64') for FP64 input; precision='bf16x3' selects
+  the split-BF16 MFMA GEMMs (FP32 accumulation, relative error ~1e-5, the
+  fast path); every path ends in a fixed-order FP64 reduction;
 * N_tri and k-means come from FP64 unit and |k| shells (exact integer N_tri).
 
 Deliberate differences from the reference, each documented where it occurs:
@@ -44,7 +46,6 @@ __all__ = ['FFTBispectrum', 'k_field', 'number_field', 'compute_Nbin', 'compute_
            'PRECISIONS']
 
 PRECISIONS = ('bf16x3', 'fp32', 'fp64')
-_MFMA_MAX_SHELLS = 48
 
 # --------------------------------------------------------------------- engines
 _ENGINES = {}
@@ -55,21 +56,21 @@ def _bins_key(kLU):
     return tuple((float(a), float(b)) for a, b in np.asarray(kLU, dtype=np.float64))
 
 
-def get_engine(nmesh, box, kLU, device=None, cic_window=0, shell_mesh=None):
-    """Shared GridEngine per (Nmesh, BoxSize, bins, window, shell mesh) key (LRU).
+def get_engine(nmesh, box, kLU, device=None, shell_mesh=None):
+    """Shared GridEngine per (Nmesh, BoxSize, bins, shell mesh) key (LRU).
 
     shell_mesh None picks the smallest exact shell mesh; Nmesh keeps the
     shells on the input mesh (needed where the shell fields themselves are
-    returned).
+    returned).  The CIC window compensation is chosen per forward transform
+    (each field's own), not per engine.
     """
     dev = require_device(device)
-    key = (int(nmesh), float(box), _bins_key(kLU), str(dev), int(cic_window),
+    key = (int(nmesh), float(box), _bins_key(kLU), str(dev),
            None if shell_mesh is None else int(shell_mesh))
     eng = _ENGINES.pop(key, None)
     if eng is None:
         kLU = np.asarray(kLU, dtype=np.float64)
-        eng = GridEngine(nmesh, box, kLU[:, 0], kLU[:, 1], device=dev, cic_window=cic_window,
-                         shell_mesh=shell_mesh)
+        eng = GridEngine(nmesh, box, kLU[:, 0], kLU[:, 1], device=dev, shell_mesh=shell_mesh)
         eng.gridinfo_cache = {}
     _ENGINES[key] = eng
     while len(_ENGINES) > _ENGINE_LIMIT:
@@ -114,8 +115,9 @@ class FFTBispectrum:
 
     precision : {'bf16x3', 'fp32', 'fp64'}, optional
         triple-product arithmetic: split-BF16 MFMA with FP32 accumulation
-        (default for float32 input), FP32 sweeps, or FP64 (default for
-        float64 input).  Final reductions are FP64 in every case.
+        (the fast path), FP32 sweeps (default for float32 input, the
+        reference's own arithmetic) or FP64 (default for float64 input).
+        Final reductions are FP64 in every case.
     device : torch device, optional
     """
     logger = logging.getLogger('FFTBispectrum')
@@ -194,7 +196,7 @@ class FFTBispectrum:
         self._shells = {}
         if precision is None:
             any64 = any(m.dtype == torch.float64 for m in self._meshes())
-            precision = 'fp64' if any64 else 'bf16x3'
+            precision = 'fp64' if any64 else 'fp32'
         if precision not in PRECISIONS:
             raise ValueError('precision must be one of %s' % (PRECISIONS,))
         self.precision = precision
@@ -205,7 +207,7 @@ class FFTBispectrum:
 
     # ----------------------------------------------------------- state
     def __getstate__(self):
-        return dict(b=self.b, k_edges=self.k_edges, attrs=self.attrs)
+        return dict(b=self.b, k_edges=self.k_edges, attrs=self.attrs, precision=self.precision)
 
     def __setstate__(self, state):
         self.attrs = state['attrs']
@@ -216,7 +218,7 @@ class FFTBispectrum:
         self._dk = {}
         self._shells = {}
         self._device = None
-        self.precision = 'bf16x3'
+        self.precision = state.get('precision', 'fp32')
 
     def _meshes(self):
         return [m for m in (self.first, self.second, self.third) if m is not None]
@@ -279,27 +281,31 @@ class FFTBispectrum:
         return _unique_bins(self.k_edges)
 
     def _engine(self, kLU):
-        win = max([m.compensation_window for m in self._meshes()] + [0])
-        return get_engine(self._N, self._L, kLU, self._device, win)
+        return get_engine(self._N, self._L, kLU, self._device)
 
     def _shellsets(self, eng, kLU, tri):
         if not self.attrs.get('painted') or self._fields is None:
             raise RuntimeError('meshes are not painted (for_grid_info_only=True)')
         fmt = {'bf16x3': L.FMT_BF16X2, 'fp32': L.FMT_F32, 'fp64': L.FMT_F64}[self.precision]
         if fmt == L.FMT_BF16X2 and not GridEngine.mfma_supported(len(kLU), len(self._fields), tri):
-            warnings.warn('%d bins with %d fields exceed the MFMA kernel (48 shells, 32 for '
+            warnings.warn('%d bins with %d fields exceed the MFMA kernel (128 shells, 96 for '
                           '<ABC>): using the FP32 path' % (len(kLU), len(self._fields)))
             fmt = L.FMT_F32
         key = _bins_key(kLU)
         sets = []
+        meshes = self._meshes()
         for fi, f in enumerate(self._fields):
-            # one set of shells per field; keep the last bin set only
-            skey = (key, fi, fmt)
+            # one set of shells per field; keep the last bin set only.  Each
+            # mesh is deconvolved by its own CIC window (measure_bs_fast.py
+            # compensates each mesh it reads, :209-223), none for a plain
+            # array.
+            win = int(meshes[fi].compensation_window)
+            skey = (key, fi, fmt, win)
             s = self._shells.get(skey)
             if s is None:
                 for k in [k for k in self._shells if k[1] == fi]:
                     del self._shells[k]
-                dk = eng.forward(f)
+                dk = eng.forward(f, window=win)
                 s = eng.shells(dk, kind=L.KIND_DELTA, fmt=fmt,
                                dtype=torch.float32 if fmt != L.FMT_F64 else torch.float64)
                 self._shells[skey] = s
@@ -446,9 +452,11 @@ def _box_n(box_size, n_mesh):
     return float(b[0]), int(n[0])
 
 
-def _single_shell(box_size, n_mesh, kmin, kmax, kind):
+def _single_shell(box_size, n_mesh, kmin, kmax, kind, power=None):
     box, N = _box_n(box_size, n_mesh)
     eng = get_engine(N, box, [[kmin, kmax]], shell_mesh=N)
+    if power is not None:
+        eng.set_kpower(power)
     s = eng.shells(None, kind=kind, fmt=L.FMT_F64, dtype=torch.float64)
     return s.data[0].view(N, N, N)
 
@@ -456,14 +464,16 @@ def _single_shell(box_size, n_mesh, kmin, kmax, kind):
 def k_field(box_size, n_mesh, kmin, kmax, p):
     """c2r of |k|^p on the closed shell [kmin, kmax]; bskit/main.py:227-277.
 
-    Supported powers: p = 0 (the number field) and p = 1 (the k field used
-    for k-means).  Returns an FP64 device tensor of shape n_mesh.
+    Any real power p (the reference weights the shell by |k|**p at
+    :264-268); p = 0 is the number field.  Returns an FP64 device tensor of
+    shape n_mesh.
     """
+    p = float(p)
     if p == 0:
         return _single_shell(box_size, n_mesh, kmin, kmax, L.KIND_UNIT)
     if p == 1:
         return _single_shell(box_size, n_mesh, kmin, kmax, L.KIND_KMAG)
-    raise NotImplementedError('k_field supports p = 0 and p = 1')
+    return _single_shell(box_size, n_mesh, kmin, kmax, L.KIND_KPOW, power=p)
 
 
 def number_field(box_size, n_mesh, kmin, kmax):
@@ -576,10 +586,10 @@ def power_spectrum(mesh, k_edges, precision=None):
     box = float(m.attrs['BoxSize'][0])
     kLU = np.asarray(k_edges, dtype=np.float64).reshape(-1, 2)
     dev = require_device()
-    eng = get_engine(N, box, kLU, dev, int(m.compensation_window))
+    eng = get_engine(N, box, kLU, dev)
     f = m.device_field(dev)
     dtype = torch.float64 if precision == 'fp64' else (torch.float32 if precision else f.dtype)
-    return eng.power_spectrum(eng.forward(f.to(dtype)))
+    return eng.power_spectrum(eng.forward(f.to(dtype), window=int(m.compensation_window)))
 
 
 def pk_FFT(mesh, kmin, kmax):

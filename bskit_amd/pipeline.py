@@ -73,6 +73,12 @@ class GridPipeline:
         self._next = (i + 1) % len(self.streams)
         s, b = self.streams[i], self._bufs[i]
         eng = self.eng
+        # the fields may have just been written on the caller's stream: order
+        # this grid after that work, and keep the fields' memory from being
+        # reused while this stream still reads them
+        s.wait_stream(torch.cuda.current_stream(eng.device))
+        for f in fields:
+            f.record_stream(s)
         with torch.cuda.stream(s):
             for f, sh in zip(fields, b['shells']):
                 dk = eng.forward(f, stream=s, work=b['work'], out=b['dk'])

@@ -99,18 +99,36 @@ class Server:
             self.engines[key] = eng
         return eng
 
-    def footprint(self, eng, nfield):
-        """HBM bytes a job holds while in flight."""
+    def footprint(self, eng, nfield, tri=None):
+        """HBM bytes a job holds while in flight: its shells, spectra and
+        output, plus every workspace its kernels use (forward, shells, and
+        the MFMA partials at the workgroup count it runs with, or the plain
+        path's block sums).  Workspaces live in the stream's grow-only
+        buffers, which run() releases whenever the stream goes idle, so
+        counting them per job bounds what the jobs in flight hold."""
         nb = eng.nb
         sz = L.size_out()
         L.call('bsk_forward_workspace', eng._plan, L.DT_F32, ctypes.byref(sz))
         fwd = sz.value
         L.call('bsk_shells_workspace', eng._plan, L.DT_F32, nb, eng.S, ctypes.byref(sz))
         shw = sz.value
-        fmt = L.FMT_BF16X2 if GridEngine.mfma_supported(nb, nfield, None) else L.FMT_F32
+        mfma = GridEngine.mfma_supported(nb, nfield, None)
+        fmt = L.FMT_BF16X2 if mfma else L.FMT_F32
         shells = nfield * eng.shell_bytes(fmt)
         spectra = nfield * eng.MX * eng.MY * eng.MZ * 8
-        return int(fwd + shw + shells + spectra)
+        trw = 0
+        ntri = 0 if tri is None else len(tri)
+        if ntri and mfma:
+            mode = {1: L.MODE_AUTO, 2: L.MODE_AAB}[nfield]
+            plan = eng.pair_plan(tri, nb, mode, levels=eng.uses_levels())
+            L.call('bsk_triples_mfma_workspace', plan.handle, eng.mfma_nwg(), ctypes.byref(sz))
+            trw = sz.value
+        elif ntri:
+            L.call('bsk_triples_plain_workspace', min(ntri, 65535), eng.plain_nchunk(),
+                   ctypes.byref(sz))
+            trw = sz.value
+        out = 8 * ntri
+        return int(fwd + shw + shells + spectra + trw + out)
 
     def prepare(self, job):
         """Engine, triangles, path and footprint of a job (host work, untimed)."""
@@ -128,7 +146,7 @@ class Server:
             if eng.uses_levels():
                 eng.shell_parts(tri, job.nfield)
         eng.triangles_device(tri)
-        job.footprint = self.footprint(eng, job.nfield)
+        job.footprint = self.footprint(eng, job.nfield, tri)
         if job.footprint > self.budget:
             raise MemoryError('job %d needs %.1f GB, above the %.1f GB budget'
                               % (job.jid, job.footprint / 1e9, self.budget / 1e9))
@@ -141,6 +159,11 @@ class Server:
         s = self.streams[si]
         w = self.works[si]
         fmt = L.FMT_BF16X2 if job.path == PATH_MFMA else L.FMT_F32
+        # order the job after whatever wrote its fields on the caller's
+        # stream, and keep their memory alive until this stream is done
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        for f in job.fields[:job.nfield]:
+            f.record_stream(s)
         with torch.cuda.stream(s):
             sets = []
             for f in job.fields[:job.nfield]:
@@ -189,6 +212,10 @@ class Server:
                     used -= j.footprint
                     per_stream[j.stream] -= 1
                     work[j.stream] -= j.footprint
+                    if per_stream[j.stream] == 0:
+                        # idle stream: its workspaces are no longer counted
+                        # in any footprint, so give them back
+                        self.works[j.stream].release()
                 else:
                     keep.append((j, ev))
             inflight = keep

@@ -117,24 +117,6 @@ def zeldovich_particles(n_side, box, seed, sigma_disp=5.0, amp=P_AMP, k0=P_K0, i
     return pos
 
 
-def cic_paint_numpy(pos, N, box):
-    """CPU cloud-in-cell reference of the device kernel (node i at i*box/N)."""
-    u = np.asarray(pos, dtype=np.float64) * (N / box)
-    fl = np.floor(u)
-    f = u - fl
-    i0 = np.mod(fl.astype(np.int64), N)
-    i1 = (i0 + 1) % N
-    grid = np.zeros(N * N * N)
-    for c in range(8):
-        ix = i1[:, 0] if c & 4 else i0[:, 0]
-        iy = i1[:, 1] if c & 2 else i0[:, 1]
-        iz = i1[:, 2] if c & 1 else i0[:, 2]
-        w = ((f[:, 0] if c & 4 else 1 - f[:, 0]) * (f[:, 1] if c & 2 else 1 - f[:, 1]) *
-             (f[:, 2] if c & 1 else 1 - f[:, 2]))
-        grid += np.bincount((ix * N + iy) * N + iz, weights=w, minlength=N ** 3)
-    return grid.reshape(N, N, N)
-
-
 # ---------------------------------------------------------------------------
 # Device generators (serving and batch workloads).  Same power spectrum and
 # normalisation as above, but the white noise comes from torch's seeded

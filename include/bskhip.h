@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define BSK_ABI_VERSION 5
+#define BSK_ABI_VERSION 6
 
 /* status codes */
 #define BSK_OK 0
@@ -54,6 +54,7 @@ extern "C" {
 #define BSK_KIND_DELTA 0 /* mask * delta(k)          (main.py:1846-1861)      */
 #define BSK_KIND_UNIT 1  /* mask                     (number_field :280-329)  */
 #define BSK_KIND_KMAG 2  /* |k| * mask               (k_field p=1 :227-277)   */
+#define BSK_KIND_KPOW 3  /* |k|^p * mask, p from bsk_plan_set_kpower (k_field :264-268) */
 /* triple-product modes (field roles of main.py:624-640) */
 #define BSK_MODE_AUTO 0 /* <AAA>: triangles taken with i >= j >= k      */
 #define BSK_MODE_AAB 1  /* <AAB>: symmetric in the first two sides      */
@@ -126,8 +127,18 @@ int bsk_plan_set_variant(bsk_plan* plan, int pass, int variant);
  * divided by prod_i (1 - 2/3 sin^2(pi n_i / nmesh_window))^(1/2).  Replaces
  * CompensateCICShotnoiseNgrid (scripts/measure/measure_bs_fast.py:45-57,
  * applied :222) and nbodykit's compensated CIC mesh (:211-217).
- * nmesh_window <= 0 disables it. */
+ * nmesh_window <= 0 disables it.  The window is read when bsk_forward is
+ * called, so fields with different windows (or none) can share one plan:
+ * select the window before each forward call.  The table of every window
+ * used stays allocated until bsk_plan_destroy, so launches in flight never
+ * read a released table. */
 int bsk_plan_set_compensation(bsk_plan* plan, int nmesh_window);
+/* Exponent p of the BSK_KIND_KPOW shells (|k|^p on the shell, |k| in box
+ * units of 2 pi / box_size per k_f), default 1.  Replaces the power
+ * argument of k_field (bskit/main.py:227-277, the weight at :264-268).
+ * A shell holding k = 0 with p < 0 gets an infinite mode, as numpy's
+ * 0.0 ** p gives there. */
+int bsk_plan_set_kpower(bsk_plan* plan, double power);
 
 /* Forward transform of a real N^3 field (C order, z fastest) into the
  * compact cube delta(k) [my][mx][mz] (complex, interleaved), normalised by
@@ -302,12 +313,16 @@ int bsk_triples_zfused(bsk_plan* plan, const bsk_pairs* pairs, const void* shell
 /* Cloud-in-cell assignment of npart particles (pos: npart x 3 float, box
  * units, C order) to the periodic mesh grid (nmesh^3 float, overwritten).
  * Mesh node i sits at i*box/nmesh.  With to_contrast the result is
- * delta = rho/rho_mean - 1.  Replaces nbodykit's
+ * delta = rho/rho_mean - 1.  Every corner weight is rounded once to 32.32
+ * fixed point and summed as integers, so the mesh is the same bits whatever
+ * order the particles are processed in (deterministic).  Replaces nbodykit's
  * CatalogSource.to_mesh(window='cic') painting as called by
  * scripts/measure/measure_bs_fast.py:209-217 (the compensation is
- * bsk_plan_set_compensation). */
+ * bsk_plan_set_compensation).  work: bsk_cic_workspace() bytes; at most
+ * 2^32 - 1 particles. */
+int bsk_cic_workspace(int nmesh, long long npart, size_t* bytes);
 int bsk_cic_paint(int nmesh, double box_size, long long npart, const float* pos, float* grid,
-                  int to_contrast, void* stream);
+                  int to_contrast, void* work, void* stream);
 
 /* Direct enumeration of closed triangles from explicit mode lists (int32
  * (nx,ny,nz) triples, components in [-N/2, N/2)): for every k1 in A and k2

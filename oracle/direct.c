@@ -131,6 +131,93 @@ int oracle_direct(int N, int nbin, const long long *n2lo, const long long *n2hi,
     return 0;
 }
 
+/* Exact periodic triangle counts for many triangles at once, by direct
+ * enumeration of mode pairs (no FFT), for shell sets that never reach the
+ * Nyquist plane (hi < (N/2)^2).  For each distinct side pair (a, b) of the
+ * list every k1 of shell a is paired with every k2 of shell b and the
+ * pairs are histogrammed by |n3|^2, n3 = -(k1 + k2) wrapped into
+ * [-N/2, N/2); a triangle (a, b, c) then counts the pairs whose |n3|^2 lies
+ * in [n2lo[c], n2hi[c]].  Shells and |wrap(.)|^2 are invariant under the 48
+ * signed permutations of the axes, so k1 runs over one representative
+ * x >= y >= z >= 0 per orbit, weighted by the orbit size.  Integer counts;
+ * equal to oracle_direct's (checked on small grids). */
+static int orbit_size(int x, int y, int z) {
+    int nz = (x != 0) + (y != 0) + (z != 0);
+    int perms = (x == y && y == z) ? 1 : ((x == y || y == z) ? 3 : 6);
+    return perms << nz;
+}
+
+int oracle_count_sym(int N, int nbin, const long long *n2lo, const long long *n2hi,
+                     int ntri, const int *tri, long long *count) {
+    if (N <= 0 || (N & 1) || nbin <= 0 || ntri < 0) return -1;
+    const long long nyq2 = (long long)(N / 2) * (N / 2);
+    for (int b = 0; b < nbin; b++)
+        if (n2hi[b] >= nyq2) return -3;
+    for (int t = 0; t < ntri; t++)
+        for (int s = 0; s < 3; s++)
+            if (tri[3 * t + s] < 0 || tri[3 * t + s] >= nbin) return -1;
+    shell_t *sh = (shell_t *)calloc(nbin, sizeof(shell_t));
+    if (!sh) return -2;
+    for (int b = 0; b < nbin; b++)
+        if (build_shell(N, n2lo[b], n2hi[b], &sh[b])) return -2;
+    /* distinct (a, b) pairs of the list */
+    int *pa = (int *)malloc(sizeof(int) * (ntri + 1)), *pb = (int *)malloc(sizeof(int) * (ntri + 1));
+    char *seen = (char *)calloc((size_t)nbin * nbin, 1);
+    int npair = 0;
+    for (int t = 0; t < ntri; t++) {
+        int a = tri[3 * t], b = tri[3 * t + 1];
+        if (!seen[(size_t)a * nbin + b]) {
+            seen[(size_t)a * nbin + b] = 1;
+            pa[npair] = a;
+            pb[npair] = b;
+            npair++;
+        }
+    }
+    const long long hmax = 3 * nyq2 + 1;
+    long long **hist = (long long **)calloc(npair ? npair : 1, sizeof(long long *));
+    int rc = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int q = 0; q < npair; q++) {
+        long long *h = (long long *)calloc(hmax, sizeof(long long));
+        if (!h) {
+            rc = -2;
+            continue;
+        }
+        const shell_t *A = &sh[pa[q]], *B = &sh[pb[q]];
+        for (int p = 0; p < A->n; p++) {
+            int x = A->ix[p], y = A->iy[p], z = A->iz[p];
+            if (!(x >= y && y >= z && z >= 0)) continue;
+            long long w = orbit_size(x, y, z);
+            for (int r = 0; r < B->n; r++) {
+                int u = wrap(-(x + B->ix[r]), N);
+                int v = wrap(-(y + B->iy[r]), N);
+                int k = wrap(-(z + B->iz[r]), N);
+                h[(long long)u * u + (long long)v * v + (long long)k * k] += w;
+            }
+        }
+        hist[q] = h;
+    }
+    if (!rc) {
+        for (int t = 0; t < ntri; t++) {
+            int a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+            int q = 0;
+            while (pa[q] != a || pb[q] != b) q++;
+            long long lo = n2lo[c] > 0 ? n2lo[c] : 0, hi = n2hi[c] < hmax - 1 ? n2hi[c] : hmax - 1;
+            long long cnt = 0;
+            for (long long n2 = lo; n2 <= hi; n2++) cnt += hist[q][n2];
+            count[t] = cnt;
+        }
+    }
+    for (int q = 0; q < npair; q++) free(hist[q]);
+    free(hist);
+    free(seen);
+    free(pa);
+    free(pb);
+    for (int b = 0; b < nbin; b++) free_shell(&sh[b]);
+    free(sh);
+    return rc;
+}
+
 /* Number of members of each shell (for sanity checks). */
 int oracle_shell_sizes(int N, int nbin, const long long *n2lo, const long long *n2hi,
                        long long *sizes) {

@@ -52,16 +52,22 @@ def n2_half(N):
     return f[:, None, None] ** 2 + f[None, :, None] ** 2 + fz[None, None, :] ** 2
 
 
-def forward(field):
-    """delta(k) = (1/N^3) sum_x delta(x) e^{-ik.x} on the half grid, FP64."""
+def forward(field, workers=None):
+    """delta(k) = (1/N^3) sum_x delta(x) e^{-ik.x} on the half grid, FP64
+    (workers: host threads of scipy's FFT for large meshes)."""
     field = np.asarray(field, dtype=np.float64)
+    if workers:
+        import scipy.fft as sfft
+        return sfft.rfftn(field, workers=workers) / field.size
     return np.fft.rfftn(field) / field.size
 
 
-def shell_field(dk_half, n2, lo, hi, N, kind='delta', kf=1.0):
+def shell_field(dk_half, n2, lo, hi, N, kind='delta', kf=1.0, power=1.0, workers=None):
     """Unnormalised c2r of the masked half spectrum.
 
-    kind 'delta': mask * delta(k); 'unit': mask; 'kmag': |k| * mask.
+    kind 'delta': mask * delta(k); 'unit': mask; 'kmag': |k| * mask;
+    'kpow': |k|**power * mask (k_field, bskit/main.py:227-277, weight
+    kk**p at :264-268).
     """
     m = (n2 >= lo) & (n2 <= hi)
     if kind == 'delta':
@@ -70,17 +76,25 @@ def shell_field(dk_half, n2, lo, hi, N, kind='delta', kf=1.0):
         spec = m.astype(np.complex128)
     elif kind == 'kmag':
         spec = np.where(m, kf * np.sqrt(n2.astype(np.float64)), 0.0).astype(np.complex128)
+    elif kind == 'kpow':
+        kk = kf * np.sqrt(n2.astype(np.float64))
+        with np.errstate(divide='ignore'):
+            spec = np.where(m, kk ** power, 0.0).astype(np.complex128)
     else:
         raise ValueError(kind)
+    if workers:
+        import scipy.fft as sfft
+        return sfft.irfftn(spec, s=(N, N, N), axes=(0, 1, 2), workers=workers) * float(N) ** 3
     return np.fft.irfftn(spec, s=(N, N, N), axes=(0, 1, 2)) * float(N) ** 3
 
 
-def shells(dk_half, N, lo, hi, kind='delta', kf=1.0):
+def shells(dk_half, N, lo, hi, kind='delta', kf=1.0, workers=None):
     n2 = n2_half(N)
-    return [shell_field(dk_half, n2, lo[b], hi[b], N, kind, kf) for b in range(len(lo))]
+    return [shell_field(dk_half, n2, lo[b], hi[b], N, kind, kf, workers=workers)
+            for b in range(len(lo))]
 
 
-def shells_on_mesh(dk_half, N, S, lo, hi, kind='delta', kf=1.0):
+def shells_on_mesh(dk_half, N, S, lo, hi, kind='delta', kf=1.0, workers=None):
     """The same masked spectra as ``shells`` evaluated on an S^3 mesh.
 
     The modes of the N-mesh half spectrum with signed components
@@ -91,7 +105,7 @@ def shells_on_mesh(dk_half, N, S, lo, hi, kind='delta', kf=1.0):
     At S == N this is ``shells`` itself.
     """
     if S == N:
-        return shells(dk_half, N, lo, hi, kind, kf)
+        return shells(dk_half, N, lo, hi, kind, kf, workers=workers)
     f = np.fft.fftfreq(S, 1.0 / S).astype(np.int64)
     n2 = n2_half(S)
     for b in range(len(lo)):
@@ -101,7 +115,8 @@ def shells_on_mesh(dk_half, N, S, lo, hi, kind='delta', kf=1.0):
     if dk_half is not None:
         ix = np.mod(f, N)
         sub = dk_half[ix][:, ix][:, :, :S // 2 + 1]
-    return [shell_field(sub, n2, lo[b], hi[b], S, kind, kf) for b in range(len(lo))]
+    return [shell_field(sub, n2, lo[b], hi[b], S, kind, kf, workers=workers)
+            for b in range(len(lo))]
 
 
 def triple_sums(fa, fb, fc, tri):
@@ -172,3 +187,35 @@ def pk_fft(field, box, kmin, kmax):
     d = np.fft.irfftn(np.where(m, forward(field), 0.0), s=(N, N, N), axes=(0, 1, 2)) * s
     p = np.sum(d * d) * float(box) ** 3 / s / nbin
     return p, nbin, kmean
+
+
+def cic_paint(pos, N, box):
+    """Cloud-in-cell density (particles per node) in FP64: node i at
+    i*box/N, a particle at mesh coordinate u = x N / box adds (1 - f) to
+    node floor(u) and f to floor(u)+1 per axis (f = u - floor(u)), indices
+    wrapped mod N.  nbodykit's CIC window as used through
+    to_mesh(window='cic') by scripts/measure/measure_bs_fast.py:209-217."""
+    u = np.asarray(pos, dtype=np.float64) * (N / box)
+    fl = np.floor(u)
+    f = u - fl
+    i0 = np.mod(fl.astype(np.int64), N)
+    i1 = (i0 + 1) % N
+    grid = np.zeros(N * N * N)
+    for c in range(8):
+        ix = i1[:, 0] if c & 4 else i0[:, 0]
+        iy = i1[:, 1] if c & 2 else i0[:, 1]
+        iz = i1[:, 2] if c & 1 else i0[:, 2]
+        w = ((f[:, 0] if c & 4 else 1 - f[:, 0]) * (f[:, 1] if c & 2 else 1 - f[:, 1]) *
+             (f[:, 2] if c & 1 else 1 - f[:, 2]))
+        grid += np.bincount((ix * N + iy) * N + iz, weights=w, minlength=N ** 3)
+    return grid.reshape(N, N, N)
+
+
+def cic_compensation(n, nmesh, nmesh_cic):
+    """Per-axis factor CompensateCICShotnoiseNgrid multiplies delta(k) by
+    (scripts/measure/measure_bs_fast.py:45-57): with the circular frequency
+    w = 2 pi n / nmesh of integer frequency n, 1 / (1 - 2/3 sin^2(w nmesh /
+    (2 nmesh_cic)))^(1/2); the factor of a mode is the product over its
+    three axes."""
+    w = 2.0 * np.pi * np.asarray(n, dtype=np.float64) / nmesh
+    return 1.0 / (1.0 - 2.0 / 3.0 * np.sin(0.5 * w * nmesh / nmesh_cic) ** 2) ** 0.5

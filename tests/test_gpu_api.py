@@ -166,16 +166,6 @@ def test_k_edges_input_and_units():
     assert np.allclose(r3['B'], r2['B'] * u ** 6, rtol=1e-12)
 
 
-def test_particles_cic_matches_numpy():
-    from bskit_amd.engine import cic_paint
-    N, L = 64, 1000.
-    pos = synth.zeldovich_particles(32, L, seed=3, sigma_disp=5.0)
-    got = cic_paint(torch.from_numpy(pos).cuda(), N, L, to_contrast=False).cpu().numpy()
-    ref = synth.cic_paint_numpy(pos, N, L)
-    assert abs(got.sum() - len(pos)) < 1e-2
-    assert np.abs(got - ref).max() < 2e-5
-
-
 def test_bk_binned_matches_fft_on_low_bins():
     """Direct enumeration equals the FFT estimator where periodic images cannot close."""
     N, L = 32, 1000.

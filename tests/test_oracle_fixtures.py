@@ -81,6 +81,29 @@ def test_direct_counts_gridinfo_fixture_512(golden_dir):
     assert np.abs(kbar / ref[:, 1:4] - 1).max() < 5e-7
 
 
+def test_symmetric_counter_matches_enumeration_and_fixtures(golden_dir):
+    """The orbit-weighted pair counter (count_triangles) equals the plain
+    enumeration on small grids (all-triangle and random cross lists, side
+    orders permuted) and reproduces the N_tri column of the 512^3 fixture."""
+    rng = np.random.default_rng(4)
+    for N, nb, w in ((16, 5, 1), (32, 6, 2), (48, 8, 2), (24, 4, 1.5)):
+        kf = est.kf_of(1000.)
+        kl = kf * (1 + w * np.arange(nb))
+        lo, hi = est.shell_limits(kl, kl + w * kf, kf)
+        idx = ob.all_triangles(kf, kf + nb * w * kf + w * kf / 2, w * kf, 1, return_indices=True)
+        for tri in (idx, rng.integers(0, nb, size=(60, 3))):
+            a, _, _ = direct.enumerate_triangles(N, lo, hi, tri)
+            assert np.array_equal(direct.count_triangles(N, lo, hi, tri), a)
+    ref = _load(golden_dir, 'Lbox1000_512_kf_3kf_3lowkbins.dat')
+    e = ob.bin_edges(FIX['kmin'], FIX['kmax'], FIX['dk'], FIX['num_lowk_bins'], FIX['dk_high'])
+    idx = ob.all_triangles(FIX['kmin'], FIX['kmax'], FIX['dk'], 1, FIX['num_lowk_bins'],
+                           FIX['dk_high'], return_indices=True)
+    lo, hi = est.shell_limits(e[:, 0], e[:, 1], est.kf_of(1000.))
+    assert np.array_equal(direct.count_triangles(512, lo, hi, idx), ref[:, 10].astype(np.int64))
+    with pytest.raises(RuntimeError):  # a shell on the Nyquist plane is refused
+        direct.count_triangles(16, [60], [64], [[0, 0, 0]])
+
+
 def test_direct_counts_gridinfo_fixture_256(golden_dir):
     """Block 1 of the 256^3 (L=500) subbox fixture, including empty bins."""
     ref = _load(golden_dir, 'Lbox1000_512_kf_3kf_3lowkbins_subbox0_block1.dat')
