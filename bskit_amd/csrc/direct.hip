@@ -3,8 +3,9 @@
 // Replaces the O(M^3) Python loop of bk_binned (bskit/main.py:834-1004):
 // for every k1 in list A and k2 in list B the third side k3 = -(k1+k2) is
 // formed (wrapped into [-N/2, N/2) when periodic, else kept as is and
-// rejected when it leaves the grid) and accepted when its integer |n|^2 lies
-// in [lo_c, hi_c].  Each accepted triple adds 1 to the count, |k1|,|k2|,|k3|
+// rejected unless it is a mode the reference collects: the stored half grid
+// x, y in [-N/2, N/2), 0 <= z <= N/2, or a conjugate -n of a mode with
+// 0 < n_z < N/2) and accepted when its integer |n|^2 lies in [lo_c, hi_c].  Each accepted triple adds 1 to the count, |k1|,|k2|,|k3|
 // (k_f units) to the k sums and dA(k1) dB(k2) dC(k3) to the complex sum.
 // delta values come from a compact half cube [ry][rx][kz] (kz >= 0) with
 // delta(-k) = conj(delta(k)).  Per-workgroup partials (6 doubles) are written
@@ -62,8 +63,13 @@ __global__ __launch_bounds__(kW) void k_direct(const int* __restrict__ ma, int n
       y = wrapc(y, N);
       z = wrapc(z, N);
     } else {
+      // k3 must be one of the collected modes: the stored half grid (x, y in
+      // [-h, h), 0 <= z <= h) or the conjugate of a mode of Hermitian weight
+      // 2 (0 < -z < h, so x, y in (-h, h])
       const int h = N / 2;
-      if (x < -h || x >= h || y < -h || y >= h || z < -h || z >= h) continue;
+      const bool in = z >= 0 ? (z <= h && x >= -h && x < h && y >= -h && y < h)
+                             : (z > -h && x > -h && x <= h && y > -h && y <= h);
+      if (!in) continue;
     }
     const int n2 = x * x + y * y + z * z;
     if (n2 < lo_c || n2 > hi_c) continue;

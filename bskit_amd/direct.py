@@ -1,12 +1,16 @@
 """Direct triangle enumeration on the GPU (bk_binned of the reference).
 
 bskit/main.py:834-1004 collects the modes of each bin with np.digitize on
-|k|^2, i.e. kmin^2 <= |k|^2 < kmax^2 (half-open), and accepts a triple when
-|k1+k2+k3|^2 < 1e-4 k_f -- exact closure on the integer lattice.  Here the
-mode lists are built on the host from integer |n|^2 limits with the same
-half-open semantics and the pair loop runs in libbskhip (bsk_direct_triangles).
-With periodic=True the closure is taken modulo N, which is what the FFT
-estimator counts; that form gives exact N_tri checks of the fast path.
+|k|^2, i.e. kmin^2 <= |k|^2 < kmax^2 (half-open), from the half grid of the
+complex field plus the conjugate of every mode of Hermitian weight 2
+(:936-941), and accepts a triple when |k1+k2+k3|^2 < 1e-4 k_f -- exact
+closure on the integer lattice, with the third mode one of the collected
+modes of its bin.  Here the mode lists are built on the host from integer
+|n|^2 limits with the same half-open semantics and the same mode sets
+(``reference_modes``), and the pair loop runs in libbskhip
+(bsk_direct_triangles).  With periodic=True the closure is taken modulo N
+over the FFT grid's modes, which is what the FFT estimator counts; that form
+gives exact N_tri checks of the fast path.
 """
 import ctypes
 import math
@@ -28,6 +32,25 @@ def half_open_limits(kmin, kmax, kf):
     lo = int(math.ceil(a * (1 - TIE_RTOL))) if a > 0 else 0
     hi = int(math.ceil(b * (1 - TIE_RTOL))) - 1
     return lo, hi
+
+
+def reference_modes(N, lo, hi):
+    """(M, 3) int32 lattice vectors of the modes bk_binned collects for a bin
+    (bskit/main.py:887-941): every stored mode of the half grid (n_x, n_y in
+    [-N/2, N/2), 0 <= n_z <= N/2) with lo <= |n|^2 <= hi, plus the conjugate
+    -n of each mode of Hermitian weight 2 (0 < n_z < N/2); the planes n_z = 0
+    and n_z = N/2 are taken once.  So for n_z < 0 the x and y components run
+    over (-N/2, N/2]."""
+    if hi < 0:
+        return np.zeros((0, 3), dtype=np.int32)
+    h = N // 2
+    r = np.arange(-h, h + 1)
+    x, y, z = np.meshgrid(r, r, r, indexing='ij')
+    n2 = x * x + y * y + z * z
+    up = (z >= 0) & (x < h) & (y < h)                # stored half grid
+    down = (z < 0) & (z > -h) & (x > -h) & (y > -h)  # conjugates of weight-2 modes
+    sel = (n2 >= lo) & (n2 <= hi) & (up | down)
+    return np.ascontiguousarray(np.stack([x[sel], y[sel], z[sel]], axis=1).astype(np.int32))
 
 
 def shell_modes(N, lo, hi):
@@ -80,8 +103,11 @@ def direct_bispectrum(mesh, bin0, bin1, bin2, periodic=False, verbose=0, second_
         ca, cb, cc = cubes[0], cubes[0], cubes[1]
     else:
         ca, cb, cc = cubes
-    ma = torch.from_numpy(shell_modes(N, *lims[0])).to(dev)
-    mb = torch.from_numpy(shell_modes(N, *lims[1])).to(dev)
+    # open closure: the reference's own mode sets (half grid plus conjugates);
+    # periodic closure: the FFT grid's modes, k3 taken modulo N
+    modes = shell_modes if periodic else reference_modes
+    ma = torch.from_numpy(modes(N, *lims[0])).to(dev)
+    mb = torch.from_numpy(modes(N, *lims[1])).to(dev)
     nblocks = 1024
     part = torch.zeros((nblocks, 6), dtype=torch.float64, device=dev)
     L.call('bsk_direct_triangles', N, 1 if periodic else 0, L.ptr(ma), ma.shape[0], L.ptr(mb),

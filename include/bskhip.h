@@ -325,9 +325,12 @@ int bsk_cic_paint(int nmesh, double box_size, long long npart, const float* pos,
                   int to_contrast, void* work, void* stream);
 
 /* Direct enumeration of closed triangles from explicit mode lists (int32
- * (nx,ny,nz) triples, components in [-N/2, N/2)): for every k1 in A and k2
- * in B, k3 = -(k1+k2) (wrapped mod N when periodic, else rejected when off
- * the grid) is accepted when n2lo_c <= |n3|^2 <= n2hi_c.  With dk_a/b/c
+ * (nx,ny,nz) triples): for every k1 in A and k2 in B, k3 = -(k1+k2) is
+ * accepted when n2lo_c <= |n3|^2 <= n2hi_c and, periodic: after wrapping
+ * into [-N/2, N/2) (the lists then hold components in [-N/2, N/2)); open:
+ * when k3 is a mode bk_binned collects, the stored half grid (n_x, n_y in
+ * [-N/2, N/2), 0 <= n_z <= N/2) or the conjugate of a mode with
+ * 0 < n_z < N/2 (the lists then hold such modes).  With dk_a/b/c
  * (FP64 compact cubes from bsk_forward, geometry mx, mz, nlo) the complex
  * sum of dA(k1) dB(k2) dC(k3) is formed too.  partial: nblocks x 6 doubles
  * (count, sum|k1|, sum|k2|, sum|k3| in k_f units, Re, Im) to be summed by

@@ -177,3 +177,30 @@ def test_bk_binned_matches_fft_on_low_bins():
     assert nt == ntf
     assert abs(B.real / Bf - 1) < 1e-10 and abs(B.imag) < 1e-8 * abs(B.real)
     assert abs(k1 / kf1 - 1) < 1e-10
+
+
+@pytest.mark.parametrize('N,bins_n2', [
+    (16, [(21.5, 40.5), (27.5, 50.5), (37.5, 68.5)]),   # above N/3, reaching the Nyquist plane
+    (16, [(24.5, 48.5)] * 3),
+    (24, [(60.5, 140.5), (80.5, 150.5), (100.5, 170.5)])])
+def test_bk_binned_open_closure_matches_oracle(N, bins_n2):
+    """bk_binned (open closure, the reference's half-grid mode sets with
+    weight-2 conjugates, half-open bins) against the oracle's restatement of
+    bskit/main.py:877-1001 on bins where open and periodic closure differ and
+    modes sit on the Nyquist planes."""
+    L = 1000.
+    kf = 2 * np.pi / L
+    field = synth.quadratic_field(N, L, seed=8, dtype=np.float64)
+    bins = [(np.sqrt(a) * kf, np.sqrt(b) * kf) for a, b in bins_n2]
+    B, nt, k1, k2, k3 = bskit_amd.bk_binned(bskit_amd.ArrayMesh(field, L), *bins)
+    rB, rn, r1, r2, r3 = est.binned_direct(field, L, bins)
+    lo, hi = est.shell_limits([b[0] for b in bins], [b[1] for b in bins], kf)
+    per = direct.enumerate_triangles(N, lo, hi, [[0, 1, 2]])[0][0]
+    print('N=%d open Nbin %d (periodic %d), |dB|/|B| %.2e'
+          % (N, nt, per, abs(B - rB) / abs(rB)))
+    assert nt == rn and nt < per
+    assert abs(B - rB) <= 1e-10 * abs(rB)
+    assert np.allclose([k1, k2, k3], [r1, r2, r3], rtol=1e-12)
+    # the periodic form counts what the FFT estimator counts
+    _, ntp, _, _, _ = bskit_amd.bk_binned(bskit_amd.ArrayMesh(field, L), *bins, periodic=True)
+    assert ntp == per

@@ -170,8 +170,11 @@ def test_config3_512cubed_particles_all_bin_sets():
             sel = [row[tuple(sorted(t, reverse=True))] for t in np.asarray(fb.k_indices).tolist()]
             a = out[(name, p)][0]
             b = out[('all', p)][0][sel]
-            if p == 'fp64':  # one FP64 sweep per triangle: the same bits in any list
+            if p == 'fp64' and name == 'equilateral':
+                # one FP64 sweep per triangle, same side order: the same bits
                 assert np.array_equal(a, b)
+            elif p == 'fp64':  # isosceles rows list the sides in another order
+                assert _rel(a, b).max() < 1e-12
             else:  # other pair plans, other FP32 chains
                 assert _rel(a, b).max() < 2e-4
     # triangle counts of every list against the direct count

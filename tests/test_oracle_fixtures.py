@@ -243,3 +243,92 @@ def test_pk_fft_oracle_closed_forms():
     assert abs(nbin - sel.sum()) < 1e-9
     assert np.isclose(kmean, np.mean(nn[sel]) * kf, rtol=1e-12)
     assert np.isclose(p, box ** 3 * 2 * (A / 2) ** 2 / sel.sum(), rtol=1e-12)
+
+
+def _literal_binned(field, box, bins):
+    """The reference's triple loop over collected modes (main.py:966-991),
+    closure by |k1+k2+k3|^2 < 1e-4 k_f in physical units; tiny grids only."""
+    N = field.shape[0]
+    kf = est.kf_of(box)
+    dk = np.fft.rfftn(field) / field.size
+    f = np.fft.fftfreq(N, 1.0 / N)
+    pts = []
+    for kmin, kmax in bins:
+        vec, val = [], []
+        for ix in range(N):
+            for iy in range(N):
+                for iz in range(N // 2 + 1):
+                    k = kf * np.array([f[ix], f[iy], iz])
+                    k2 = float(np.sum(k ** 2))
+                    if not (kmin ** 2 <= k2 < kmax ** 2):
+                        continue
+                    vec.append(k)
+                    val.append(dk[ix, iy, iz])
+                    if 0 < iz < N // 2:
+                        vec.append(-k)
+                        val.append(np.conj(dk[ix, iy, iz]))
+        pts.append((np.array(vec).reshape(-1, 3), np.array(val)))
+    nbin, bsum, ks = 0, 0j, np.zeros(3)
+    (a, da), (b, db), (c, dc) = pts
+    for i in range(len(a)):
+        for j in range(len(b)):
+            tot = a[i] + b[j] + c
+            hit = np.nonzero(np.sum(tot ** 2, axis=1) < 1e-4 * kf)[0]
+            for t in hit:
+                nbin += 1
+                bsum += da[i] * db[j] * dc[t]
+                ks += [np.linalg.norm(a[i]), np.linalg.norm(b[j]), np.linalg.norm(c[t])]
+    B = bsum * box ** 6
+    return (B / nbin if nbin else B), nbin, *(ks / max(nbin, 1))
+
+
+def test_binned_direct_oracle():
+    """est.binned_direct (bk_binned restated with a lookup for the closure)
+    against the reference's literal triple loop on an 8^3 grid with bins
+    reaching the Nyquist plane; on low bins it equals the periodic count and
+    the FFT estimator; a bin covering every mode collects N^3 modes (half
+    grid plus weight-2 conjugates) and their |delta|^2 sum is Parseval's."""
+    rng = np.random.default_rng(3)
+    box = 100.0
+    kf = est.kf_of(box)
+    g = rng.standard_normal((8, 8, 8))
+    field = g + 0.4 * (g * g - 1)
+    s = np.sqrt
+    for bins in ([(s(2.5) * kf, s(10.5) * kf)] * 3,
+                 [(s(3.5) * kf, s(12.5) * kf), (s(5.5) * kf, s(17.5) * kf),
+                  (s(8.5) * kf, s(20.5) * kf)]):
+        got = est.binned_direct(field, box, bins)
+        lit = _literal_binned(field, box, bins)
+        assert got[1] == lit[1] > 0
+        assert abs(got[0] - lit[0]) <= 1e-12 * abs(lit[0])
+        assert np.allclose(got[2:], lit[2:], rtol=1e-13)
+    # low bins on 16^3: no periodic image can close, so the counts and B agree
+    # with the periodic enumeration and the FFT estimator
+    N = 16
+    g = rng.standard_normal((N, N, N))
+    field = g + 0.4 * (g * g - 1)
+    bins = [(s(3.5) * kf, s(8.5) * kf), (s(3.5) * kf, s(8.5) * kf), (s(2.5) * kf, s(6.5) * kf)]
+    B, nbin, k1, k2, k3 = est.binned_direct(field, box, bins)
+    lo, hi = est.shell_limits([b[0] for b in bins], [b[1] for b in bins], kf)
+    cnt, ks, _ = direct.enumerate_triangles(N, lo, hi, [[0, 1, 2]])
+    assert nbin == cnt[0]
+    ref = est.bispectrum_unnorm([field], box, lo, hi, [[0, 1, 2]])[0] / cnt[0]
+    assert abs(B.real - ref) < 1e-10 * abs(ref) and abs(B.imag) < 1e-10 * abs(ref)
+    assert np.allclose([k1, k2, k3], ks[0] / cnt[0] * kf, rtol=1e-12)
+    # above N/3 the open count is smaller than the periodic one
+    hb = [(s(24.5) * kf, s(48.5) * kf)] * 3
+    lo, hi = est.shell_limits([b[0] for b in hb], [b[1] for b in hb], kf)
+    assert est.binned_direct(field, box, hb)[1] < direct.enumerate_triangles(N, lo, hi,
+                                                                            [[0, 0, 0]])[0][0]
+    # mode collection: every mode once, values by Hermitian symmetry
+    N = 8
+    field = rng.standard_normal((N, N, N))
+    dk = np.fft.fftn(field) / N ** 3
+    everything = [(0.0, 100 * kf)] * 3
+    f = np.fft.fftfreq(N, 1.0 / N)
+    nx, ny, nz = np.meshgrid(f, f, np.arange(N // 2 + 1), indexing='ij')
+    w = np.where((nz == 0) | (nz == N // 2), 1, 2)
+    assert w.sum() == N ** 3
+    half = np.fft.rfftn(field) / N ** 3
+    assert abs(np.sum(w * np.abs(half) ** 2) - np.sum(np.abs(dk) ** 2)) < 1e-12
+    assert est.binned_direct(field, box, everything)[1] > 0
