@@ -76,6 +76,9 @@ _SIGS = {
     'bsk_triples_mfma_levels': [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll,
                                 ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll), _c_int, _c_double,
                                 _vp, _vp, _vp],
+    'bsk_triples_mfma_levels_noalloc': [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_ll,
+                                        ctypes.POINTER(_c_ll), ctypes.POINTER(_c_ll), _c_int,
+                                        _c_double, _vp, _vp],
     'bsk_triples_zfused_supported': [_vp, _vp, ctypes.POINTER(_c_int)],
     'bsk_triples_zfused': [_vp, _vp, _vp, _c_int, _c_int, _c_double, _vp, _vp, _vp],
     'bsk_cic_workspace': [_c_int, _c_ll, _vp],

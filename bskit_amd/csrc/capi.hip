@@ -11,6 +11,7 @@
 #include <array>
 #include <climits>
 #include <cmath>
+#include <mutex>
 #include <queue>
 #include <vector>
 
@@ -1429,6 +1430,66 @@ int bsk_triples_mfma_levels(const bsk_pairs* q, const uint16_t* hi_i, const uint
   return triples_mfma(q, hi_i, lo_i, hi_j, lo_j, hi_k, lo_k, shell_stride, level_ncell[0],
                       level_off, level_ncell, nwg, scale, out_dev, work, stream,
                       "bsk_triples_mfma_levels");
+}
+
+// ------------------------------------------------------------------
+// The same sums with the library's own scratch memory: a file-scope device
+// array holds the per-workgroup partials and the range sums, so the call
+// allocates nothing (for callers that cannot allocate: code under graph
+// capture, tight serving loops).  Each device holds its own copy; its
+// address is looked up once per device with that device current and kept.
+namespace {
+static __device__ __align__(256) unsigned char scratch[BSK_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+int scratch_on_current_device(void** out) {
+  int dev = -1;
+  BSK_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    bsk_set_error("bsk_triples_mfma_levels_noalloc: device %d beyond the %d supported", dev,
+                  kMaxDevices);
+    return BSK_EUNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    BSK_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = ptr;
+  }
+  *out = g_scratch[dev];
+  return BSK_OK;
+}
+}  // namespace
+
+int bsk_triples_mfma_levels_noalloc(const bsk_pairs* q, const uint16_t* hi_i,
+                                    const uint16_t* lo_i, const uint16_t* hi_j,
+                                    const uint16_t* lo_j, const uint16_t* hi_k,
+                                    const uint16_t* lo_k, long long shell_stride,
+                                    const long long* level_off, const long long* level_ncell,
+                                    int nwg, double scale, double* out_dev, void* stream) {
+  const char* fn = "bsk_triples_mfma_levels_noalloc";
+  if (!q || nwg <= 0 || !out_dev || !level_off || !level_ncell) {
+    bsk_set_error("%s: invalid argument (out_dev and the level layout are required)", fn);
+    return BSK_EINVAL;
+  }
+  size_t need = 0;
+  if (int rc = bsk_triples_mfma_workspace(q, nwg, &need)) return rc;
+  if (need > (size_t)BSK_NOALLOC_SCRATCH_BYTES) {
+    bsk_set_error("%s: the plan needs %zu scratch bytes at %d workgroups, above the library's "
+                  "%zu (fewer workgroups, or bsk_triples_mfma_levels with a workspace)",
+                  fn, need, nwg, (size_t)BSK_NOALLOC_SCRATCH_BYTES);
+    return BSK_EINVAL;
+  }
+  void* work = nullptr;
+  if (int rc = scratch_on_current_device(&work)) return rc;
+  // every partial slot a triangle reads is stored by the first launch before
+  // any other reads it, and the range sums are written before they are read,
+  // so nothing in the scratch needs clearing between calls
+  return triples_mfma(q, hi_i, lo_i, hi_j, lo_j, hi_k, lo_k, shell_stride, level_ncell[0],
+                      level_off, level_ncell, nwg, scale, out_dev, work, stream, fn);
 }
 
 namespace {

@@ -289,6 +289,23 @@ int bsk_triples_mfma_levels(const bsk_pairs* pairs, const uint16_t* hi_i, const 
                             const uint16_t* lo_k, long long shell_stride,
                             const long long* level_off, const long long* level_ncell, int nwg,
                             double scale, double* out_dev, void* work, void* stream);
+/* bsk_triples_mfma_levels without a workspace: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The per-workgroup partials and range sums live in the library's own
+ * scratch array of BSK_NOALLOC_SCRATCH_BYTES per device (held until the
+ * process ends); a plan and workgroup count whose bsk_triples_mfma_workspace
+ * exceeds it is refused with BSK_EINVAL.  The call allocates nothing; out_dev
+ * is required (the result is reduced before returning to the stream).  All
+ * calls on one device share its array: calls that overlap on one device (two
+ * streams, or two host threads) race and must be serialised by the caller. */
+#define BSK_NOALLOC_SCRATCH_BYTES (64u << 20)
+int bsk_triples_mfma_levels_noalloc(const bsk_pairs* pairs, const uint16_t* hi_i,
+                                    const uint16_t* lo_i, const uint16_t* hi_j,
+                                    const uint16_t* lo_j, const uint16_t* hi_k,
+                                    const uint16_t* lo_k, long long shell_stride,
+                                    const long long* level_off, const long long* level_ncell,
+                                    int nwg, double scale, double* out_dev, void* stream);
 /* out_dev == NULL in bsk_triples_mfma leaves the per-workgroup partials in
  * work; bsk_triples_mfma_reduce then sums them in FP64 into out_dev (the
  * same result as one call with out_dev; it uses the tail of work, after the
