@@ -182,6 +182,79 @@ def run_path(eng, wl, path, fmt, steps, warmup, torch, ntri_dev, timer=None, sta
     return b, elapsed
 
 
+def config_accuracy(torch, np):
+    """Max relative B error of the FP32 and split-BF16 paths against the FP64
+    path (all on the GPU; the FP64 path is pinned to the host FP64 oracle by
+    the GPU tests) for the two smaller workloads: a 64^3 FP64 field with 8
+    bins of 2k_f (all triangles, T = 98) and a 256^3 FP32 field with 16 bins
+    of 3k_f (T = 564), both the seeded local-quadratic field."""
+    import bskit_amd
+    from bskit_amd import synth
+    out = {}
+    kf = 2 * math.pi / 1000.0
+    for name, N, dt, kw in (('64^3 fp64, 8 bins of 2k_f, T=98', 64, np.float64,
+                             dict(kmin=kf, kmax=18 * kf, dk=2 * kf)),
+                            ('256^3 fp32, 16 bins of 3k_f, T=564', 256, np.float32,
+                             dict(kmin=kf, kmax=50.5 * kf, dk=3 * kf))):
+        field = synth.quadratic_field(N, 1000.0, seed=1, dtype=dt)
+        res = {}
+        for prec in ('fp64', 'fp32', 'bf16x3'):
+            src = field if prec == 'fp64' else field.astype(np.float32)
+            fb = bskit_amd.FFTBispectrum(src, Nmesh=N, BoxSize=1000.0, precision=prec, **kw)
+            res[prec] = fb.measure_bispectrum_faster()['B']
+        ref = res['fp64']
+        out[name] = {'T': int(len(ref)),
+                     'fp32_max_rel_err': float(np.max(np.abs(res['fp32'] / ref - 1))),
+                     'bf16x3_max_rel_err': float(np.max(np.abs(res['bf16x3'] / ref - 1)))}
+    return out
+
+
+def cpu_host_info():
+    """CPU model string and NUMA layout of the host (Linux sysfs/procfs)."""
+    model = None
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nodes = {}
+    base = '/sys/devices/system/node'
+    try:
+        for d in sorted(os.listdir(base)):
+            if d.startswith('node') and d[4:].isdigit():
+                with open(os.path.join(base, d, 'cpulist')) as f:
+                    nodes[d] = f.read().strip()
+    except OSError:
+        pass
+    try:
+        allowed = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = []
+    return {'cpu_model': model, 'numa_nodes': nodes, 'threads_allowed': len(allowed),
+            'logical_cpus': os.cpu_count()}
+
+
+def profile_counters(name):
+    """{counter: per-launch mean} from a committed rocprofv3 counter summary
+    table (profiles/<name>), or {} when absent."""
+    path = os.path.join(ROOT, 'profiles', name)
+    out = {}
+    if not os.path.exists(path):
+        return out
+    with open(path) as f:
+        for line in f:
+            w = [x.strip() for x in line.strip().strip('|').split('|')]
+            if len(w) == 2:
+                try:
+                    out[w[0]] = float(w[1])
+                except ValueError:
+                    pass
+    return out
+
+
 def profiled_traffic(kernel, nmesh, shell_mesh):
     """HBM bytes per launch of kernel from the committed rocprofv3 summary
     (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), or
@@ -229,6 +302,7 @@ def run_serving(args, torch, np):
                 'throughput_jobs_per_s': len(jobs) / res['makespan_s'],
                 'peak_footprint_GB': res['peak_footprint_bytes'] / 1e9,
                 'hbm_budget_GB': srv.budget / 1e9, 'streams': len(srv.streams),
+                'tuned_classes': len(srv.tuned),
                 'fields': gpu_s,
                 'mix': 'N uniform {128,192,256,384,512,768}, nb uniform [8,40], '
                        'auto/cross 1:1, fast-path B_unnorm per job copied to host; '
@@ -256,6 +330,7 @@ def run_batch(args, torch, np):
     el = (time.perf_counter() - t0) / reps
     res = {'pairs': len(pairs), 'triangles': cb.ntri, 'graph_ms': el * 1e3,
            'pairs_per_s': len(pairs) / el, 'replays': reps, 'branches': cb.nbranch,
+           'tuned_classes': {'%d^3/S%d' % k: v for k, v in cb.tuned.items()},
            'layout': '32 @128^3, 24 @256^3, 8 @384^3, nb uniform [8,24], <AAB>'}
     log('[batch] %d pairs in %.2f ms per graph replay' % (len(pairs), el * 1e3))
     del cb, pairs
@@ -410,7 +485,7 @@ def main():
                'kernel_ms': float(np.mean(ts.pair_durations('kernel')))}
         log('[bench] %s: %.3f ms/grid' % (alt_path, alt['ms_per_grid']))
 
-    res_plain = None
+    res_plain = res_tiled = None
     if not args.no_plain:
         tp = StageTimer(torch)
         b_plain, el_plain = run_path(eng, wl, 'plain', L.FMT_F32, args.plain_steps, 1, torch,
@@ -420,6 +495,16 @@ def main():
                      'triples_ms': float(np.mean(tp.durations('shells', 'triples')))}
         b_plain = b_plain.cpu().numpy()
         log('[bench] fp32 plain: %.1f ms/grid' % res_plain['ms_per_grid'])
+        # the LDS-tiled FP32 vector path (every shell cell staged once per
+        # 32-cell tile for all triangles): the strongest FP32 baseline
+        tt = StageTimer(torch)
+        b_tiled, el_tiled = run_path(eng, wl, 'tiled', L.FMT_F32, args.plain_steps, 1, torch,
+                                     ntri_dev, tt)
+        res_tiled = {'grids_per_s': args.plain_steps / el_tiled,
+                     'ms_per_grid': el_tiled / args.plain_steps * 1e3,
+                     'triples_ms': float(np.mean(tt.durations('shells', 'triples')))}
+        b_tiled = b_tiled.cpu().numpy()
+        log('[bench] fp32 tiled: %.1f ms/grid' % res_tiled['ms_per_grid'])
 
     acc = {}
     if not args.no_fp64:
@@ -433,6 +518,7 @@ def main():
         if res_plain is not None:
             rel_p = np.abs(b_plain[ok] / b64[ok] - 1)
             acc['fp32_max_rel_err'] = float(rel_p.max())
+            acc['fp32_tiled_max_rel_err'] = float(np.abs(b_tiled[ok] / b64[ok] - 1).max())
         log('[bench] accuracy vs FP64: %s' % acc)
 
     # roofline of the dominant kernel: the split-BF16 MFMA pair kernel, timed
@@ -456,9 +542,21 @@ def main():
         ai = flops / alg_bytes
         bound = 'hbm' if ai * HBM_PEAK_GBS * 1e9 < BF16_PEAK_TFLOPS * 1e12 else 'mfma'
         achieved = alg_bytes / t_k / 1e9
+        ctr = profile_counters('%s_%d_counters.md' % (kernel, N)) if kernel == 'k_triple_pairs' \
+            else {}
+        # issued MFMA (16x16x32 bf16: 16384 flop each) per launch from the
+        # committed counters, over this run's kernel time, against the dense
+        # bf16 peak
+        mfma_util = (ctr['SQ_INSTS_MFMA'] * 16384.0 / (t_k / launches) / (BF16_PEAK_TFLOPS * 1e12)
+                     if 'SQ_INSTS_MFMA' in ctr else None)
         return {'kernel': kernel, 'bound': bound, 'achieved': achieved,
                 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                 'traffic': profiled_traffic(kernel, N, eng.S),
+                'traffic_source': ('profiles/hbm_traffic_%d_s%d.txt (rocprofv3 FETCH_SIZE + '
+                                   'WRITE_SIZE per dispatch, copy-calibrated; the file names the '
+                                   'commit it was measured at)' % (N, eng.S)),
+                'mfma_issue_util': mfma_util,
+                'counters_source': ('profiles/%s_%d_counters.md' % (kernel, N)) if ctr else None,
                 'kernel_ms': grid_ms / launches, 'launches_per_grid': launches,
                 'algorithmic_bytes': alg_bytes / launches,
                 'useful_tflops': flops / t_k / 1e12, 'arith_intensity_flop_per_byte': ai,
@@ -499,6 +597,26 @@ def main():
     shells_alg = 8.0 * H + nsh * 4 * (float(eng.layout.stride) if levels else ncell)
     stage_gbs = {'shells_GBps': shells_alg / t_sh / 1e9,
                  'forward_GBps': (4.0 * N ** 3 + 8.0 * H) / (stage_ms['forward'] * 1e-3) / 1e9}
+    # algorithmic bytes of one grid in the estimator's own form (shells and
+    # triple reads on the N^3 input mesh, s = 4 B per shell value) beside the
+    # form this path runs (shells on the S^3 shell mesh, each triangle over its
+    # sublattice level): the N^3 figure over the step time exceeds the HBM
+    # peak, which the exact shell-mesh and sublattice reductions explain
+    Hn = N * N * (N // 2 + 1)
+    bytes_n3 = (4.0 * N ** 3 + 8 * Hn) + (8 * Hn + nsh * 4.0 * N ** 3) + nsh * 4.0 * N ** 3
+    bytes_s = (4.0 * N ** 3 + 8 * H) + shells_alg + roof['algorithmic_bytes'] * roof[
+        'launches_per_grid'] if main_path == 'mfma' else None
+    step_bytes = {'n3_mesh_form_GB': bytes_n3 / 1e9,
+                  'n3_mesh_form_rate_GBps_per_step': bytes_n3 / (sequential['ms_per_grid'] * 1e-3)
+                  / 1e9,
+                  'shell_mesh_form_GB': None if bytes_s is None else bytes_s / 1e9,
+                  'shell_mesh_form_rate_GBps_per_step': None if bytes_s is None else
+                  bytes_s / (sequential['ms_per_grid'] * 1e-3) / 1e9}
+
+    acc_cfg = None
+    if not args.no_fp64:
+        acc_cfg = config_accuracy(torch, np)
+        log('[bench] accuracy by config: %s' % acc_cfg)
 
     latency = None
     if not args.no_serve:
@@ -514,9 +632,9 @@ def main():
         field_h = wl['field'].cpu().numpy()
         lo, hi = est.shell_limits(wl['k_low'], wl['k_high'], wl['kf'])
         log('[bench] cpu baseline sample...')
-        r = baseline.time_sample(field_h, lo, hi, wl['tri'], n_shell_sample=2, n_tri_sample=6)
+        r = baseline.time_sample(field_h, lo, hi, wl['tri'], n_shell_sample=4, n_tri_sample=24)
         cpu = {'value': 1.0 / r['seconds_per_grid'], 'unit': 'grids/s', 'cores': r['cores'],
-               'kind': 'port',
+               'kind': 'port', 'host': cpu_host_info(),
                'sample': ('reference-structured numpy/scipy path on %d host threads: full forward '
                           'r2c (%.1fs), %d shell c2r (%.2fs each), %d triangle sums (%.2fs each); '
                           'extrapolated to %d shells and %d triangles'
@@ -553,13 +671,18 @@ def main():
                                    % (args.inflight, args.inflight) if args.inflight > 1
                                    else 'single GPU')},
         'fp32_plain': res_plain,
+        'fp32_tiled': res_tiled,
         # one grid at a time on both sides
         'speedup_vs_fp32_plain': ((sequential['grids_per_s'] / res_plain['grids_per_s'])
                                   if res_plain else None),
+        'speedup_vs_fp32_tiled': ((sequential['grids_per_s'] / res_tiled['grids_per_s'])
+                                  if res_tiled else None),
+        'accuracy_by_config': acc_cfg,
         'one_grid_at_a_time': sequential,
         'stage_ms': stage_ms,
         'other_path': alt,
         'stage_GBps': stage_gbs,
+        'step_algorithmic_bytes': step_bytes,
         'gridinfo_once_s': t_gridinfo,
         'autotune': tuned,
         'accuracy_vs_fp64': acc,

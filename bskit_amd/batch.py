@@ -52,8 +52,12 @@ class _Pair:
 class CrossBatch:
     """pairs: list of (A, B, box, kLU) with A, B float32 device N^3 fields."""
 
-    def __init__(self, pairs, device=None, nbranch=4):
+    def __init__(self, pairs, device=None, nbranch=4, autotune=True):
+        """autotune: tune the radix plans and the pair kernel's workgroup count
+        per mesh class (Nmesh, shell mesh) on the first pair of the class
+        (bskit_amd.tune.tune_class) before anything is recorded."""
         self.device = require_device(device)
+        self.tuned = {}
         self.nbranch = max(1, min(int(nbranch), len(pairs)))
         self.engines = {}
         self.pairs = []
@@ -67,6 +71,11 @@ class CrossBatch:
                 self.engines[key] = eng
             tri = B.closed_triangle_indices(kLU, 2).astype(np.int32)
             path = PATH_MFMA if GridEngine.mfma_supported(eng.nb, 2, tri) else PATH_PLAIN
+            if autotune and path == PATH_MFMA and len(tri):
+                from . import tune
+                cfg = tune.tune_class(eng, tri=tri, field=a, nfield=2)
+                self.tuned[(eng.N, eng.S)] = {'variants': cfg.get('variants'),
+                                              'mfma_nwg': cfg.get('mfma_nwg')}
             self.pairs.append(_Pair(eng, a, b, tri, path, float(box)))
         self.branches = [torch.cuda.Stream(device=self.device) for _ in range(self.nbranch)]
         self.works = [Workspace(self.device) for _ in range(self.nbranch)]

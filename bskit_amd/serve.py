@@ -66,20 +66,26 @@ class Server:
     """Admission + continuous batching of Jobs on one device."""
 
     def __init__(self, device=None, hbm_budget=None, nstreams=4, poll_s=2e-5, assign='work',
-                 depth=None):
+                 depth=None, autotune=True):
         """assign: the stream an arriving job is issued to -- 'work', the one
         with the least outstanding work (the footprints of its jobs in flight,
         which grow with the mesh and the shell set), so a small job does not
         queue behind a large one where another stream is nearly idle; or
         'count', the one with the fewest jobs in flight.  depth: at most this
         many jobs in flight per stream (None: no limit); arrivals beyond it wait
-        in the host queue and go to the first stream with room."""
+        in the host queue and go to the first stream with room.  autotune: tune
+        the radix plans and the pair kernel's workgroup count once per mesh
+        class (Nmesh, shell mesh) on the first job of the class, in prepare
+        (bskit_amd.tune.tune_class), and apply it to every engine of the
+        class."""
         if assign not in ('work', 'count'):
             raise ValueError("assign must be 'work' or 'count'")
         if depth is not None and depth < 1:
             raise ValueError('depth must be >= 1 or None')
         self.assign = assign
         self.depth = depth
+        self.autotune = autotune
+        self.tuned = {}  # mesh class -> configuration applied
         self.device = require_device(device)
         free, _ = torch.cuda.mem_get_info(self.device)
         self.budget = int(hbm_budget) if hbm_budget else int(min(256e9, 0.85 * free))
@@ -140,6 +146,11 @@ class Server:
             self._tri[key] = tri
         job.tri = tri
         job.path = PATH_MFMA if GridEngine.mfma_supported(eng.nb, job.nfield, tri) else PATH_PLAIN
+        if self.autotune and job.path == PATH_MFMA and len(tri):
+            from . import tune
+            cfg = tune.tune_class(eng, tri=tri, field=job.fields[0], nfield=job.nfield)
+            self.tuned[(eng.N, eng.S)] = {'variants': cfg.get('variants'),
+                                          'mfma_nwg': cfg.get('mfma_nwg')}
         if job.path == PATH_MFMA and len(tri):
             mode = {1: L.MODE_AUTO, 2: L.MODE_AAB}[job.nfield]
             eng.pair_plan(tri, eng.nb, mode, levels=eng.uses_levels())

@@ -13,8 +13,11 @@ Tunables of one GridEngine key (Nmesh, shell mesh, retained cube, bins):
 
 Results live in a process-wide cache (a dict guarded by a lock) persisted
 as JSON at $BSK_TUNE_CACHE (default ~/.cache/bskit_amd/tune.json), keyed by
-device name and workload shape, so a tuned shape is applied to every later
-engine without re-timing.  Every candidate computes the same quantity (the
+device name and workload shape (meshes, bins, field roles and a hash of the
+triangle list), so a tuned shape is applied to every later engine without
+re-timing.  ``tune_class`` shares one tuning among the engines of a mesh
+class (Nmesh, shell mesh): the serving loop and the graph batches tune one
+representative job per class and apply it to the rest.  Every candidate computes the same quantity (the
 FFT variants differ only in rounding), so tuning never changes what is
 measured, only how fast.
 """
@@ -60,15 +63,24 @@ def _save():
         pass  # a read-only home keeps the in-process cache only
 
 
-def key_of(eng, ntri=0):
+def key_of(eng, ntri=0, nfield=1, tri_hash=''):
     dev = torch.cuda.get_device_name(eng.device)
     lv = int(bool(eng.uses_levels()))  # sublattice-level MFMA sums
-    return '%s|N%d|S%d|m%d|nb%d|T%d|L%d' % (dev, eng.N, eng.S, eng.m, eng.nb, ntri, lv)
+    return '%s|N%d|S%d|m%d|nb%d|T%d|L%d|F%d|%s' % (dev, eng.N, eng.S, eng.m, eng.nb, ntri, lv,
+                                                   nfield, tri_hash)
 
 
-def lookup(eng, ntri=0):
+def _tri_hash(tri):
+    import hashlib
+    if tri is None:
+        return ''
+    t = np.ascontiguousarray(np.asarray(tri, dtype=np.int32))
+    return hashlib.sha1(t.tobytes()).hexdigest()[:12]
+
+
+def lookup(eng, ntri=0, nfield=1, tri=None):
     with _LOCK:
-        return _load().get(key_of(eng, ntri))
+        return _load().get(key_of(eng, ntri, nfield, _tri_hash(tri)))
 
 
 def apply(eng, cfg):
@@ -95,17 +107,19 @@ def _time(fn, reps=3):
     return a.elapsed_time(b) / reps
 
 
-def tune(eng, tri=None, field=None, reps=3, force=False, log=None):
+def tune(eng, tri=None, field=None, reps=3, force=False, log=None, nfield=1):
     """Time the candidates on this engine's shapes and store the best.
 
     field: a float32 device field of the engine's mesh (random if None);
-    tri: triangle list for the MFMA workgroup count (skipped if None).
+    tri: triangle list for the MFMA workgroup count (skipped if None);
+    nfield: 1 (auto <AAA>) or 2 (cross <AAB>: the pair plan of that mode,
+    timed with the field's shells on both roles).
     Returns the configuration dict (also applied to the engine).
     """
     ntri = 0 if tri is None else len(tri)
-    k = key_of(eng, ntri)
+    k = key_of(eng, ntri, nfield, _tri_hash(tri))
     if not force:
-        cfg = lookup(eng, ntri)
+        cfg = lookup(eng, ntri, nfield, tri)
         if cfg is not None:
             apply(eng, cfg)
             return cfg
@@ -141,20 +155,21 @@ def tune(eng, tri=None, field=None, reps=3, force=False, log=None):
     for p, v in enumerate(best[1]):
         eng.set_variant(3 + p, v)
     cfg = {'variants': [int(v) for v in variants], 'timings_ms': timings}
-    if tri is not None and ntri and eng.mfma_supported(eng.nb, 1, tri):
+    if tri is not None and ntri and eng.mfma_supported(eng.nb, nfield, tri):
         eng.shells(dk, fmt=L.FMT_BF16X2, out=shells)
         out = torch.empty(ntri, dtype=torch.float64, device=eng.device)
         nchunk = eng.ncell // 128
         cands = sorted({n for n in (256, 512, 768, 1024, 1536, 2048) if n <= max(1, nchunk // 4)})
         cands = cands or [eng.mfma_nwg()]
         best = None
+        sets = [shells] * nfield
         for n in cands:
-            t = _time(lambda: eng.triples([shells], tri, path='mfma', out=out, nwg=n), reps)
+            t = _time(lambda: eng.triples(sets, tri, path='mfma', out=out, nwg=n), reps)
             timings['mfma_nwg%d' % n] = t
             if best is None or t < best[0]:
                 best = (t, n)
         cfg['mfma_nwg'] = int(best[1])
-        if eng.zfused_supported(tri):
+        if nfield == 1 and eng.zfused_supported(tri):
             best = None
             for n in (256, 512, 768, 1024):
                 t = _time(lambda: eng.bispectrum_zfused(dk, tri, out=out, nwg=n), reps)
@@ -176,5 +191,25 @@ def tune(eng, tri=None, field=None, reps=3, force=False, log=None):
     with _LOCK:
         _load()[k] = cfg
         _save()
+    apply(eng, cfg)
+    return cfg
+
+
+_CLASS = {}
+
+
+def tune_class(eng, tri=None, field=None, nfield=1, log=None):
+    """Tune the first engine of its (device, Nmesh, shell mesh) class in full
+    and apply that configuration to every later engine of the class (the
+    radix plans depend on the FFT lengths, the pair kernel's workgroup count
+    mostly on the cell count); returns the configuration applied."""
+    ck = (torch.cuda.get_device_name(eng.device), eng.N, eng.S)
+    with _LOCK:
+        cfg = _CLASS.get(ck)
+    if cfg is None:
+        cfg = tune(eng, tri=tri, field=field, nfield=nfield, log=log)
+        with _LOCK:
+            _CLASS[ck] = cfg
+        return cfg
     apply(eng, cfg)
     return cfg

@@ -410,6 +410,17 @@ def test_autotune_picks_valid_config_and_keeps_results(tmp_path, monkeypatch):
                       scale=1.0 / eng.ncell).cpu().numpy()
     assert np.abs(got - ref).max() <= 1e-4 * np.abs(ref).max()
     eng2 = _engine(N, box, lo, hi)
-    assert tune.lookup(eng2, len(tri)) == cfg
+    assert tune.lookup(eng2, len(tri), tri=tri) == cfg
+    assert tune.lookup(eng2, len(tri), tri=tri[::-1].copy()) is None  # another list
     assert eng2.autotune(tri=tri) == cfg         # cached: no re-timing
     assert eng2.mfma_nwg() == cfg.get('mfma_nwg', eng2.mfma_nwg())
+    # cross <AAB> lists are tuned with their own pair plans
+    tri2 = bins.closed_triangle_indices(np.stack([lo, hi], 1), 2)
+    cfg2 = tune.tune(eng2, tri=tri2, field=f, nfield=2)
+    assert cfg2.get('mfma_nwg') and 'zfused' not in cfg2
+    # one tuning per mesh class, applied to the next engine of the class
+    monkeypatch.setattr(tune, '_CLASS', {})
+    e3, e4 = _engine(N, box, lo, hi), _engine(N, box, lo, hi)
+    c3 = tune.tune_class(e3, tri=tri, field=f)
+    c4 = tune.tune_class(e4, tri=tri2, nfield=2)
+    assert c3 is c4 and e4.tuned_nwg == c3.get('mfma_nwg')
