@@ -163,6 +163,16 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
   cplx<T>* d1 = reinterpret_cast<cplx<T>*>(work);
   cplx<T>* d2 = d1 + (size_t)nsh * g.MY * N * g.MZP;
   const T kf = (T)(2.0 * M_PI / p->box);
+  // the pass kernels address one shell's inverse-x output, and one shell
+  // row of the level layout, with 32-bit byte offsets from a 64-bit base
+  const size_t esz0 = (fmt == BSK_FMT_F64) ? 8 : (fmt == BSK_FMT_F32 ? 4 : 2);
+  if ((unsigned long long)N * g.MY * g.MZP * sizeof(cplx<T>) >= (1ull << 32) ||
+      ((stages & BSK_STAGE_LEVELS) && (unsigned long long)stride * esz0 >= (1ull << 32))) {
+    bsk_set_error("bsk_shells: a shell of this plan spans 4 GiB or more in one pass (shell mesh "
+                  "%d, retained cube %d x %d x %d); use FP32/BF16X2 or a smaller shell reach",
+                  N, g.MX, g.MY, g.MZ);
+    return BSK_EUNSUPPORTED;
+  }
   int rc = BSK_OK;
   if (stages & BSK_STAGE_X) {
     rc = bsk_inv_x<T>(N, p->variant[3], dk, d1, tw, g, p->d_lo, p->d_hi, shell0, nsh, kind, kf,
@@ -254,7 +264,7 @@ int bsk_plan_create(bsk_plan** out, int nmesh, double box_size, int nbin, const 
   *out = nullptr;
   if (!bsk_fft_size_supported(nmesh)) {
     bsk_set_error("bsk_plan_create: Nmesh=%d is not a supported FFT size (even, 5-smooth, "
-                  "16..768, half length with a radix plan)", nmesh);
+                  "16..1024, half length with a radix plan)", nmesh);
     return BSK_EUNSUPPORTED;
   }
   bsk_plan* p = new bsk_plan();

@@ -275,4 +275,5 @@ BSK_PLAN(512, 0, 8, 8, 8) BSK_PLAN(512, 1, 4, 4, 4, 8)
 BSK_PLAN(576, 0, 8, 8, 3, 3) BSK_PLAN(576, 1, 4, 4, 4, 3, 3)
 BSK_PLAN(640, 0, 8, 8, 2, 5) BSK_PLAN(640, 1, 8, 4, 4, 5)
 BSK_PLAN(768, 0, 8, 8, 4, 3) BSK_PLAN(768, 1, 4, 4, 4, 4, 3)
+BSK_PLAN(1024, 0, 8, 8, 4, 4) BSK_PLAN(1024, 1, 4, 4, 4, 4, 4)
 #undef BSK_PLAN

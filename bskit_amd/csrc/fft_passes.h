@@ -498,11 +498,14 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   constexpr int N = 2 * NH;
   constexpr int LD = pencil_ld<NH>();
   constexpr int KH = NH / 2 + 1;  // k = 0..NH/2 (each with its mirror NH - k)
-  constexpr int KQ = pow2_at_least(KH), RK = W / KQ, NPK = P / RK;
+  // lanes per row for the pairs: at most the workgroup, each lane then
+  // taking KS pairs of its row (KS = 1 unless NH/2 + 1 > W)
+  constexpr int KQ = pow2_at_least(KH) < W ? pow2_at_least(KH) : W;
+  constexpr int KS = (KH + KQ - 1) / KQ, RK = W / KQ, NPK = P / RK;
   constexpr int QQ = pow2_at_least(NH / 2), RQ = W / QQ, NPQ = P / RQ;
   using Lay = ShellLevels<NH>;
   static_assert(NH % 2 == 0, "z c2r needs N divisible by 4");
-  static_assert(KQ <= W && P % RK == 0 && P % RQ == 0, "row mapping of the z c2r");
+  static_assert(P % RK == 0 && P % RQ == 0, "row mapping of the z c2r");
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
@@ -548,24 +551,29 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   auto prow = [&](int p) { return p < n1 ? r1 + step * p : r2 + step * (p - n1); };
   // (k, NH - k) pairs: lane k of rows r0, r0 + RK, ...; up to 8 rows (16
   // loads) in flight per batch
-  const int k = tid & (KQ - 1), r0 = tid / KQ, j = NH - k;
-  const bool kact = k < KH;
-  const cplx<T> wk = kact ? cconj(tw[k]) : cplx<T>{(T)0, (T)0};  // exp(+2 pi i k / N)
+  const int r0 = tid / KQ;
   constexpr int B = NPK < 8 ? NPK : 8;
 #pragma unroll
-  for (int q0 = 0; q0 < NPK; q0 += B) {
-    cplx<T> xk[B], xj[B];
+  for (int ks = 0; ks < KS; ++ks) {
+    const int k = (tid & (KQ - 1)) + ks * KQ, j = NH - k;
+    const bool kact = k < KH;
+    const cplx<T> wk = kact ? cconj(tw[k]) : cplx<T>{(T)0, (T)0};  // exp(+2 pi i k / N)
 #pragma unroll
-    for (int q = 0; q < B; ++q) {
-      const int p = r0 + (q0 + q) * RK;
-      const bool ok = q0 + q < NPK && kact && p < np;
-      const unsigned r = (unsigned)(prow(p) * MZP);
-      xk[q] = (ok && k < mzb) ? get_u32(src, r + k) : cplx<T>{(T)0, (T)0};
-      xj[q] = (ok && j < mzb) ? get_u32(src, r + j) : cplx<T>{(T)0, (T)0};
+    for (int q0 = 0; q0 < NPK; q0 += B) {
+      cplx<T> xk[B], xj[B];
+#pragma unroll
+      for (int q = 0; q < B; ++q) {
+        const int p = r0 + (q0 + q) * RK;
+        const bool ok = q0 + q < NPK && kact && p < np;
+        const unsigned r = (unsigned)(prow(p) * MZP);
+        xk[q] = (ok && k < mzb) ? get_u32(src, r + k) : cplx<T>{(T)0, (T)0};
+        xj[q] = (ok && j < mzb) ? get_u32(src, r + j) : cplx<T>{(T)0, (T)0};
+      }
+#pragma unroll
+      for (int q = 0; q < B; ++q)
+        if (q0 + q < NPK && kact)
+          c2r_pair<T>(buf + (r0 + (q0 + q) * RK) * LD, k, NH, xk[q], xj[q], wk);
     }
-#pragma unroll
-    for (int q = 0; q < B; ++q)
-      if (q0 + q < NPK && kact) c2r_pair<T>(buf + (r0 + (q0 + q) * RK) * LD, k, NH, xk[q], xj[q], wk);
   }
   __syncthreads();
   if (gather)

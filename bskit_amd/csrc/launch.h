@@ -8,7 +8,7 @@
 #define BSK_FFT_SIZES(X)                                                                  \
   X(16) X(24) X(32) X(40) X(48) X(64) X(72) X(80) X(96) X(120) X(128) X(144) X(160)       \
   X(192) X(200) X(216) X(240) X(256) X(288) X(320) X(384) X(400) X(432) X(480) X(512)    \
-  X(576) X(640) X(768)
+  X(576) X(640) X(768) X(1024)
 
 bool bsk_fft_size_supported(int N);
 // Smallest supported mesh size strictly greater than n (0 if none).

@@ -626,6 +626,102 @@ class GridEngine:
             return out
         raise ValueError('unknown triple path %r' % (path,))
 
+    # ------------------------------------------------------- x-slab streaming
+    def slab_planes(self, nfield, fmt, max_bytes=None):
+        """x planes per slab so that nfield sets of nb shell slabs fit in
+        max_bytes (default 40% of free HBM); a divisor of S whose slab cell
+        count is a multiple of the pair kernel's 128-cell chunk."""
+        S, plane = self.S, self.S * self.S
+        if max_bytes is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            max_bytes = int(free * 0.4)
+        per = {L.FMT_F64: 8, L.FMT_F32: 4, L.FMT_BF16X2: 4}[fmt]
+        xs = int(max(1, min(S, max_bytes // max(1, nfield * self.nb * plane * per))))
+        while xs > 1 and (S % xs or (xs * plane) % 128):
+            xs -= 1
+        if S % xs or (xs * plane) % 128:
+            raise ValueError('no x slab of the %d^3 shell mesh fits %d bytes' % (S, max_bytes))
+        return xs
+
+    def bispectrum_slabs(self, dks, tri, path=PATH_MFMA, scale=1.0, max_bytes=None, out=None,
+                         stream=None, nwg=None, xslab=None):
+        """scale * sum_x A_i B_j C_k over the shell mesh for every (i, j, k) of
+        tri with the shells made and summed one x slab at a time, so the shell
+        fields never exist whole: for shell sets larger than HBM (the
+        reference's production 1024^3 / 80-bin runs).  dks: spectra from
+        forward(), one (auto) or two (<AAB>: sides A, A, B).  path 'mfma'
+        (split-BF16, FP32 spectra) or 'plain' (the spectra's precision).
+        Slab sums are added in FP64 in a fixed order (deterministic)."""
+        tri = np.ascontiguousarray(np.asarray(tri, dtype=np.int32).reshape(-1, 3))
+        T = len(tri)
+        nf = len(dks)
+        if nf not in (1, 2):
+            raise ValueError('one (auto) or two (<AAB>) spectra')
+        if T and (tri.min() < 0 or tri.max() >= self.nb):
+            raise IndexError('triangle shell index out of range')
+        f64 = dks[0].dtype == torch.complex128
+        if path == PATH_MFMA:
+            if f64:
+                raise ValueError('the MFMA path needs FP32 (complex64) spectra')
+            fmt = L.FMT_BF16X2
+        elif path == PATH_PLAIN:
+            fmt = L.FMT_F64 if f64 else L.FMT_F32
+        else:
+            raise ValueError('path must be mfma or plain')
+        dt = L.DT_F64 if f64 else L.DT_F32
+        if out is None:
+            out = torch.zeros(T, dtype=torch.float64, device=self.device)
+        else:
+            out.zero_()
+        if T == 0:
+            return out
+        S, nb = self.S, self.nb
+        xs = int(xslab) if xslab else self.slab_planes(nf, fmt, max_bytes)
+        if S % xs or (xs * S * S) % 128:
+            raise ValueError('x slab of %d planes does not tile the %d^3 shell mesh' % (xs, S))
+        cells = xs * S * S
+        st = stream_handle(stream)
+        nbytes = L.size_out()
+        L.call('bsk_shells_workspace', self._plan, dt, nb, xs, ctypes.byref(nbytes))
+        works = [self.work.get('slab_w%d' % f, nbytes.value) for f in range(nf)]
+        sets = []  # one slab of every shell per field, C order
+        for f in range(nf):
+            if fmt == L.FMT_BF16X2:
+                hi = torch.empty((nb, cells), dtype=torch.int16, device=self.device)
+                lo = torch.empty((nb, cells), dtype=torch.int16, device=self.device)
+                sets.append(ShellSet(fmt, hi, lo, cells))
+            else:
+                sets.append(ShellSet(fmt, torch.empty((nb, cells), dtype=torch.float64 if f64
+                                                      else torch.float32, device=self.device),
+                                     None, cells))
+        # inverse x of every shell once per field (kept in its workspace)
+        for f in range(nf):
+            L.call('bsk_shells', self._plan, dt, L.KIND_DELTA, L.ptr(dks[f]), 0, nb, 0, xs,
+                   L.STAGE_X, fmt, None, None, cells, 0, L.ptr(works[f]), st)
+        part = torch.empty(T, dtype=torch.float64, device=self.device)
+        if path == PATH_MFMA:
+            plan = self.pair_plan(tri, nb, _MODE[nf], levels=False)
+            nwg = int(nwg) if nwg else int(max(1, min(1024, cells // 128 // 8)))
+            L.call('bsk_triples_mfma_workspace', plan.handle, nwg, ctypes.byref(nbytes))
+            wm = self.work.get('slab_mfma', nbytes.value)
+        for x0 in range(0, S, xs):
+            for f in range(nf):
+                sh = sets[f]
+                L.call('bsk_shells', self._plan, dt, L.KIND_DELTA, L.ptr(dks[f]), 0, nb, x0, xs,
+                       L.STAGE_YZ, fmt, L.ptr(sh.data), L.ptr(sh.lo), cells, 0,
+                       L.ptr(works[f]), st)
+            if path == PATH_MFMA:
+                a, c = sets[0], sets[-1]
+                L.call('bsk_triples_mfma', plan.handle, L.ptr(a.data), L.ptr(a.lo),
+                       L.ptr(a.data), L.ptr(a.lo), L.ptr(c.data), L.ptr(c.lo), cells, cells,
+                       nwg, float(scale), L.ptr(part), L.ptr(wm), st)
+            else:
+                self.triples(sets, tri, path=PATH_PLAIN, scale=scale, out=part, stream=stream)
+            with torch.cuda.stream(stream if stream is not None else
+                                   torch.cuda.current_stream(self.device)):
+                out += part
+        return out
+
     # ------------------------------------------------------------- grid info
     def grid_info(self, tri, max_bytes=None, stream=None):
         """(N_tri, kbar) for each triangle (i, j, k) of bin indices, FP64.

@@ -121,6 +121,9 @@ class FFTBispectrum:
     device : torch device, optional
     """
     logger = logging.getLogger('FFTBispectrum')
+    # shell sets above this fraction of free HBM are made and summed x slab
+    # by x slab instead of resident
+    slab_threshold = 0.6
 
     def __init__(self, source, Nmesh=None, BoxSize=None, dk=None, kmin=None, kmax=None,
                  num_lowk_bins=0, dk_high=-1., dmu=None, mu_min=None, mu_max=None,
@@ -283,14 +286,18 @@ class FFTBispectrum:
     def _engine(self, kLU):
         return get_engine(self._N, self._L, kLU, self._device)
 
-    def _shellsets(self, eng, kLU, tri):
-        if not self.attrs.get('painted') or self._fields is None:
-            raise RuntimeError('meshes are not painted (for_grid_info_only=True)')
+    def _shell_format(self, kLU, tri):
         fmt = {'bf16x3': L.FMT_BF16X2, 'fp32': L.FMT_F32, 'fp64': L.FMT_F64}[self.precision]
         if fmt == L.FMT_BF16X2 and not GridEngine.mfma_supported(len(kLU), len(self._fields), tri):
             warnings.warn('%d bins with %d fields exceed the MFMA kernel (128 shells, 96 for '
                           '<ABC>): using the FP32 path' % (len(kLU), len(self._fields)))
             fmt = L.FMT_F32
+        return fmt
+
+    def _shellsets(self, eng, kLU, tri):
+        if not self.attrs.get('painted') or self._fields is None:
+            raise RuntimeError('meshes are not painted (for_grid_info_only=True)')
+        fmt = self._shell_format(kLU, tri)
         key = _bins_key(kLU)
         sets = []
         meshes = self._meshes()
@@ -317,6 +324,20 @@ class FFTBispectrum:
         """factor * sum_x d_i d_j d_k / ncell: the reference's L^6/N^3 sum
         (main.py:1875-1882) evaluated on the engine's shell mesh."""
         eng = self._engine(kLU)
+        if not self.attrs.get('painted') or self._fields is None:
+            raise RuntimeError('meshes are not painted (for_grid_info_only=True)')
+        nf = len(self._fields)
+        fmt = self._shell_format(kLU, tri)
+        free, _ = torch.cuda.mem_get_info(eng.device)
+        if nf <= 2 and nf * eng.shell_bytes(fmt) > self.slab_threshold * free:
+            # the shell sets do not fit in HBM: make and sum them x slab by x
+            # slab (each field's spectrum kept, shells never whole)
+            meshes = self._meshes()
+            dks = [eng.forward(f, window=int(m.compensation_window))
+                   for f, m in zip(self._fields, meshes)]
+            path = 'mfma' if fmt == L.FMT_BF16X2 else 'plain'
+            out = eng.bispectrum_slabs(dks, tri, path=path, scale=factor / float(eng.ncell))
+            return out.cpu().numpy()
         sets, path = self._shellsets(eng, kLU, tri)
         out = eng.triples(sets, tri, path=path, scale=factor / float(eng.ncell))
         return out.cpu().numpy()

@@ -18,7 +18,7 @@
  *   - Grids are cubic: Nmesh^3 cells in a box of side box_size.  Supported
  *     Nmesh: the even 5-smooth sizes 16, 24, 32, 40, 48, 64, 72, 80, 96, 120,
  *     128, 144, 160, 192, 200, 216, 240, 256, 288, 320, 384, 400, 432, 480,
- *     512, 576, 640, 768.
+ *     512, 576, 640, 768, 1024.
  *   - Shell membership: bin b holds the modes with n2lo[b] <= |n|^2 <= n2hi[b]
  *     (n = k / k_f integer vector, components in [-N/2, N/2]); the integer
  *     limits are the closed interval k_low <= |k| <= k_high with modes
