@@ -328,11 +328,19 @@ def run_batch(args, torch, np):
         cb.replay()
     torch.cuda.synchronize()
     el = (time.perf_counter() - t0) / reps
+    # the same batch launched eagerly on the branch streams (host-issued)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cb.run_eager()
+    torch.cuda.synchronize()
+    el_eager = (time.perf_counter() - t0) / reps
     res = {'pairs': len(pairs), 'triangles': cb.ntri, 'graph_ms': el * 1e3,
+           'eager_ms': el_eager * 1e3,
            'pairs_per_s': len(pairs) / el, 'replays': reps, 'branches': cb.nbranch,
            'tuned_classes': {'%d^3/S%d' % k: v for k, v in cb.tuned.items()},
            'layout': '32 @128^3, 24 @256^3, 8 @384^3, nb uniform [8,24], <AAB>'}
-    log('[batch] %d pairs in %.2f ms per graph replay' % (len(pairs), el * 1e3))
+    log('[batch] %d pairs in %.2f ms per replay of the branch graphs (%.2f ms eager)'
+        % (len(pairs), el * 1e3, el_eager * 1e3))
     del cb, pairs
     torch.cuda.empty_cache()
     return res

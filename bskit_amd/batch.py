@@ -4,10 +4,13 @@ The reference runs one measure_bispectrum_faster-style job per grid pair
 (scripts/measure/measure_bs_fast.py:231-260, roles <AAB> of
 bskit/main.py:624-640).  ``CrossBatch`` takes a list of pairs (A, B fields
 with their own mesh size and bins), preallocates every buffer, and records
-the forward transforms, shells and <AAB> triple sums of all pairs into one
-torch.cuda.CUDAGraph (a hipGraph on ROCm).  Pairs are spread over branch
-streams forked from the capture stream, so the graph holds independent
-chains that the device overlaps; one ``replay()`` measures the whole batch.
+the forward transforms, shells and <AAB> triple sums of all pairs into
+a hipGraph (torch.cuda.CUDAGraph).  Pairs are spread over ``nbranch``
+streams by estimated cost (longest first onto the least loaded stream),
+forked from and joined to the capture stream, so the graph holds one
+independent chain per branch; ``per_branch=True`` records one graph per
+branch stream instead, launched each on its stream.  One ``replay()`` runs
+the whole batch.
 
 Graph rules kept here: every buffer (spectra, shells, outputs, workspaces,
 triangle lists, MFMA layouts) exists before capture, and nothing in the
@@ -39,6 +42,8 @@ class _Pair:
         # level layout: write only the parts of each shell the sums read
         self.parts = (eng.shell_parts(tri, 2) if path == PATH_MFMA and
                       self.sets[0].layout is not None else None)
+        # relative cost: two forward transforms, two shell sets, the sums
+        self.cost = 2.0 * eng.N ** 3 + 2.0 * eng.nb * eng.ncell + len(tri) * eng.ncell / 64.0
 
     def enqueue(self, work):
         eng = self.eng
@@ -52,11 +57,13 @@ class _Pair:
 class CrossBatch:
     """pairs: list of (A, B, box, kLU) with A, B float32 device N^3 fields."""
 
-    def __init__(self, pairs, device=None, nbranch=4, autotune=True):
+    def __init__(self, pairs, device=None, nbranch=4, autotune=True, per_branch=False):
         """autotune: tune the radix plans and the pair kernel's workgroup count
         per mesh class (Nmesh, shell mesh) on the first pair of the class
-        (bskit_amd.tune.tune_class) before anything is recorded."""
+        (bskit_amd.tune.tune_class) before anything is recorded.  per_branch:
+        one graph per branch stream instead of one graph of every branch."""
         self.device = require_device(device)
+        self.per_branch = bool(per_branch)
         self.tuned = {}
         self.nbranch = max(1, min(int(nbranch), len(pairs)))
         self.engines = {}
@@ -79,39 +86,73 @@ class CrossBatch:
             self.pairs.append(_Pair(eng, a, b, tri, path, float(box)))
         self.branches = [torch.cuda.Stream(device=self.device) for _ in range(self.nbranch)]
         self.works = [Workspace(self.device) for _ in range(self.nbranch)]
-        self.graph = None
+        # longest first onto the least loaded branch; cost ~ the cells the
+        # pair's transforms, shells and triangle sums touch
+        load = [0.0] * self.nbranch
+        self.assign = [[] for _ in range(self.nbranch)]
+        for i in sorted(range(len(self.pairs)), key=lambda i: -self.pairs[i].cost):
+            b = min(range(self.nbranch), key=lambda k: (load[k], k))
+            self.assign[b].append(i)
+            load[b] += self.pairs[i].cost
+        self.graphs = None
 
-    def _enqueue_all(self):
-        main = torch.cuda.current_stream(self.device)
-        for s in self.branches:
-            s.wait_stream(main)
-        for i, p in enumerate(self.pairs):
-            bi = i % self.nbranch
-            with torch.cuda.stream(self.branches[bi]):
-                p.enqueue(self.works[bi])
-        for s in self.branches:
-            main.wait_stream(s)
+    def _enqueue_branch(self, b):
+        for i in self.assign[b]:
+            self.pairs[i].enqueue(self.works[b])
 
     def run_eager(self):
-        """Launch the batch without a graph (also sizes workspaces and caches)."""
-        self._enqueue_all()
+        """Launch the batch without graphs on the branch streams (also sizes
+        workspaces and caches)."""
+        main = torch.cuda.current_stream(self.device)
+        for b, s in enumerate(self.branches):
+            s.wait_stream(main)
+            with torch.cuda.stream(s):
+                self._enqueue_branch(b)
+        for s in self.branches:
+            main.wait_stream(s)
         return [p.out for p in self.pairs]
 
     def capture(self):
-        """Record the batch into one graph (after one eager run)."""
+        """Record the batch (after one eager run): one graph whose branches
+        fork from and join the capture stream (``per_branch`` False), or one
+        graph per branch stream."""
         self.run_eager()
         torch.cuda.synchronize(self.device)
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            self._enqueue_all()
-        self.graph = g
-        return g
+        graphs = []
+        if self.per_branch:
+            for b, s in enumerate(self.branches):
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=s):
+                    self._enqueue_branch(b)
+                graphs.append(g)
+        else:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.run_eager()
+            graphs.append(g)
+        torch.cuda.synchronize(self.device)
+        self.graphs = graphs
+        return graphs
 
     def replay(self):
-        if self.graph is None:
+        """Replay the batch; the caller's stream waits for all of it."""
+        if self.graphs is None:
             self.capture()
-        self.graph.replay()
+        if not self.per_branch:
+            self.graphs[0].replay()
+            return [p.out for p in self.pairs]
+        main = torch.cuda.current_stream(self.device)
+        for g, s in zip(self.graphs, self.branches):
+            s.wait_stream(main)
+            with torch.cuda.stream(s):
+                g.replay()
+        for s in self.branches:
+            main.wait_stream(s)
         return [p.out for p in self.pairs]
+
+    @property
+    def graph(self):
+        return self.graphs
 
     @property
     def ntri(self):

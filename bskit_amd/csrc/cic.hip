@@ -14,9 +14,9 @@
 // painted mesh is the same bits in every run.
 //
 // Tiled path (N a multiple of 16): the mesh is cut into T^3 tiles (T = 16).
-//  1. k_cic_bin: each particle's base node floor(u) (wrapped) and fractions;
-//     its tile's count gets one atomic increment, whose return value is the
-//     particle's slot inside its tile.
+//  1. k_cic_bin: each particle's base node floor(u) (wrapped) and its tile;
+//     the particles of a wave that share a tile take consecutive slots inside
+//     it from one atomic add on the tile's count.
 //  2. k_cic_scan: exclusive scan of the tile counts (one workgroup).
 //  3. k_cic_place: the particle's record (base node within the tile, three
 //     fractions) goes to offset[tile] + slot: particles grouped by tile.
@@ -102,17 +102,37 @@ __global__ __launch_bounds__(256) void k_fill_u64(unsigned long long* g, long lo
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) g[i] = 0;
 }
 
+// Slots are taken per wave: the lanes of a wave holding particles of one tile
+// (neighbouring particles of a catalogue usually share tiles) get consecutive
+// slots from a single atomic add of their count, so a tile's counter sees
+// one atomic per wave instead of one per particle.
 __global__ __launch_bounds__(256) void k_cic_bin(const float* __restrict__ pos, long long np, int N,
                                                  float to_mesh, uint32_t* __restrict__ count,
                                                  uint32_t* __restrict__ slot) {
   const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  int b[3];
-  float f[3];
-  particle(pos, p, N, to_mesh, b, f);
-  const int nt = N / kT;
-  const int tile = ((b[0] / kT) * nt + b[1] / kT) * nt + b[2] / kT;
-  slot[p] = atomicAdd(count + tile, 1u);
+  const bool valid = p < np;
+  int tile = -1;
+  if (valid) {
+    int b[3];
+    float f[3];
+    particle(pos, p, N, to_mesh, b, f);
+    const int nt = N / kT;
+    tile = ((b[0] / kT) * nt + b[1] / kT) * nt + b[2] / kT;
+  }
+  const int lane = __lane_id();
+  unsigned long long todo = __ballot(valid);  // wave-uniform
+  uint32_t mine = 0;
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int t = __shfl(tile, leader);
+    const unsigned long long same = __ballot(valid && tile == t) & todo;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(count + t, (uint32_t)__popcll(same));
+    base = __shfl(base, leader);
+    if ((same >> lane) & 1ull) mine = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  if (valid) slot[p] = mine;
 }
 
 // exclusive scan of n counts into off[0..n] (one workgroup of 1024 threads)
