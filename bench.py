@@ -593,7 +593,12 @@ def main():
                       for pm, w0, wn, *_ in plan.segments)
         else:
             alg = nsh * 4 * ncell
-        return roofline('k_triple_pairs', ms, alg, plan.npass, flops_sep, plan)
+        # launches: a segment's item-list passes run as one launch when the
+        # workgroup count is a multiple of 8 and its lists are not replicated
+        nwg = eng.mfma_nwg()
+        launches = sum(1 if (npass > 1 and rep == 1 and nwg % 8 == 0) else npass
+                       for _, _, _, npass, _, rep in plan.segments)
+        return roofline('k_triple_pairs', ms, alg, launches, flops_sep, plan)
 
     roof = path_roofline(main_path, float(np.mean(timer.pair_durations('kernel'))))
     if alt is not None:

@@ -1406,13 +1406,21 @@ int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
     a.accumulate = g > 0;
     a.rep = q->seg_rep[g];
     const int ni = q->seg_ni[g];
-    for (int k = 0; k < q->seg_pass[g]; ++k, ++p, e0 += (size_t)kPairWaves * ni) {
-      a.items = q->d_items + e0;
-      a.outpos = q->d_outpos + e0;
+    const int np = q->seg_pass[g];
+    // a segment's passes (item lists over the same cells) in one launch,
+    // the workgroups of one cell range on one XCD, when the count allows
+    const bool fold = np > 1 && a.rep == 1 && nwg % 8 == 0;
+    a.npass = fold ? np : 1;
+    a.pass_stride = kPairWaves * ni;
+    for (int k = 0; k < np; k += a.npass) {
+      a.items = q->d_items + e0 + (size_t)k * kPairWaves * ni;
+      a.outpos = q->d_outpos + e0 + (size_t)k * kPairWaves * ni;
       if (a.ncell == 0) continue;
       int rc = bsk_triple_pairs_launch(a, q->sets, ni, s);
       if (rc) return rc;
     }
+    p += np;
+    e0 += (size_t)np * kPairWaves * ni;
   }
   if (!out_dev) return BSK_OK;  // partials only; bsk_triples_mfma_reduce finishes
   return bsk_reduce_mfma_launch(a.partial, nwg, q->item_total, q->ntri, q->d_slot, scale,

@@ -76,6 +76,14 @@ struct PairArgs {
   const int* pairs;       // [tiles + 1][16]: (row in set 0) | (row in set 1) << 16
   float* partial;         // [nwg][item_total][256]
   int item_total;         // items per workgroup in the partial layout
+  // item-list passes run by one launch (1: one list); with npass > 1 the
+  // launch has npass * nwg workgroups in groups of 8 * npass, group g's
+  // workgroup 8 p + x running pass p (list at items + p * pass_stride) over
+  // cell range 8 g + x: the workgroups sharing a cell range are on one XCD
+  // (workgroups go round-robin over the 8 XCDs), so the later passes read
+  // the staged shells from that XCD's L2
+  int npass;
+  int pass_stride;
 };
 int bsk_pair_items_round(int ni);  // compiled items-per-wave count >= ni (0: too many)
 int bsk_pair_item_cap(int sets, int nshell);  // items per wave that compile without spills

@@ -166,16 +166,23 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
   uint16_t* const hKs = hK + a.koff * kLDH;  // row of staged third-side shell 0
   uint16_t* const lKs = lK + a.koff * kLDH;
 
+  // the cell range (lw) and item-list pass of this workgroup
+  int lw = blockIdx.x, pass = 0;
+  if (a.npass > 1) {
+    pass = (blockIdx.x >> 3) % a.npass;
+    lw = (blockIdx.x / (8 * a.npass)) * 8 + (blockIdx.x & 7);
+  }
   // item descriptors and partial positions of this wave and the LDS rows of
   // this lane's pair
   int desc[NI], oij[NI], opos[NI];
-  setup<NI, kLDF>(desc, oij, opos, a.items, a.outpos, a.pairs, wave, col, a.lo_row, n0);
+  setup<NI, kLDF>(desc, oij, opos, a.items + pass * a.pass_stride,
+                  a.outpos + pass * a.pass_stride, a.pairs, wave, col, a.lo_row, n0);
   State<NI> st;
   init(st);
 
   const long long nch = a.ncell / kCK;
-  const long long ch0 = nch * blockIdx.x / a.nwg;
-  const long long ch1 = nch * (blockIdx.x + 1) / a.nwg;
+  const long long ch0 = nch * lw / a.nwg;
+  const long long ch1 = nch * (lw + 1) / a.nwg;
   Pref<PV> p0, p1, p2;
   if (ch0 < ch1) {
     fetch(p0, a.hi[0], a.lo[0], a.stride, n0, ch0 * kCK);
@@ -200,7 +207,7 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
         kstep<NI, kLDH>(st, desc, oij, fA, fB, hK, lK, ks * 32 + grp * 4, ks * 32 + grp * 8, col,
                         kc);
   }
-  float* const out = a.partial + (long long)blockIdx.x * a.item_total * 256;
+  float* const out = a.partial + (long long)lw * a.item_total * 256;
   if (rep == 1) {
     store_at(st, out, opos, a.accumulate != 0, grp, col);
     return;
@@ -271,7 +278,7 @@ int launch_pairs(const PairArgs& a, size_t lds, hipStream_t s) {
   static const hipError_t attr = hipFuncSetAttribute(
       (const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   BSK_HIP_CHECK(attr);
-  hipLaunchKernelGGL(fn, dim3(a.nwg), dim3(kNT), lds, s, a);
+  hipLaunchKernelGGL(fn, dim3(a.nwg * a.npass), dim3(kNT), lds, s, a);
   BSK_LAUNCH_CHECK();
   return BSK_OK;
 }
@@ -336,6 +343,10 @@ int bsk_triple_pairs_launch(const PairArgs& a, int sets, int ni, hipStream_t s) 
   }
   if (a.nwg <= 0 || a.stride % 8) {
     bsk_set_error("mfma triple: nwg > 0 and shell strides in multiples of 8 elements");
+    return BSK_EINVAL;
+  }
+  if (a.npass < 1 || (a.npass > 1 && (a.nwg % 8 || a.rep != 1))) {
+    bsk_set_error("mfma triple: folded passes need nwg a multiple of 8 and no replicas");
     return BSK_EINVAL;
   }
   int nmax = a.nsh[0];
