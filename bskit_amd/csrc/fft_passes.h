@@ -551,7 +551,9 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   auto prow = [&](int p) { return p < n1 ? r1 + step * p : r2 + step * (p - n1); };
   // (k, NH - k) pairs: lane k of rows r0, r0 + RK, ...; up to 8 rows (16
   // loads) in flight per batch
-  const int r0 = tid / KQ;
+  // with at least a wave of lanes per row every lane of a wave has the same
+  // row: make that explicit so the row index math runs in scalar registers
+  const int r0 = KQ >= 64 ? __builtin_amdgcn_readfirstlane(tid / KQ) : tid / KQ;
   constexpr int B = NPK < 8 ? NPK : 8;
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks) {
@@ -587,7 +589,8 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   const long long obase = (bl * shell_stride + cell0 + row0 * N) * esz;
   char* const ohi = static_cast<char*>(out) + obase;
   char* const olo = out_lo ? static_cast<char*>(out_lo) + obase : nullptr;
-  const int q = tid & (QQ - 1), rq = tid / QQ;
+  const int q = tid & (QQ - 1);
+  const int rq = QQ >= 64 ? __builtin_amdgcn_readfirstlane(tid / QQ) : tid / QQ;  // see r0
   if (q < NH / 2) {
 #pragma unroll
     for (int i = 0; i < NPQ; ++i) {
