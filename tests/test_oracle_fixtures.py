@@ -332,3 +332,44 @@ def test_binned_direct_oracle():
     half = np.fft.rfftn(field) / N ** 3
     assert abs(np.sum(w * np.abs(half) ** 2) - np.sum(np.abs(dk) ** 2)) < 1e-12
     assert est.binned_direct(field, box, everything)[1] > 0
+
+
+def test_cross_triangle_lists_by_independent_enumeration():
+    """The <AAB> and <ABC> lists (bskit/main.py:1355-1389) against plain-loop
+    enumerations written from their rules: <AAB> pairs the sides (k1, k2)
+    with k1 >= k2 and takes any k3; <ABC> takes every ordered triple; each
+    keeps the triples whose bins can close (every side's lower edge at most
+    the sum of the other two upper edges) and lists them by (k1, k2, k3) bin,
+    k1 varying slowest.  Product and oracle generators both checked."""
+    from bskit_amd import bins as pb
+    kf = est.kf_of(1000.)
+    counts = {}
+    for nb, w, lowk, dkh in ((8, 2.0, 0, -1.0), (16, 3.0, 0, -1.0), (7, 2.0, 3, 6.0)):
+        kw = dict(kmin=kf, kmax=kf + nb * w * kf + w * kf / 2, dk=w * kf)
+        if lowk:
+            kw.update(num_lowk_bins=lowk, dk_high=dkh * kf)
+        e = ob.bin_edges(kw['kmin'], kw['kmax'], kw['dk'], kw.get('num_lowk_bins', 0),
+                         kw.get('dk_high', -1.0))
+        n = len(e)
+
+        def closes(a, b, c):
+            return (e[a][1] + e[b][1] >= e[c][0] and e[a][1] + e[c][1] >= e[b][0]
+                    and e[b][1] + e[c][1] >= e[a][0])
+
+        aab = sorted((j, i, k) for i in range(n) for j in range(i, n) for k in range(n)
+                     if closes(j, i, k))
+        abc = sorted((i, j, k) for i in range(n) for j in range(n) for k in range(n)
+                     if closes(i, j, k))
+        for gen in (pb.generate_triangle_bin_list, ob.all_triangles):
+            if gen is ob.all_triangles:
+                get = lambda nf: gen(kw['kmin'], kw['kmax'], kw['dk'], nf,
+                                     kw.get('num_lowk_bins', 0), kw.get('dk_high', -1.0),
+                                     return_indices=True)
+            else:
+                get = lambda nf: gen(num_fields=nf, return_indices=True, **kw)
+            assert [tuple(r) for r in np.asarray(get(2)).tolist()] == aab
+            assert [tuple(r) for r in np.asarray(get(3)).tolist()] == abc
+        assert all(r[0] >= r[1] for r in aab)
+        counts[(nb, w)] = len(aab)
+    # <AAB> counts at the 8 x 2k_f and 16 x 3k_f binnings
+    assert counts[(8, 2.0)] == 231 and counts[(16, 3.0)] == 1469
