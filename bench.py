@@ -334,8 +334,34 @@ def run_batch(args, torch, np):
         cb.run_eager()
     torch.cuda.synchronize()
     el_eager = (time.perf_counter() - t0) / reps
+    # algorithmic HBM bytes of one batch: per pair, the two forward
+    # transforms (field read, compact spectrum written), the two shell sets
+    # (spectrum read, the layout parts written) and the pair kernel's staged
+    # shell reads (each segment's window once); the whole batch over one
+    # replay against the HBM peak
+    alg = 0.0
+    for p in cb.pairs:
+        eng = p.eng
+        H = eng.MX * eng.MY * eng.MZ
+        alg += 2 * (4.0 * eng.N ** 3 + 8.0 * H)
+        if p.path == 'mfma':
+            plan = eng.pair_plan(p.tri, eng.nb, 1, levels=p.sets[0].layout is not None)
+            if p.sets[0].layout is not None:
+                parts = eng.layout.part_cells()
+                written = sum(4.0 * sum(parts[q] for q in range(4) if pm >> q & 1)
+                              for pm in p.parts.host)
+                alg += sum(4.0 * wn * sum(parts[q] for q in range(4) if pm >> q & 1)
+                           for pm, w0, wn, *_ in plan.segments)
+            else:
+                written = 4.0 * eng.nb * eng.ncell
+                alg += 4.0 * eng.nb * eng.ncell
+            alg += 2 * (8.0 * H + written)
+    roof = {'kernel': 'whole batch (forward, shells and pair kernels of every pair)',
+            'bound': 'hbm', 'achieved': alg / el / 1e9, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+            'frac': alg / el / 1e9 / HBM_PEAK_GBS, 'traffic': None,
+            'algorithmic_bytes': alg}
     res = {'pairs': len(pairs), 'triangles': cb.ntri, 'graph_ms': el * 1e3,
-           'eager_ms': el_eager * 1e3,
+           'eager_ms': el_eager * 1e3, 'roofline': roof,
            'pairs_per_s': len(pairs) / el, 'replays': reps, 'branches': cb.nbranch,
            'tuned_classes': {'%d^3/S%d' % k: v for k, v in cb.tuned.items()},
            'layout': '32 @128^3, 24 @256^3, 8 @384^3, nb uniform [8,24], <AAB>'}
