@@ -182,12 +182,12 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
   if (stages & BSK_STAGE_Y)  // y only: D2 stays in the workspace
     return bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, nsh, p->d_hi, shell0, s);
   if (!(stages & BSK_STAGE_YZ)) return BSK_OK;
-  rc = bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, nsh, p->d_hi, shell0, s);
-  if (rc) return rc;
   // shells [shell0, shell0+nsh) of the output arrays, cells of the slab
   const size_t esz = (fmt == BSK_FMT_F64) ? 8 : (fmt == BSK_FMT_F32 ? 4 : 2);
   char* o = reinterpret_cast<char*>(out) + (size_t)shell0 * stride * esz;
   char* ol = out_lo ? reinterpret_cast<char*>(out_lo) + (size_t)shell0 * stride * esz : nullptr;
+  rc = bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, nsh, p->d_hi, shell0, s);
+  if (rc) return rc;
   return bsk_inv_z<T>(N, p->variant[5], d2, o, ol, tw, g.MZ, g.MZP, (long long)xs * N, cell0,
                       stride, fmt, nsh, p->d_hi, shell0, (stages & BSK_STAGE_LEVELS) ? 1 : 0,
                       parts, s);
@@ -1388,7 +1388,7 @@ int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
   a.item_total = q->item_total;
   hipStream_t s = as_stream(stream);
   size_t e0 = 0;  // first list entry of the launch
-  for (int g = 0, p = 0; g < q->nseg; ++g) {
+  for (int g = 0; g < q->nseg; ++g) {
     // the segment's cells (its parts are consecutive) of its shell window
     const int fp = __builtin_ctz(q->seg_parts[g]), lp = 31 - __builtin_clz(q->seg_parts[g]);
     const long long c0 = pbeg[fp], c1 = pend[lp];
@@ -1419,7 +1419,6 @@ int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
       int rc = bsk_triple_pairs_launch(a, q->sets, ni, s);
       if (rc) return rc;
     }
-    p += np;
     e0 += (size_t)np * kPairWaves * ni;
   }
   if (!out_dev) return BSK_OK;  // partials only; bsk_triples_mfma_reduce finishes

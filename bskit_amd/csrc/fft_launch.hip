@@ -20,6 +20,11 @@ template <typename T, int N>
 constexpr int pxy() {
   return (sizeof(T) == 8 || N > 640) ? 8 : 16;
 }
+// The forward y/x passes from 512 up: 8 (more workgroups per CU)
+template <typename T, int N>
+constexpr int pfwd() {
+  return N >= 512 ? 8 : pxy<T, N>();
+}
 // Rows per workgroup for the contiguous z passes: the power of two nearest
 // below about 4096 (FP64: 2048) complex values per workgroup, so the
 // butterfly -> (row, column) split of the LDS stages is a mask and a shift.
@@ -82,7 +87,7 @@ template int bsk_fwd_z<double>(int, int, const double*, cplx<double>*, const cpl
 namespace {
 template <typename T, int N, int V>
 void fwd_y_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, hipStream_t s) {
-  constexpr int P = pxy<T, N>();
+  constexpr int P = pfwd<T, N>();
   hipLaunchKernelGGL((k_fwd_y<T, N, V, P, kW>), dim3(cdiv(g.MZ, P), N), dim3(kW), 0, s, in, out,
                      tw, g);
 }
@@ -105,7 +110,7 @@ namespace {
 template <typename T, int N, int V>
 void fwd_x_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, T scale, const T* comp,
              hipStream_t s) {
-  constexpr int P = pxy<T, N>();
+  constexpr int P = pfwd<T, N>();
   hipLaunchKernelGGL((k_fwd_x<T, N, V, P, kW>), dim3(cdiv(g.MZ, P), g.MY), dim3(kW), 0, s, in,
                      out, tw, g, scale, comp);
 }

@@ -341,6 +341,10 @@ int bsk_triple_pairs_launch(const PairArgs& a, int sets, int ni, hipStream_t s) 
     bsk_set_error("mfma triple: cell count %lld not a multiple of %d", a.ncell, kCK);
     return BSK_EINVAL;
   }
+  if (a.nsh[0] < 1 || a.nsh[1] < 1 || a.nsh[2] < 1) {
+    bsk_set_error("mfma triple: every role set needs a shell");
+    return BSK_EINVAL;
+  }
   if (a.nwg <= 0 || a.stride % 8) {
     bsk_set_error("mfma triple: nwg > 0 and shell strides in multiples of 8 elements");
     return BSK_EINVAL;
