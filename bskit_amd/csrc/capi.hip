@@ -38,6 +38,14 @@ struct bsk_plan {
   int variant[6];
   int* d_lo = nullptr;
   int* d_hi = nullptr;
+  // block entries of the inverse x and y passes (the blocks that hold modes
+  // of each shell; [0]: 16-pencil blocks, [1]: 8-pencil blocks), shells in
+  // order, prefix[b] = first entry of shell b
+  struct BlockList {
+    int2* d = nullptr;
+    std::vector<int> prefix;
+  };
+  BlockList xblk[2], yblk[2];
   cplx<float>* tw_f = nullptr;
   cplx<double>* tw_d = nullptr;
   cplx<float>* tws_f = nullptr;  // exp(-2 pi i t / S) when S != N
@@ -153,6 +161,56 @@ int forward_t(bsk_plan* p, const T* field, cplx<T>* dk, void* work, hipStream_t 
   return bsk_fwd_x<T>(N, p->variant[2], e2, dk, tw, g, scale, comp, s);
 }
 
+// Host twin of shell_mz: kz from this value on hold no mode of a shell with
+// |n|^2 <= hi.
+int host_shell_mz(int MZ, int hi) {
+  if (hi < 0) return 0;
+  long long r = (long long)std::sqrt((double)hi);
+  while (r * r > hi) --r;
+  while ((r + 1) * (r + 1) <= hi) ++r;
+  return (int)std::min<long long>(r + 1, MZ);
+}
+
+int upload_blocks(bsk_plan::BlockList& bl, const std::vector<int2>& v) {
+  if (v.empty()) return BSK_OK;
+  hipError_t e = hipMalloc(&bl.d, sizeof(int2) * v.size());
+  if (e == hipSuccess) e = hipMemcpy(bl.d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    bsk_set_error("plan block lists: %s", hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? BSK_ENOMEM : BSK_EHIP;
+  }
+  return BSK_OK;
+}
+
+// The workgroups of the inverse x and y passes that hold modes: x blocks
+// (ry, kz0) with ny^2 + kz0^2 <= n2hi (the kernel's own skip test), y blocks
+// kz0 < mz_b.  Independent of the shell mesh.
+int build_block_lists(bsk_plan* p) {
+  const Geom& g = p->g;
+  for (int k = 0; k < 2; ++k) {
+    const int P = k ? 8 : 16;
+    std::vector<int2> xl, yl;
+    p->xblk[k].prefix.assign(p->nbin + 1, 0);
+    p->yblk[k].prefix.assign(p->nbin + 1, 0);
+    for (int b = 0; b < p->nbin; ++b) {
+      p->xblk[k].prefix[b] = (int)xl.size();
+      p->yblk[k].prefix[b] = (int)yl.size();
+      const int hi = p->n2hi[b];
+      for (int kz0 = 0; kz0 < host_shell_mz(g.MZ, hi); kz0 += P) yl.push_back(make_int2(b, kz0));
+      for (int ry = 0; ry < g.MY; ++ry) {
+        const long long ny = g.nlo + ry;
+        for (int kz0 = 0; kz0 < g.MZ; kz0 += P)
+          if (ny * ny + (long long)kz0 * kz0 <= hi) xl.push_back(make_int2(b, (ry << 16) | kz0));
+      }
+    }
+    p->xblk[k].prefix[p->nbin] = (int)xl.size();
+    p->yblk[k].prefix[p->nbin] = (int)yl.size();
+    if (int rc = upload_blocks(p->xblk[k], xl)) return rc;
+    if (int rc = upload_blocks(p->yblk[k], yl)) return rc;
+  }
+  return BSK_OK;
+}
+
 template <typename T>
 int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int x0, int xs,
              int stages, int fmt, void* out, void* out_lo, long long stride, long long cell0,
@@ -174,19 +232,26 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
     return BSK_EUNSUPPORTED;
   }
   int rc = BSK_OK;
+  // the block entries of shells [shell0, shell0 + nsh) for this pencil count
+  const int kb = bsk_pxy(sizeof(T) == 8, N) == 8 ? 1 : 0;
+  const bsk_plan::BlockList &XB = p->xblk[kb], &YB = p->yblk[kb];
+  const int2* xb = XB.d + XB.prefix[shell0];
+  const int nxb = XB.prefix[shell0 + nsh] - XB.prefix[shell0];
+  const int2* yb = YB.d + YB.prefix[shell0];
+  const int nyb = YB.prefix[shell0 + nsh] - YB.prefix[shell0];
   if (stages & BSK_STAGE_X) {
-    rc = bsk_inv_x<T>(N, p->variant[3], dk, d1, tw, g, p->d_lo, p->d_hi, shell0, nsh, kind, kf,
-                      (T)p->kpower, s);
+    rc = bsk_inv_x<T>(N, p->variant[3], dk, d1, tw, g, p->d_lo, p->d_hi, shell0, xb, nxb, kind,
+                      kf, (T)p->kpower, s);
     if (rc) return rc;
   }
   if (stages & BSK_STAGE_Y)  // y only: D2 stays in the workspace
-    return bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, nsh, p->d_hi, shell0, s);
+    return bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, yb, nyb, p->d_hi, shell0, s);
   if (!(stages & BSK_STAGE_YZ)) return BSK_OK;
   // shells [shell0, shell0+nsh) of the output arrays, cells of the slab
   const size_t esz = (fmt == BSK_FMT_F64) ? 8 : (fmt == BSK_FMT_F32 ? 4 : 2);
   char* o = reinterpret_cast<char*>(out) + (size_t)shell0 * stride * esz;
   char* ol = out_lo ? reinterpret_cast<char*>(out_lo) + (size_t)shell0 * stride * esz : nullptr;
-  rc = bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, nsh, p->d_hi, shell0, s);
+  rc = bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, yb, nyb, p->d_hi, shell0, s);
   if (rc) return rc;
   return bsk_inv_z<T>(N, p->variant[5], d2, o, ol, tw, g.MZ, g.MZP, (long long)xs * N, cell0,
                       stride, fmt, nsh, p->d_hi, shell0, (stages & BSK_STAGE_LEVELS) ? 1 : 0,
@@ -312,6 +377,7 @@ int bsk_plan_create(bsk_plan** out, int nmesh, double box_size, int nbin, const 
   }
   rc = make_twiddles<float>(N, &p->tw_f);
   if (!rc) rc = make_twiddles<double>(N, &p->tw_d);
+  if (!rc) rc = build_block_lists(p);
   if (!rc) rc = set_shell_mesh(p, auto_shell_mesh(N, m));
   if (rc) {
     bsk_plan_destroy(p);
@@ -325,6 +391,10 @@ int bsk_plan_destroy(bsk_plan* p) {
   if (!p) return BSK_OK;
   (void)hipFree(p->d_lo);
   (void)hipFree(p->d_hi);
+  for (int k = 0; k < 2; ++k) {
+    (void)hipFree(p->xblk[k].d);
+    (void)hipFree(p->yblk[k].d);
+  }
   (void)hipFree(p->tw_f);
   (void)hipFree(p->tw_d);
   (void)hipFree(p->tws_f);

@@ -13,12 +13,9 @@ namespace {
 
 constexpr int kW = 256;
 
-// Pencils per workgroup for the strided x/y passes: 16 consecutive kz values
-// (128 B in FP32) per grid point; FP64 and lengths above 640 use 8 to stay
-// inside the LDS budget.
 template <typename T, int N>
 constexpr int pxy() {
-  return (sizeof(T) == 8 || N > 640) ? 8 : 16;
+  return bsk_pxy(sizeof(T) == 8, N);
 }
 // The forward y/x passes from 512 up: 8 (more workgroups per CU)
 template <typename T, int N>
@@ -36,7 +33,9 @@ constexpr int pz() {
   return q;
 }
 
-inline unsigned cdiv(long long a, long long b) { return (unsigned)((a + b - 1) / b); }
+[[maybe_unused]] inline unsigned cdiv(long long a, long long b) {
+  return (unsigned)((a + b - 1) / b);
+}
 
 }  // namespace
 
@@ -133,50 +132,55 @@ template int bsk_fwd_x<double>(int, int, const cplx<double>*, cplx<double>*, con
 namespace {
 template <typename T, int N, int V>
 void inv_x_n(const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g, const int* lo,
-             const int* hi, int shell0, int nsh, int kind, T kf, T kp, hipStream_t s) {
+             const int* hi, int shell0, const int2* blocks, int nblocks, int kind, T kf, T kp,
+             hipStream_t s) {
   constexpr int P = pxy<T, N>();
-  hipLaunchKernelGGL((k_inv_x_mask<T, N, V, P, kW>), dim3(cdiv(g.MZ, P), g.MY, nsh), dim3(kW),
-                     0, s, dk, out, tw, g, lo, hi, shell0, kind, kf, kp);
+  hipLaunchKernelGGL((k_inv_x_mask<T, N, V, P, kW>), dim3(nblocks), dim3(kW), 0, s, dk, out, tw,
+                     g, lo, hi, shell0, blocks, kind, kf, kp);
 }
 }  // namespace
 template <typename T>
 int bsk_inv_x(int N, int V, const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g,
-              const int* lo, const int* hi, int shell0, int nsh, int kind, T kf, T kp,
-              hipStream_t s) {
-#define C(n) BSK_VCASE(n, (inv_x_n<T, n, 0>(dk, out, tw, g, lo, hi, shell0, nsh, kind, kf, kp, s)), \
-                       (inv_x_n<T, n, 1>(dk, out, tw, g, lo, hi, shell0, nsh, kind, kf, kp, s)))
+              const int* lo, const int* hi, int shell0, const int2* blocks, int nblocks, int kind,
+              T kf, T kp, hipStream_t s) {
+  if (nblocks <= 0) return BSK_OK;  // no shell holds a mode
+#define C(n) BSK_VCASE(n, (inv_x_n<T, n, 0>(dk, out, tw, g, lo, hi, shell0, blocks, nblocks, kind, kf, kp, s)), \
+                       (inv_x_n<T, n, 1>(dk, out, tw, g, lo, hi, shell0, blocks, nblocks, kind, kf, kp, s)))
   BSK_SWITCH(T, N, V, C)
 #undef C
 }
 template int bsk_inv_x<float>(int, int, const cplx<float>*, cplx<float>*, const cplx<float>*, Geom,
-                              const int*, const int*, int, int, int, float, float, hipStream_t);
+                              const int*, const int*, int, const int2*, int, int, float, float,
+                              hipStream_t);
 template int bsk_inv_x<double>(int, int, const cplx<double>*, cplx<double>*, const cplx<double>*,
-                               Geom, const int*, const int*, int, int, int, double, double,
-                               hipStream_t);
+                               Geom, const int*, const int*, int, const int2*, int, int, double,
+                               double, hipStream_t);
 #endif
 
 #if BSK_PASS == 4
 namespace {
 template <typename T, int N, int V>
-void inv_y_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0, int xs, int nsh,
-             const int* n2hi, int shell0, hipStream_t s) {
+void inv_y_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0, int xs,
+             const int2* blocks, int nblocks, const int* n2hi, int shell0, hipStream_t s) {
   constexpr int P = pxy<T, N>();
-  hipLaunchKernelGGL((k_inv_y<T, N, V, P, kW>), dim3(cdiv(g.MZ, P), xs, nsh), dim3(kW), 0, s, in,
-                     out, tw, g, x0, xs, n2hi, shell0);
+  hipLaunchKernelGGL((k_inv_y<T, N, V, P, kW>), dim3(nblocks, xs), dim3(kW), 0, s, in, out, tw, g,
+                     x0, xs, blocks, n2hi, shell0);
 }
 }  // namespace
 template <typename T>
 int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0,
-              int xs, int nsh, const int* n2hi, int shell0, hipStream_t s) {
-#define C(n) BSK_VCASE(n, (inv_y_n<T, n, 0>(in, out, tw, g, x0, xs, nsh, n2hi, shell0, s)), \
-                       (inv_y_n<T, n, 1>(in, out, tw, g, x0, xs, nsh, n2hi, shell0, s)))
+              int xs, const int2* blocks, int nblocks, const int* n2hi, int shell0,
+              hipStream_t s) {
+  if (nblocks <= 0 || xs <= 0) return BSK_OK;  // no shell holds a mode
+#define C(n) BSK_VCASE(n, (inv_y_n<T, n, 0>(in, out, tw, g, x0, xs, blocks, nblocks, n2hi, shell0, s)), \
+                       (inv_y_n<T, n, 1>(in, out, tw, g, x0, xs, blocks, nblocks, n2hi, shell0, s)))
   BSK_SWITCH(T, N, V, C)
 #undef C
 }
 template int bsk_inv_y<float>(int, int, const cplx<float>*, cplx<float>*, const cplx<float>*, Geom,
-                              int, int, int, const int*, int, hipStream_t);
+                              int, int, const int2*, int, const int*, int, hipStream_t);
 template int bsk_inv_y<double>(int, int, const cplx<double>*, cplx<double>*, const cplx<double>*,
-                               Geom, int, int, int, const int*, int, hipStream_t);
+                               Geom, int, int, const int2*, int, const int*, int, hipStream_t);
 #endif
 
 #if BSK_PASS == 5

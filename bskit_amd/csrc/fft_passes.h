@@ -247,25 +247,29 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
 }
 
 // ------------------------------------------------- S1: masked inverse x FFT
-// grid (ceil(MZ/P), MY, nshell).  kind: delta / unit / |k| / |k|^kp.  Workgroups whose
-// pencils all lie outside the shell's disk ny^2 + kz^2 <= n2hi write nothing.
+// grid (nblocks): block entries (shell, ry << 16 | kz0) of the launch's
+// shells, only the P-pencil blocks that meet the shell's disk ny^2 + kz^2
+// <= n2hi (the rest of D1 is never read).  kind: delta / unit / |k| / |k|^kp.
+// The XCD remap gives each XCD a contiguous range of entries (neighbouring
+// kz blocks of one row).
 template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk,
                                                   cplx<T>* __restrict__ out,
                                                   const cplx<T>* __restrict__ twg, Geom g,
                                                   const int* __restrict__ n2lo,
                                                   const int* __restrict__ n2hi, int shell0,
-                                                  int kind, T kf, T kp) {
+                                                  const int2* __restrict__ blocks, int kind, T kf,
+                                                  T kp) {
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
-  const Blk bk = xcd_block();
-  const long long ry = bk.y;
-  const int kz0 = bk.x * P;
-  const int bl = bk.z;
+  const int2 ent = blocks[xcd_linear(blockIdx.x, gridDim.x)];
+  const long long ry = ent.y >> 16;
+  const int kz0 = ent.y & 0xffff;
+  const int bl = ent.x - shell0;
   const int lo = n2lo[shell0 + bl], hi = n2hi[shell0 + bl];
   const int ny = g.nlo + (int)ry;
-  if (ny * ny + kz0 * kz0 > hi) return;  // uniform: every pencil is zero
+  if (ny * ny + kz0 * kz0 > hi) return;  // uniform: every pencil is zero (not listed)
   load_twiddles<T, N, W>(tw, twg);
   const cplx<T>* src = dk + ry * g.MX * g.MZ;
   stage_in<T, P * N, W>(
@@ -305,24 +309,26 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
 }
 
 // -------------------------------------------------------- S2: inverse y FFT
-// grid (ceil(MZ/P), xs, nshell); x = x0 + logical block y.  Reads only the
-// shell's disk of D1; kz blocks from mz_b on are all zero and not written.
+// grid (nblocks, xs): block entries (shell, kz0) of the launch's shells,
+// only the kz blocks below the shell's mz_b (the rest of D2 is zero and
+// never read); x = x0 + logical block y.  Reads only the shell's disk of D1.
 template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
                                              cplx<T>* __restrict__ out,
                                              const cplx<T>* __restrict__ twg, Geom g, int x0,
-                                             int xs, const int* __restrict__ n2hi,
-                                             int shell0) {
+                                             int xs, const int2* __restrict__ blocks,
+                                             const int* __restrict__ n2hi, int shell0) {
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const Blk bk = xcd_block();
   const int xl = bk.y;
   const long long x = x0 + xl;
-  const int kz0 = bk.x * P;
-  const long long bl = bk.z;
+  const int2 ent = blocks[bk.x];
+  const int kz0 = ent.y;
+  const long long bl = ent.x - shell0;
   const int hi = n2hi[shell0 + bl];
-  if (kz0 >= shell_mz(g.MZ, hi)) return;  // uniform: zero block, never read
+  if (kz0 >= shell_mz(g.MZ, hi)) return;  // uniform: zero block (not listed)
   load_twiddles<T, N, W>(tw, twg);
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
   const cplx<T>* src = in + (bl * N + x) * g.MY * g.MZP + kz0;
