@@ -171,9 +171,17 @@ BSK_DEV void fft_lds(cplx<T>* buf, const cplx<T>* tw, Radices<Rs...>) {
 }
 
 // ------------------------------------------- runtime pencil count (np <= PMAX)
+// Quotient by a divisor n known only at run time: multiply-high with
+// m = ceil(2^32 / n), exact for dividends below 2^32 / n; n = 1 (whose m
+// does not fit 32 bits) is m = 0, the quotient the dividend itself.
+BSK_DEV unsigned div_magic(int n) {
+  return n > 1 ? (unsigned)((0x100000000ull + (unsigned)n - 1) / (unsigned)n) : 0u;
+}
+BSK_DEV int div_by_magic(unsigned b, unsigned m) { return m ? (int)__umulhi(b, m) : (int)b; }
+
 // The same stages over np pencils known only at run time: butterfly b maps to
-// column j = b / np and pencil p = b - j*np, the quotient by a multiply-high
-// with mdiv = ceil(2^32 / np) (exact for b < 2^32 / np, far above PMAX * L).
+// column j = b / np and pencil p = b - j*np, the quotient from
+// mdiv = div_magic(np) (exact for b < 2^32 / np, far above PMAX * L).
 template <typename T, int L, int R, int NS, bool INV, int PMAX, int W, int LD, int TWN>
 BSK_DEV void stockham_stage_n(cplx<T>* buf, const cplx<T>* tw, int np, unsigned mdiv, int tid) {
   constexpr int NB = L / R;
@@ -184,7 +192,7 @@ BSK_DEV void stockham_stage_n(cplx<T>* buf, const cplx<T>* tw, int np, unsigned 
   for (int q = 0; q < BPT; ++q) {
     const int b = tid + q * W;
     if (b < tot) {
-      const int j = (int)__umulhi((unsigned)b, mdiv), p = b - j * np;
+      const int j = div_by_magic((unsigned)b, mdiv), p = b - j * np;
       const cplx<T>* src = buf + p * LD + j;
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q][r] = src[r * NB];
@@ -195,7 +203,7 @@ BSK_DEV void stockham_stage_n(cplx<T>* buf, const cplx<T>* tw, int np, unsigned 
   for (int q = 0; q < BPT; ++q) {
     const int b = tid + q * W;
     if (b < tot) {
-      const int j = (int)__umulhi((unsigned)b, mdiv), p = b - j * np;
+      const int j = div_by_magic((unsigned)b, mdiv), p = b - j * np;
       const int k = j % NS;
       if (NS > 1) {
         const int ks = k * (TWN / (NS * R));

@@ -586,7 +586,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   __syncthreads();
   if (gather)
     fft_lds_n<T, NH, true, P, W, LD, N>(
-        buf, tw, np, (unsigned)((0x100000000ull + (unsigned)np - 1) / (unsigned)np), tid,
+        buf, tw, np, div_magic(np), tid,
         typename Plan<NH, V>::seq{});
   else
     fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});

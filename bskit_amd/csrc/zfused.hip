@@ -86,7 +86,7 @@ __global__ __launch_bounds__(kZT, 4) void k_triple_zrows(ZArgs a) {
 
   const long long r0 = a.nrows * blockIdx.x / a.nwg;
   const long long r1 = a.nrows * (blockIdx.x + 1) / a.nwg;
-  const unsigned mdiv = (unsigned)((0x100000000ull + (unsigned)nsh - 1) / (unsigned)nsh);
+  const unsigned mdiv = div_magic(nsh);
   __syncthreads();
 
   // this thread's mirrored-pair items (shell b, k) are the same for all

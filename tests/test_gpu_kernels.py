@@ -358,10 +358,11 @@ def test_mfma_random_lists_match_fp64(mode, nb, ntri):
 
 
 @pytest.mark.parametrize('N,nb,width,S', [(64, 8, 2, 64), (128, 12, 2, 80), (256, 8, 8, 200),
-                                          (256, 32, 2, 200)])
+                                          (256, 32, 2, 200), (64, 1, 22, 64)])
 def test_zfused_matches_separate_passes_and_fp64(N, nb, width, S):
     """The fused z c2r + split-BF16 sums against the separate z pass + MFMA
-    kernel (same shells up to FP32 summation order) and the FP64 plain path."""
+    kernel (same shells up to FP32 summation order) and the FP64 plain path;
+    also with one shell (a runtime pencil count of one in its transforms)."""
     from bskit_amd import _lib as L
     from bskit_amd import bins
     box = 1000.0
