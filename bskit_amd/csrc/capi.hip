@@ -42,7 +42,7 @@ struct bsk_plan {
   // of each shell; [0]: 16-pencil blocks, [1]: 8-pencil blocks), shells in
   // order, prefix[b] = first entry of shell b
   struct BlockList {
-    int2* d = nullptr;
+    int4* d = nullptr;
     std::vector<int> prefix;
   };
   BlockList xblk[2], yblk[2];
@@ -171,10 +171,10 @@ int host_shell_mz(int MZ, int hi) {
   return (int)std::min<long long>(r + 1, MZ);
 }
 
-int upload_blocks(bsk_plan::BlockList& bl, const std::vector<int2>& v) {
+int upload_blocks(bsk_plan::BlockList& bl, const std::vector<int4>& v) {
   if (v.empty()) return BSK_OK;
-  hipError_t e = hipMalloc(&bl.d, sizeof(int2) * v.size());
-  if (e == hipSuccess) e = hipMemcpy(bl.d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice);
+  hipError_t e = hipMalloc(&bl.d, sizeof(int4) * v.size());
+  if (e == hipSuccess) e = hipMemcpy(bl.d, v.data(), sizeof(int4) * v.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     bsk_set_error("plan block lists: %s", hipGetErrorString(e));
     return e == hipErrorOutOfMemory ? BSK_ENOMEM : BSK_EHIP;
@@ -183,24 +183,26 @@ int upload_blocks(bsk_plan::BlockList& bl, const std::vector<int2>& v) {
 }
 
 // The workgroups of the inverse x and y passes that hold modes: x blocks
-// (ry, kz0) with ny^2 + kz0^2 <= n2hi (the kernel's own skip test), y blocks
-// kz0 < mz_b.  Independent of the shell mesh.
+// (ry, kz0) with ny^2 + kz0^2 <= n2hi, y blocks kz0 < mz_b, each entry with
+// the shell's thresholds.  Independent of the shell mesh.
 int build_block_lists(bsk_plan* p) {
   const Geom& g = p->g;
   for (int k = 0; k < 2; ++k) {
     const int P = k ? 8 : 16;
-    std::vector<int2> xl, yl;
+    std::vector<int4> xl, yl;
     p->xblk[k].prefix.assign(p->nbin + 1, 0);
     p->yblk[k].prefix.assign(p->nbin + 1, 0);
     for (int b = 0; b < p->nbin; ++b) {
       p->xblk[k].prefix[b] = (int)xl.size();
       p->yblk[k].prefix[b] = (int)yl.size();
-      const int hi = p->n2hi[b];
-      for (int kz0 = 0; kz0 < host_shell_mz(g.MZ, hi); kz0 += P) yl.push_back(make_int2(b, kz0));
+      const int lo = p->n2lo[b], hi = p->n2hi[b];
+      for (int kz0 = 0; kz0 < host_shell_mz(g.MZ, hi); kz0 += P)
+        yl.push_back(make_int4(b, kz0, hi, 0));
       for (int ry = 0; ry < g.MY; ++ry) {
         const long long ny = g.nlo + ry;
         for (int kz0 = 0; kz0 < g.MZ; kz0 += P)
-          if (ny * ny + (long long)kz0 * kz0 <= hi) xl.push_back(make_int2(b, (ry << 16) | kz0));
+          if (ny * ny + (long long)kz0 * kz0 <= hi)
+            xl.push_back(make_int4(b, (ry << 16) | kz0, lo, hi));
       }
     }
     p->xblk[k].prefix[p->nbin] = (int)xl.size();
@@ -235,23 +237,23 @@ int shells_t(bsk_plan* p, int kind, const cplx<T>* dk, int shell0, int nsh, int 
   // the block entries of shells [shell0, shell0 + nsh) for this pencil count
   const int kb = bsk_pxy(sizeof(T) == 8, N) == 8 ? 1 : 0;
   const bsk_plan::BlockList &XB = p->xblk[kb], &YB = p->yblk[kb];
-  const int2* xb = XB.d + XB.prefix[shell0];
+  const int4* xb = XB.d + XB.prefix[shell0];
   const int nxb = XB.prefix[shell0 + nsh] - XB.prefix[shell0];
-  const int2* yb = YB.d + YB.prefix[shell0];
+  const int4* yb = YB.d + YB.prefix[shell0];
   const int nyb = YB.prefix[shell0 + nsh] - YB.prefix[shell0];
   if (stages & BSK_STAGE_X) {
-    rc = bsk_inv_x<T>(N, p->variant[3], dk, d1, tw, g, p->d_lo, p->d_hi, shell0, xb, nxb, kind,
-                      kf, (T)p->kpower, s);
+    rc = bsk_inv_x<T>(N, p->variant[3], dk, d1, tw, g, shell0, xb, nxb, kind, kf, (T)p->kpower,
+                      s);
     if (rc) return rc;
   }
   if (stages & BSK_STAGE_Y)  // y only: D2 stays in the workspace
-    return bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, yb, nyb, p->d_hi, shell0, s);
+    return bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, yb, nyb, shell0, s);
   if (!(stages & BSK_STAGE_YZ)) return BSK_OK;
   // shells [shell0, shell0+nsh) of the output arrays, cells of the slab
   const size_t esz = (fmt == BSK_FMT_F64) ? 8 : (fmt == BSK_FMT_F32 ? 4 : 2);
   char* o = reinterpret_cast<char*>(out) + (size_t)shell0 * stride * esz;
   char* ol = out_lo ? reinterpret_cast<char*>(out_lo) + (size_t)shell0 * stride * esz : nullptr;
-  rc = bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, yb, nyb, p->d_hi, shell0, s);
+  rc = bsk_inv_y<T>(N, p->variant[4], d1, d2, tw, g, x0, xs, yb, nyb, shell0, s);
   if (rc) return rc;
   return bsk_inv_z<T>(N, p->variant[5], d2, o, ol, tw, g.MZ, g.MZP, (long long)xs * N, cell0,
                       stride, fmt, nsh, p->d_hi, shell0, (stages & BSK_STAGE_LEVELS) ? 1 : 0,

@@ -131,56 +131,51 @@ template int bsk_fwd_x<double>(int, int, const cplx<double>*, cplx<double>*, con
 #if BSK_PASS == 3
 namespace {
 template <typename T, int N, int V>
-void inv_x_n(const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g, const int* lo,
-             const int* hi, int shell0, const int2* blocks, int nblocks, int kind, T kf, T kp,
-             hipStream_t s) {
+void inv_x_n(const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g, int shell0,
+             const int4* blocks, int nblocks, int kind, T kf, T kp, hipStream_t s) {
   constexpr int P = pxy<T, N>();
   hipLaunchKernelGGL((k_inv_x_mask<T, N, V, P, kW>), dim3(nblocks), dim3(kW), 0, s, dk, out, tw,
-                     g, lo, hi, shell0, blocks, kind, kf, kp);
+                     g, shell0, blocks, kind, kf, kp);
 }
 }  // namespace
 template <typename T>
 int bsk_inv_x(int N, int V, const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g,
-              const int* lo, const int* hi, int shell0, const int2* blocks, int nblocks, int kind,
-              T kf, T kp, hipStream_t s) {
+              int shell0, const int4* blocks, int nblocks, int kind, T kf, T kp, hipStream_t s) {
   if (nblocks <= 0) return BSK_OK;  // no shell holds a mode
-#define C(n) BSK_VCASE(n, (inv_x_n<T, n, 0>(dk, out, tw, g, lo, hi, shell0, blocks, nblocks, kind, kf, kp, s)), \
-                       (inv_x_n<T, n, 1>(dk, out, tw, g, lo, hi, shell0, blocks, nblocks, kind, kf, kp, s)))
+#define C(n) BSK_VCASE(n, (inv_x_n<T, n, 0>(dk, out, tw, g, shell0, blocks, nblocks, kind, kf, kp, s)), \
+                       (inv_x_n<T, n, 1>(dk, out, tw, g, shell0, blocks, nblocks, kind, kf, kp, s)))
   BSK_SWITCH(T, N, V, C)
 #undef C
 }
 template int bsk_inv_x<float>(int, int, const cplx<float>*, cplx<float>*, const cplx<float>*, Geom,
-                              const int*, const int*, int, const int2*, int, int, float, float,
-                              hipStream_t);
+                              int, const int4*, int, int, float, float, hipStream_t);
 template int bsk_inv_x<double>(int, int, const cplx<double>*, cplx<double>*, const cplx<double>*,
-                               Geom, const int*, const int*, int, const int2*, int, int, double,
-                               double, hipStream_t);
+                               Geom, int, const int4*, int, int, double, double, hipStream_t);
 #endif
 
 #if BSK_PASS == 4
 namespace {
 template <typename T, int N, int V>
 void inv_y_n(const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0, int xs,
-             const int2* blocks, int nblocks, const int* n2hi, int shell0, hipStream_t s) {
+             const int4* blocks, int nblocks, int shell0, hipStream_t s) {
   constexpr int P = pxy<T, N>();
   hipLaunchKernelGGL((k_inv_y<T, N, V, P, kW>), dim3(nblocks, xs), dim3(kW), 0, s, in, out, tw, g,
-                     x0, xs, blocks, n2hi, shell0);
+                     x0, xs, blocks, shell0);
 }
 }  // namespace
 template <typename T>
 int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0,
-              int xs, const int2* blocks, int nblocks, const int* n2hi, int shell0,
-              hipStream_t s) {
+              int xs, const int4* blocks, int nblocks, int shell0, hipStream_t s) {
   if (nblocks <= 0 || xs <= 0) return BSK_OK;  // no shell holds a mode
-#define C(n) BSK_VCASE(n, (inv_y_n<T, n, 0>(in, out, tw, g, x0, xs, blocks, nblocks, n2hi, shell0, s)), \
-                       (inv_y_n<T, n, 1>(in, out, tw, g, x0, xs, blocks, nblocks, n2hi, shell0, s)))
+#define C(n) BSK_VCASE(n, (inv_y_n<T, n, 0>(in, out, tw, g, x0, xs, blocks, nblocks, shell0, s)), \
+                       (inv_y_n<T, n, 1>(in, out, tw, g, x0, xs, blocks, nblocks, shell0, s)))
   BSK_SWITCH(T, N, V, C)
 #undef C
 }
 template int bsk_inv_y<float>(int, int, const cplx<float>*, cplx<float>*, const cplx<float>*, Geom,
-                              int, int, const int2*, int, const int*, int, hipStream_t);
+                              int, int, const int4*, int, int, hipStream_t);
 template int bsk_inv_y<double>(int, int, const cplx<double>*, cplx<double>*, const cplx<double>*,
-                               Geom, int, int, const int2*, int, const int*, int, hipStream_t);
+                               Geom, int, int, const int4*, int, int, hipStream_t);
 #endif
 
 #if BSK_PASS == 5

@@ -127,6 +127,29 @@ BSK_DEV void set_u32(V* base, unsigned e, V v) {
   *reinterpret_cast<V*>(reinterpret_cast<char*>(base) + e * (unsigned)sizeof(V)) = v;
 }
 
+// Twiddles in two steps: loads into registers (issued before a pass's data
+// loads, so both latencies overlap) and the LDS stores after them.
+template <typename T, int N, int W>
+struct TwiddleRegs {
+  cplx<T> v[(N + W - 1) / W];
+};
+template <typename T, int N, int W>
+BSK_DEV void fetch_twiddles(TwiddleRegs<T, N, W>& r, const cplx<T>* __restrict__ twg) {
+#pragma unroll
+  for (int i = 0; i < (N + W - 1) / W; ++i) {
+    const int t = (int)threadIdx.x + i * W;
+    r.v[i] = (N % W == 0 || t < N) ? twg[t] : cplx<T>{(T)0, (T)0};
+  }
+}
+template <typename T, int N, int W>
+BSK_DEV void put_twiddles(const TwiddleRegs<T, N, W>& r, cplx<T>* tw) {
+#pragma unroll
+  for (int i = 0; i < (N + W - 1) / W; ++i) {
+    const int t = (int)threadIdx.x + i * W;
+    if (N % W == 0 || t < N) tw[t] = r.v[i];
+  }
+}
+
 template <typename T, int N, int W>
 BSK_DEV void load_twiddles(cplx<T>* tw, const cplx<T>* __restrict__ twg) {
   stage_in<T, N, W>([&](int i) { return twg[i]; }, [&](int i, cplx<T> v) { tw[i] = v; });
@@ -256,21 +279,19 @@ template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk,
                                                   cplx<T>* __restrict__ out,
                                                   const cplx<T>* __restrict__ twg, Geom g,
-                                                  const int* __restrict__ n2lo,
-                                                  const int* __restrict__ n2hi, int shell0,
-                                                  const int2* __restrict__ blocks, int kind, T kf,
-                                                  T kp) {
+                                                  int shell0, const int4* __restrict__ blocks,
+                                                  int kind, T kf, T kp) {
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
-  const int2 ent = blocks[xcd_linear(blockIdx.x, gridDim.x)];
+  TwiddleRegs<T, N, W> twr;
+  fetch_twiddles<T, N, W>(twr, twg);
+  const int4 ent = blocks[xcd_linear(blockIdx.x, gridDim.x)];
   const long long ry = ent.y >> 16;
   const int kz0 = ent.y & 0xffff;
   const int bl = ent.x - shell0;
-  const int lo = n2lo[shell0 + bl], hi = n2hi[shell0 + bl];
+  const int lo = ent.z, hi = ent.w;  // the shell's n2lo, n2hi
   const int ny = g.nlo + (int)ry;
-  if (ny * ny + kz0 * kz0 > hi) return;  // uniform: every pencil is zero (not listed)
-  load_twiddles<T, N, W>(tw, twg);
   const cplx<T>* src = dk + ry * g.MX * g.MZ;
   stage_in<T, P * N, W>(
       [&](int e) {
@@ -297,6 +318,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
         const int xi = e / P, p = e - xi * P;
         buf[p * LD + xi] = v;
       });
+  put_twiddles<T, N, W>(twr, tw);
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
   cplx<T>* const dst = out + ((long long)bl * N * g.MY + ry) * g.MZP + kz0;
@@ -316,20 +338,20 @@ template <typename T, int N, int V, int P, int W>
 __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
                                              cplx<T>* __restrict__ out,
                                              const cplx<T>* __restrict__ twg, Geom g, int x0,
-                                             int xs, const int2* __restrict__ blocks,
-                                             const int* __restrict__ n2hi, int shell0) {
+                                             int xs, const int4* __restrict__ blocks,
+                                             int shell0) {
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
+  TwiddleRegs<T, N, W> twr;
+  fetch_twiddles<T, N, W>(twr, twg);
   const Blk bk = xcd_block();
   const int xl = bk.y;
   const long long x = x0 + xl;
-  const int2 ent = blocks[bk.x];
+  const int4 ent = blocks[bk.x];
   const int kz0 = ent.y;
   const long long bl = ent.x - shell0;
-  const int hi = n2hi[shell0 + bl];
-  if (kz0 >= shell_mz(g.MZ, hi)) return;  // uniform: zero block (not listed)
-  load_twiddles<T, N, W>(tw, twg);
+  const int hi = ent.z;  // the shell's n2hi
   const int np = g.MZ - kz0 < P ? g.MZ - kz0 : P;
   const cplx<T>* src = in + (bl * N + x) * g.MY * g.MZP + kz0;
   stage_in<T, P * N, W>(
@@ -344,6 +366,7 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
         const int yi = e / P, p = e - yi * P;
         buf[p * LD + yi] = v;
       });
+  put_twiddles<T, N, W>(twr, tw);
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
   cplx<T>* const dst = out + (bl * xs + xl) * N * g.MZP + kz0;

@@ -29,17 +29,15 @@ int bsk_fwd_x(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, 
 constexpr int bsk_pxy(bool f64, int n) { return (f64 || n > 640) ? 8 : 16; }
 
 // The inverse x and y passes launch only their workgroups that hold modes
-// of the shell: blocks lists (shell, ry, kz0) for x (ry << 16 | kz0 in .y)
-// and (shell, kz0) for y, P-pencil blocks, shells in order; the launch takes
-// the entries of its shells.
+// of the shell: block lists (shell, ry << 16 | kz0, n2lo, n2hi) for x and
+// (shell, kz0, n2hi, 0) for y, P-pencil blocks, shells in order; the launch
+// takes the entries of its shells.
 template <typename T>
 int bsk_inv_x(int N, int V, const cplx<T>* dk, cplx<T>* out, const cplx<T>* tw, Geom g,
-              const int* lo, const int* hi, int shell0, const int2* blocks, int nblocks, int kind,
-              T kf, T kp, hipStream_t s);
+              int shell0, const int4* blocks, int nblocks, int kind, T kf, T kp, hipStream_t s);
 template <typename T>
 int bsk_inv_y(int N, int V, const cplx<T>* in, cplx<T>* out, const cplx<T>* tw, Geom g, int x0,
-              int xs, const int2* blocks, int nblocks, const int* n2hi, int shell0,
-              hipStream_t s);
+              int xs, const int4* blocks, int nblocks, int shell0, hipStream_t s);
 template <typename T>
 int bsk_inv_z(int N, int V, const cplx<T>* in, void* out, void* out_lo, const cplx<T>* tw, int MZ,
               int MZP, long long nrows, long long cell0, long long stride, int fmt, int nsh,
