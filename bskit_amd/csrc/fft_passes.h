@@ -167,7 +167,8 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
   const int tid = threadIdx.x;
-  load_twiddles<T, N, W>(tw, twg);
+  TwiddleRegs<T, N, W> twr;
+  fetch_twiddles<T, N, W>(twr, twg);
   const long long row0 = (long long)blockIdx.x * P;
   const cplx<T>* src = reinterpret_cast<const cplx<T>*>(in) + row0 * NH;
   const int nr = (int)(nrows - row0 < P ? nrows - row0 : P);
@@ -179,6 +180,7 @@ __global__ __launch_bounds__(W) void k_fwd_z(const T* __restrict__ in,
         const int p = e / NH;
         buf[p * LD + (e - p * NH)] = v;
       });
+  put_twiddles<T, N, W>(twr, tw);
   __syncthreads();
   fft_lds<T, NH, false, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
   cplx<T>* const dst = out + row0 * MZP;
@@ -202,7 +204,8 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
-  load_twiddles<T, N, W>(tw, twg);
+  TwiddleRegs<T, N, W> twr;
+  fetch_twiddles<T, N, W>(twr, twg);
   const Blk bk = xcd_block();
   const long long x = bk.y;
   const int kz0 = bk.x * P;
@@ -217,6 +220,7 @@ __global__ __launch_bounds__(W) void k_fwd_y(const cplx<T>* __restrict__ in,
         const int y = e / P, p = e - y * P;
         buf[p * LD + y] = v;
       });
+  put_twiddles<T, N, W>(twr, tw);
   __syncthreads();
   fft_lds<T, N, false, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
   cplx<T>* const dst = out + x * g.MY * g.MZP + kz0;
@@ -239,7 +243,8 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
   constexpr int LD = pencil_ld<N>();
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
-  load_twiddles<T, N, W>(tw, twg);
+  TwiddleRegs<T, N, W> twr;
+  fetch_twiddles<T, N, W>(twr, twg);
   const Blk bk = xcd_block();
   const long long ry = bk.y;
   const int kz0 = bk.x * P;
@@ -255,6 +260,7 @@ __global__ __launch_bounds__(W) void k_fwd_x(const cplx<T>* __restrict__ in,
         const int x = e / P, p = e - x * P;
         buf[p * LD + x] = v;
       });
+  put_twiddles<T, N, W>(twr, tw);
   __syncthreads();
   fft_lds<T, N, false, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
   const int ny = g.nlo + (int)ry;
