@@ -257,17 +257,26 @@ def profile_counters(name):
 
 def profiled_traffic(kernel, nmesh, shell_mesh):
     """HBM bytes per launch of kernel from the committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration), or
-    None when no summary covers this workload."""
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration): the
+    row named by the bare kernel name (the mean over a grid's launches of
+    all its instances) if there is one, else its only instance; None when
+    no summary covers this workload."""
     path = os.path.join(ROOT, 'profiles', 'hbm_traffic_%d_s%d.txt' % (nmesh, shell_mesh))
     if not os.path.exists(path):
         return None
+    exact, inst = None, []
     with open(path) as f:
         for line in f:
             w = line.split()
-            if w and not line.startswith('#') and w[0].split('<')[0] == kernel:
-                return float(w[-1])
-    return None
+            if not w or line.startswith('#'):
+                continue
+            if w[0] == kernel:
+                exact = float(w[-1])
+            elif w[0].split('<')[0] == kernel:
+                inst.append(float(w[-1]))
+    if exact is not None:
+        return exact
+    return inst[0] if len(inst) == 1 else None
 
 
 def run_serving(args, torch, np):
