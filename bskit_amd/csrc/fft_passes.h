@@ -477,6 +477,24 @@ BSK_DEV void store_pair(int fmt, char* hi, char* lo, unsigned e, T a, T b,
   }
 }
 
+// Four reals a, b, c, d at element offsets e .. e + 3 (e a multiple of 4).
+template <typename T>
+BSK_DEV void store_four(int fmt, char* hi, char* lo, unsigned e, T a, T b, T c, T d,
+                        const SplitConsts& k) {
+  if constexpr (sizeof(T) == 8) {
+    put_u32(hi, e * 8u, make_double2(a, b));
+    put_u32(hi, e * 8u + 16u, make_double2(c, d));
+  } else if (fmt == BSK_FMT_BF16X2) {
+    uint32_t h0, l0, h1, l1;
+    bf16_split2(a, b, h0, l0, k);
+    bf16_split2(c, d, h1, l1, k);
+    put_u32(hi, e * 2u, make_uint2(h0, h1));
+    put_u32(lo, e * 2u, make_uint2(l0, l1));
+  } else {
+    put_u32(hi, e * 4u, make_float4(a, b, c, d));
+  }
+}
+
 template <typename T>
 BSK_DEV void store_zero(int fmt, char* hi, char* lo, unsigned e) {
   if constexpr (sizeof(T) == 8) {
@@ -517,7 +535,8 @@ struct ShellLevels {
 };
 
 // Lanes map to (row, k) with a power-of-two lane count per row (KQ for the
-// KH mirrored pairs, QQ for the NH/2 output quads), so no index divides.
+// KH mirrored pairs, QQ for the NH/4 output groups of four complex values), so
+// no index divides.
 //
 // lv != 0 (whole mesh: nrows = N^2, cell0 = 0): each shell is written in the
 // sublattice level layout instead of C order (ShellLevels: a permutation of
@@ -537,9 +556,10 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   // taking KS pairs of its row (KS = 1 unless NH/2 + 1 > W)
   constexpr int KQ = pow2_at_least(KH) < W ? pow2_at_least(KH) : W;
   constexpr int KS = (KH + KQ - 1) / KQ, RK = W / KQ, NPK = P / RK;
-  constexpr int QQ = pow2_at_least(NH / 2), RQ = W / QQ, NPQ = P / RQ;
+  constexpr int QQ = pow2_at_least(NH / 4) > W / P ? pow2_at_least(NH / 4) : W / P;
+  constexpr int RQ = W / QQ, NPQ = P / RQ;
   using Lay = ShellLevels<NH>;
-  static_assert(NH % 2 == 0, "z c2r needs N divisible by 4");
+  static_assert(NH % 4 == 0, "z c2r stores need N divisible by 8");
   static_assert(P % RK == 0 && P % RQ == 0, "row mapping of the z c2r");
   __shared__ cplx<T> buf[P * LD];
   __shared__ cplx<T> tw[N];
@@ -619,28 +639,32 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
         typename Plan<NH, V>::seq{});
   else
     fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
-  // two complex values (four reals) per lane and store
+  // four complex values (eight reals, z = 8q .. 8q + 7) per lane and store:
+  // in the level layout the four even z and the four odd z are each four
+  // consecutive cells of one part
   const SplitConsts sc = split_consts();
   const long long obase = (bl * shell_stride + cell0 + row0 * N) * esz;
   char* const ohi = static_cast<char*>(out) + obase;
   char* const olo = out_lo ? static_cast<char*>(out_lo) + obase : nullptr;
   const int q = tid & (QQ - 1);
   const int rq = QQ >= 64 ? __builtin_amdgcn_readfirstlane(tid / QQ) : tid / QQ;  // see r0
-  if (q < NH / 2) {
+  if (q < NH / 4) {
 #pragma unroll
     for (int i = 0; i < NPQ; ++i) {
       const int p = rq + i * RQ;
       if (p >= np) continue;
-      const cplx<T>* b = buf + p * LD + 2 * q;
+      const cplx<T>* b = buf + p * LD + 4 * q;
+      const cplx<T> v0 = b[0], v1 = b[1], v2 = b[2], v3 = b[3];
       if (!lv) {
-        store_quad<T>(fmt, ohi, olo, (unsigned)(p * N + 4 * q), b[0], b[1], sc);
+        store_quad<T>(fmt, ohi, olo, (unsigned)(p * N + 8 * q), v0, v1, sc);
+        store_quad<T>(fmt, ohi, olo, (unsigned)(p * N + 8 * q + 4), v2, v3, sc);
         continue;
       }
-      // z = 4q, 4q + 2 (b[0].x, b[1].x) and z = 4q + 1, 4q + 3 (b[0].y, b[1].y):
-      // the same z/2 = 2q, 2q + 1 in the part of each z parity
+      // z = 8q + 2t (vt.x) and 8q + 2t + 1 (vt.y), t < 4: z/2 = 4q + t in the
+      // part of each z parity
       const int row = (int)row0 + prow(p), x = row / N, y = row - x * N;
       const int xy = (x & 1) * 2 + (y & 1);  // 0: both even, 3: both odd
-      const unsigned pos = (unsigned)(((x >> 1) * NH + (y >> 1)) * NH + 2 * q);
+      const unsigned pos = (unsigned)(((x >> 1) * NH + (y >> 1)) * NH + 4 * q);
 #pragma unroll
       for (int zp = 0; zp < 2; ++zp) {
         int part;
@@ -654,9 +678,12 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
           off = part == 2 ? (unsigned)Lay::OFF[2]
                           : (unsigned)(Lay::OFF[3] + (long long)xy * Lay::E);
         }
-        if (pm >> part & 1)
-          store_pair<T>(fmt, shi, slo, off + pos, zp ? b[0].y : b[0].x, zp ? b[1].y : b[1].x,
-                        sc);
+        if (pm >> part & 1) {
+          if (zp)
+            store_four<T>(fmt, shi, slo, off + pos, v0.y, v1.y, v2.y, v3.y, sc);
+          else
+            store_four<T>(fmt, shi, slo, off + pos, v0.x, v1.x, v2.x, v3.x, sc);
+        }
       }
     }
   }
