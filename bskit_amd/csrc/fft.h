@@ -170,6 +170,24 @@ BSK_DEV void fft_lds(cplx<T>* buf, const cplx<T>* tw, Radices<Rs...>) {
   stockham_run<T, L, INV, P, W, LD, TWN, 1, Rs...>(buf, tw);
 }
 
+// Entries of the twiddle table a radix sequence reads: the stage of radix R
+// after NS points reads tw[k * (TWN / (NS * R)) * r] for k < NS, r < R, so a
+// table of this many entries (often well short of TWN) serves the transform.
+template <int TWN, int NS, int R, int... Rest>
+constexpr int tw_span_run() {
+  constexpr int m = NS > 1 ? (NS - 1) * (TWN / (NS * R)) * (R - 1) + 1 : 1;
+  if constexpr (sizeof...(Rest) > 0) {
+    constexpr int n = tw_span_run<TWN, NS * R, Rest...>();
+    return m > n ? m : n;
+  } else {
+    return m;
+  }
+}
+template <int TWN, int... Rs>
+constexpr int tw_span(Radices<Rs...>) {
+  return tw_span_run<TWN, 1, Rs...>();
+}
+
 // ------------------------------------------- runtime pencil count (np <= PMAX)
 // Quotient by a divisor n known only at run time: multiply-high with
 // m = ceil(2^32 / n), exact for dividends below 2^32 / n; n = 1 (whose m

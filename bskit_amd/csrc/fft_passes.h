@@ -288,10 +288,13 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
                                                   int shell0, const int4* __restrict__ blocks,
                                                   int kind, T kf, T kp) {
   constexpr int LD = pencil_ld<N>();
+  // the entries of the length-N table the plan reads (157 of 200 at N = 200,
+  // which keeps the FP32 workgroup under 1/6 of the CU's LDS)
+  constexpr int TWS = tw_span<N>(typename Plan<N, V>::seq{});
   __shared__ cplx<T> buf[P * LD];
-  __shared__ cplx<T> tw[N];
-  TwiddleRegs<T, N, W> twr;
-  fetch_twiddles<T, N, W>(twr, twg);
+  __shared__ cplx<T> tw[TWS];
+  TwiddleRegs<T, TWS, W> twr;
+  fetch_twiddles<T, TWS, W>(twr, twg);
   const int4 ent = blocks[xcd_linear(blockIdx.x, gridDim.x)];
   const long long ry = ent.y >> 16;
   const int kz0 = ent.y & 0xffff;
@@ -324,7 +327,7 @@ __global__ __launch_bounds__(W) void k_inv_x_mask(const cplx<T>* __restrict__ dk
         const int xi = e / P, p = e - xi * P;
         buf[p * LD + xi] = v;
       });
-  put_twiddles<T, N, W>(twr, tw);
+  put_twiddles<T, TWS, W>(twr, tw);
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
   cplx<T>* const dst = out + ((long long)bl * N * g.MY + ry) * g.MZP + kz0;
@@ -347,10 +350,13 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
                                              int xs, const int4* __restrict__ blocks,
                                              int shell0) {
   constexpr int LD = pencil_ld<N>();
+  // the entries of the length-N table the plan reads (157 of 200 at N = 200,
+  // which keeps the FP32 workgroup under 1/6 of the CU's LDS)
+  constexpr int TWS = tw_span<N>(typename Plan<N, V>::seq{});
   __shared__ cplx<T> buf[P * LD];
-  __shared__ cplx<T> tw[N];
-  TwiddleRegs<T, N, W> twr;
-  fetch_twiddles<T, N, W>(twr, twg);
+  __shared__ cplx<T> tw[TWS];
+  TwiddleRegs<T, TWS, W> twr;
+  fetch_twiddles<T, TWS, W>(twr, twg);
   const Blk bk = xcd_block();
   const int xl = bk.y;
   const long long x = x0 + xl;
@@ -372,7 +378,7 @@ __global__ __launch_bounds__(W) void k_inv_y(const cplx<T>* __restrict__ in,
         const int yi = e / P, p = e - yi * P;
         buf[p * LD + yi] = v;
       });
-  put_twiddles<T, N, W>(twr, tw);
+  put_twiddles<T, TWS, W>(twr, tw);
   __syncthreads();
   fft_lds<T, N, true, P, W, LD, N>(buf, tw, typename Plan<N, V>::seq{});
   cplx<T>* const dst = out + (bl * xs + xl) * N * g.MZP + kz0;
@@ -562,9 +568,15 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   static_assert(NH % 4 == 0, "z c2r stores need N divisible by 8");
   static_assert(P % RK == 0 && P % RQ == 0, "row mapping of the z c2r");
   __shared__ cplx<T> buf[P * LD];
-  __shared__ cplx<T> tw[N];
+  // the length-NH transform's table, tw[t] = twg[2t] = exp(-2 pi i t / NH),
+  // only the entries its plan reads (77 of the 200 of twg at N = 200, which
+  // keeps the FP32 workgroup under 1/6 of the CU's LDS: six workgroups per CU
+  // instead of five); the mirrored-pair twiddles exp(+2 pi i k / N) come
+  // from twg itself
+  constexpr int TWS = tw_span<NH>(typename Plan<NH, V>::seq{});
+  __shared__ cplx<T> tw[TWS];
   const int tid = threadIdx.x;
-  load_twiddles<T, N, W>(tw, twg);
+  stage_in<T, TWS, W>([&](int i) { return twg[2 * i]; }, [&](int i, cplx<T> v) { tw[i] = v; });
   __syncthreads();
   const long long row0 = (long long)blockIdx.x * P;
   const long long bl = blockIdx.y;
@@ -614,7 +626,7 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   for (int ks = 0; ks < KS; ++ks) {
     const int k = (tid & (KQ - 1)) + ks * KQ, j = NH - k;
     const bool kact = k < KH;
-    const cplx<T> wk = kact ? cconj(tw[k]) : cplx<T>{(T)0, (T)0};  // exp(+2 pi i k / N)
+    const cplx<T> wk = kact ? cconj(twg[k]) : cplx<T>{(T)0, (T)0};  // exp(+2 pi i k / N)
 #pragma unroll
     for (int q0 = 0; q0 < NPK; q0 += B) {
       cplx<T> xk[B], xj[B];
@@ -634,11 +646,11 @@ __global__ __launch_bounds__(W) void k_inv_z(const cplx<T>* __restrict__ in, voi
   }
   __syncthreads();
   if (gather)
-    fft_lds_n<T, NH, true, P, W, LD, N>(
+    fft_lds_n<T, NH, true, P, W, LD, NH>(
         buf, tw, np, div_magic(np), tid,
         typename Plan<NH, V>::seq{});
   else
-    fft_lds<T, NH, true, P, W, LD, N>(buf, tw, typename Plan<NH, V>::seq{});
+    fft_lds<T, NH, true, P, W, LD, NH>(buf, tw, typename Plan<NH, V>::seq{});
   // four complex values (eight reals, z = 8q .. 8q + 7) per lane and store:
   // in the level layout the four even z and the four odd z are each four
   // consecutive cells of one part
