@@ -21,7 +21,7 @@ KIND_DELTA, KIND_UNIT, KIND_KMAG, KIND_KPOW = 0, 1, 2, 3
 MODE_AUTO, MODE_AAB, MODE_ABC = 0, 1, 2
 STAGE_X, STAGE_YZ, STAGE_Y, STAGE_LEVELS = 1, 2, 4, 8
 LEVELS = 4
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 _c_int, _c_ll, _c_double, _vp = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
 _c_size_p = ctypes.POINTER(ctypes.c_size_t)

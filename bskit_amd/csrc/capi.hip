@@ -771,6 +771,7 @@ int bsk_triples_tiled(int dtype, const void* fields, long long shell_stride, int
 // include each part (bit L): every level includes E.
 constexpr int kParts = 4;
 constexpr unsigned kPartLevels[kParts] = {0x3, 0xf, 0x5, 0x1};
+static_assert(kParts == kPairSlots, "one partial slot per triangle and segment");
 
 struct bsk_pairs {
   int mode = 0;
@@ -782,9 +783,10 @@ struct bsk_pairs {
   int ni = 1;          // largest items per wave of a launch (compiled count)
   int npass = 1;       // launches in all
   // launch segments, in launch order: runs of consecutive parts with the same
-  // levels (the first holds E and every item, its launches store the
-  // partials, the rest add to them); per segment the parts (bit p), the
-  // staged shell window (first, count) and the launches
+  // levels (the first holds E, whose cells every level includes), each with
+  // its own pair tiles and items over the triangles of its levels; per
+  // segment the parts (bit p), the staged shell window (first, count) and the
+  // launches
   int nseg = 1;
   int seg_parts[kParts] = {0xf, 0, 0, 0};
   int seg_win[kParts][2] = {{0, 1}, {0, 1}, {0, 1}, {0, 1}};
@@ -792,12 +794,12 @@ struct bsk_pairs {
   int seg_ni[kParts] = {1, 1, 1, 1};  // items per wave of the segment's launches
   int seg_rep[kParts] = {1, 1, 1, 1};  // replicas of its wave lists (each runs 1/rep of the K steps)
   int item_total = 0;  // distinct items: partial positions per workgroup
-  std::vector<int> slot;
+  std::vector<int> slot;  // [ntri][kParts]: each triangle's slot per segment, -1 for none
   std::vector<int> h_items, h_outpos, h_pairs;
   int* d_items = nullptr;   // per launch [waves * ni of its segment]
   int* d_outpos = nullptr;  // the same shape
   int* d_pairs = nullptr;   // [ntile + 1][16]
-  int* d_slot = nullptr;    // [ntri]
+  int* d_slot = nullptr;    // [ntri][kParts]
 };
 
 namespace {
@@ -1091,31 +1093,9 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
   const int W = kPairWaves, cap = std::min(4, bsk_pair_item_cap(sets, nshell));
   std::vector<int> idx[BSK_LEVELS];
   for (int t = 0; t < ntri; ++t) idx[level ? level[t] : 0].push_back(t);
-  // per level: tiles and units in absolute shell indices
-  LevelPlan lp[BSK_LEVELS];
-  int tile0[BSK_LEVELS + 1] = {0};
   unsigned present = 0;
-  for (int L = 0; L < BSK_LEVELS; ++L) {
-    lp[L] = plan_level(mode, nkb, cap, idx[L], tri);
-    tile0[L + 1] = tile0[L] + (int)lp[L].tiles.tile_mask.size();
-    if (!idx[L].empty()) present |= 1u << L;
-  }
-  const int ntile = tile0[BSK_LEVELS];
-  if (ntile >= 0xffff) {
-    bsk_set_error("%s: %d pair tiles exceed 65534; split the triangle list", fn, ntile);
-    return BSK_EUNSUPPORTED;
-  }
-  // partial positions: one per (level tile, kb) item
-  std::vector<int> item_of((size_t)ntile * nkb, -1);
-  int nitem = 0;
   for (int L = 0; L < BSK_LEVELS; ++L)
-    for (int ti = 0; ti < (int)lp[L].tiles.tile_mask.size(); ++ti)
-      for (int kb = 0; kb < nkb; ++kb)
-        if (lp[L].tiles.tile_mask[ti] >> kb & 1u) item_of[(size_t)(tile0[L] + ti) * nkb + kb] = nitem++;
-  if ((long long)std::max(nitem, 1) * 256 >= (1LL << 28)) {  // slot field below the level bits
-    bsk_set_error("%s: %d items exceed the slot range; split the triangle list", fn, nitem);
-    return BSK_EUNSUPPORTED;
-  }
+    if (!idx[L].empty()) present |= 1u << L;
   // segments: runs of consecutive parts with the same present levels, the
   // one holding E (every level) first
   struct Seg {
@@ -1133,30 +1113,56 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
   }
   if (segs.empty()) segs.push_back({0xf, 1u});  // empty list: one pass of padding
   std::stable_partition(segs.begin(), segs.end(), [](const Seg& g) { return (g.parts >> 1) & 1; });
-  // per segment: the units of its levels binned into passes
+  // per segment one plan over the triangles of all its levels: a pair (a, b)
+  // that triangles of several levels need is formed once on the segment's
+  // cells (on E, whose cells every level includes, the levels share most
+  // pairs: at 512^3 / 32 bins 15 tiles instead of 27 planned level by level)
+  const int nseg = (int)segs.size();
+  std::vector<LevelPlan> lp(nseg);
+  std::vector<int> tile0(nseg + 1, 0);
+  for (int g = 0; g < nseg; ++g) {
+    std::vector<int> sel;
+    for (int L = 0; L < BSK_LEVELS; ++L)
+      if (segs[g].lv >> L & 1) sel.insert(sel.end(), idx[L].begin(), idx[L].end());
+    std::sort(sel.begin(), sel.end());
+    lp[g] = plan_level(mode, nkb, cap, sel, tri);
+    tile0[g + 1] = tile0[g] + (int)lp[g].tiles.tile_mask.size();
+  }
+  const int ntile = tile0[nseg];
+  if (ntile >= 0xffff) {
+    bsk_set_error("%s: %d pair tiles exceed 65534; split the triangle list", fn, ntile);
+    return BSK_EUNSUPPORTED;
+  }
+  // partial positions: one per (segment tile, kb) item, each stored by its
+  // segment's launches alone
+  std::vector<int> item_of((size_t)ntile * nkb, -1);
+  int nitem = 0;
+  for (int g = 0; g < nseg; ++g)
+    for (int ti = 0; ti < (int)lp[g].tiles.tile_mask.size(); ++ti)
+      for (int kb = 0; kb < nkb; ++kb)
+        if (lp[g].tiles.tile_mask[ti] >> kb & 1u) item_of[(size_t)(tile0[g] + ti) * nkb + kb] = nitem++;
+  if ((long long)std::max(nitem, 1) * 256 >= (1LL << 28)) {  // slot field below the level bits
+    bsk_set_error("%s: %d items exceed the slot range; split the triangle list", fn, nitem);
+    return BSK_EUNSUPPORTED;
+  }
+  // per segment: its units binned into passes
   struct SegPlan {
     int lo, nsh, npass, ni, rep = 1;
     std::vector<int> unit_tile;  // global tile
     std::vector<unsigned> unit_mask;
     std::vector<std::vector<int>> bins;
   };
-  std::vector<SegPlan> sp(segs.size());
+  std::vector<SegPlan> sp(nseg);
   int maxni = 1;
   size_t nentry = 0;
-  for (size_t g = 0; g < segs.size(); ++g) {
-    int lo = INT_MAX, hi = 0;
-    for (int L = 0; L < BSK_LEVELS; ++L) {
-      if (!(segs[g].lv >> L & 1) || idx[L].empty()) continue;
-      lo = std::min(lo, lp[L].lo);
-      hi = std::max(hi, lp[L].hi);
-      for (size_t u = 0; u < lp[L].unit_tile.size(); ++u) {
-        sp[g].unit_tile.push_back(tile0[L] + lp[L].unit_tile[u]);
-        sp[g].unit_mask.push_back(lp[L].unit_mask[u]);
-      }
+  for (int g = 0; g < nseg; ++g) {
+    for (size_t u = 0; u < lp[g].unit_tile.size(); ++u) {
+      sp[g].unit_tile.push_back(tile0[g] + lp[g].unit_tile[u]);
+      sp[g].unit_mask.push_back(lp[g].unit_mask[u]);
     }
-    if (lo == INT_MAX) lo = 0, hi = 1;
-    sp[g].lo = lo;
-    sp[g].nsh = hi - lo;
+    const bool any = !lp[g].tri.empty();
+    sp[g].lo = any ? lp[g].lo : 0;
+    sp[g].nsh = any ? lp[g].hi - lp[g].lo : 1;
     int ml = 1;
     if (sp[g].unit_mask.empty()) {
       sp[g].npass = 1;
@@ -1216,10 +1222,12 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
   // kernel's zero row); the last tile is all padding
   std::vector<int> prs((size_t)(ntile + 1) * 16, 0xffff | (0xffff << 16));
   bsk_pairs* q = new bsk_pairs();
-  q->slot.resize(ntri);
+  // triangle slots, one per segment (-1: the triangle's level does not
+  // include the segment's parts; the first segment holds every triangle)
+  q->slot.assign((size_t)kParts * ntri, -1);
   int npair = 0;
-  for (int L = 0; L < BSK_LEVELS; ++L) {
-    const PairTiles& tl = lp[L].tiles;
+  for (int g = 0; g < nseg; ++g) {
+    const PairTiles& tl = lp[g].tiles;
     const int nt = (int)tl.tile_mask.size();
     std::vector<std::pair<int, int>> where(tl.pairs.size());  // (global tile, row)
     for (int ti = 0; ti < nt; ++ti) {
@@ -1227,26 +1235,28 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
       const int p1 = ti + 1 < nt ? tl.tile_of_pair_start[ti + 1] : (int)tl.pairs.size();
       for (int p = p0; p < p1; ++p) {
         const long long k = tl.pairs[p].first;
-        prs[(size_t)(tile0[L] + ti) * 16 + (p - p0)] =
+        prs[(size_t)(tile0[g] + ti) * 16 + (p - p0)] =
             (int)(k >> 32) | ((int)(k & 0xffffffff) << 16);
-        where[p] = {tile0[L] + ti, p - p0};
+        where[p] = {tile0[g] + ti, p - p0};
       }
     }
-    // triangle slots: item * 256 + row * 16 + column, level in bits 28-29
+    // slot: item * 256 + row * 16 + column, the triangle's level in bits 28-29
     std::vector<long long> keys(tl.pairs.size());
     for (size_t p = 0; p < keys.size(); ++p) keys[p] = tl.pairs[p].first;
     std::vector<size_t> kidx(keys.size());
     for (size_t p = 0; p < kidx.size(); ++p) kidx[p] = p;
     std::sort(kidx.begin(), kidx.end(), [&](size_t x, size_t y) { return keys[x] < keys[y]; });
-    for (size_t t = 0; t < lp[L].tri.size(); ++t) {
-      const Role& r = lp[L].roles[t];
+    for (size_t t = 0; t < lp[g].tri.size(); ++t) {
+      const Role& r = lp[g].roles[t];
       const long long k = pkey(r.a, r.b);
       auto it = std::lower_bound(kidx.begin(), kidx.end(), k,
                                  [&](size_t x, long long v) { return keys[x] < v; });
       const int p = (int)*it;
       const int ti = where[p].first, row = where[p].second, c = r.c;
-      const int g = item_of[(size_t)ti * nkb + c / 16];
-      q->slot[lp[L].tri[t]] = g * 256 + row * 16 + (c % 16) + (L << 28);
+      const int it_ = item_of[(size_t)ti * nkb + c / 16];
+      const int gt = lp[g].tri[t];
+      const int L = level ? level[gt] : 0;
+      q->slot[(size_t)kParts * gt + g] = it_ * 256 + row * 16 + (c % 16) + (L << 28);
     }
     npair += (int)tl.pairs.size();
   }
@@ -1374,7 +1384,7 @@ int bsk_pairs_slots(const bsk_pairs* q, int* slot_host) {
     bsk_set_error("bsk_pairs_slots: invalid argument");
     return BSK_EINVAL;
   }
-  for (int t = 0; t < q->ntri; ++t) slot_host[t] = q->slot[t];
+  std::copy(q->slot.begin(), q->slot.end(), slot_host);
   return BSK_OK;
 }
 
@@ -1474,8 +1484,8 @@ int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
     a.lo_row = w0;
     a.koff = w0 % 16;
     a.nkb = (a.koff + wn + 15) / 16;
-    a.ncell = c1 - c0;
-    a.accumulate = g > 0;
+    a.ncell = c1 - c0;  // 0 (an empty part) still stores the segment's zero partials
+    a.accumulate = 0;   // every item belongs to one segment
     a.rep = q->seg_rep[g];
     const int ni = q->seg_ni[g];
     const int np = q->seg_pass[g];
@@ -1487,7 +1497,6 @@ int triples_mfma(const bsk_pairs* q, const uint16_t* hi_i, const uint16_t* lo_i,
     for (int k = 0; k < np; k += a.npass) {
       a.items = q->d_items + e0 + (size_t)k * kPairWaves * ni;
       a.outpos = q->d_outpos + e0 + (size_t)k * kPairWaves * ni;
-      if (a.ncell == 0) continue;
       int rc = bsk_triple_pairs_launch(a, q->sets, ni, s);
       if (rc) return rc;
     }

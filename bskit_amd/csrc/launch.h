@@ -67,6 +67,7 @@ int bsk_reduce_tiled_launch(const double* partial, int ntri, int nsum, int nchun
 constexpr int kPairWaves = 8;    // waves per workgroup of k_triple_pairs
 constexpr int kPairChunk = 128;  // cells per staged chunk
 constexpr int kPairMaxItems = 8; // items per wave and launch
+constexpr int kPairSlots = 4;    // partial slots per triangle: one per launch segment
 struct PairArgs {
   const uint16_t* hi[3];  // role sets: 0 = first pair side, 1 = second, 2 = third side
   const uint16_t* lo[3];

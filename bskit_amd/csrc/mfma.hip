@@ -237,8 +237,9 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
 
 // Reduction of the per-workgroup FP32 partials in two coalesced steps, in
 // a fixed order (deterministic): k_reduce_slots sums, per partial slot, the
-// workgroups of each of kRedSplit ranges in FP64; k_reduce_tri adds the
-// ranges of each triangle's slot and applies the scale and the level factor.
+// workgroups of each of kRedSplit ranges in FP64; k_reduce_tri adds, per
+// triangle, the ranges of its slots (one per launch segment of its level,
+// kPairSlots entries, -1 for none) and applies the scale and the level factor.
 constexpr int kRedSplit = 16;
 
 __global__ __launch_bounds__(256) void k_reduce_slots(const float* __restrict__ partial, int nwg,
@@ -256,18 +257,26 @@ __global__ __launch_bounds__(256) void k_reduce_slots(const float* __restrict__ 
   sums[(long long)c * nslot + e] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
+static_assert(kPairSlots == 4, "k_reduce_tri reads a triangle's slots as one int4");
+
 __global__ __launch_bounds__(256) void k_reduce_tri(const double* __restrict__ sums, int nslot,
                                                     int ntri, const int* __restrict__ slot,
                                                     double scale, double* __restrict__ out) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= ntri) return;
-  const int v = slot[t];
-  const int e = v & 0x0fffffff;
-  // bits 28-29: sublattice level L, whose cells are 2^-L of the mesh
-  const double f = (double)(1 << ((v >> 28) & 3));
+  const int4 v = *reinterpret_cast<const int4*>(slot + (long long)kPairSlots * t);
+  const int vs[kPairSlots] = {v.x, v.y, v.z, v.w};
+  // bits 28-29: sublattice level L, whose cells are 2^-L of the mesh (the
+  // first slot is always present)
+  const double f = (double)(1 << ((v.x >> 28) & 3));
   double s = 0.0;
 #pragma unroll
-  for (int c = 0; c < kRedSplit; ++c) s += sums[(long long)c * nslot + e];
+  for (int g = 0; g < kPairSlots; ++g) {
+    if (vs[g] < 0) continue;
+    const int e = vs[g] & 0x0fffffff;
+#pragma unroll
+    for (int c = 0; c < kRedSplit; ++c) s += sums[(long long)c * nslot + e];
+  }
   out[t] = s * scale * f;
 }
 

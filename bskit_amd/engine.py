@@ -217,7 +217,9 @@ class PairPlan:
         return out
 
     def slots(self):
-        out = np.zeros(self.ntri, dtype=np.int32)
+        """(T, 4) partial slots of each triangle, one per launch segment
+        (-1: none); the triangle's sum is the sum of its slots x 2^level."""
+        out = np.zeros((self.ntri, 4), dtype=np.int32)
         L.call('bsk_pairs_slots', self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
         return out
 

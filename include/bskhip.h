@@ -34,7 +34,7 @@
 extern "C" {
 #endif
 
-#define BSK_ABI_VERSION 6
+#define BSK_ABI_VERSION 7
 
 /* status codes */
 #define BSK_OK 0
@@ -249,8 +249,10 @@ int bsk_pairs_info(const bsk_pairs* pairs, int* npair, int* ntile, int* nitem,
  * level[t] (bsk_plan_triangle_levels) and scaled by 2^level[t]; run it with
  * bsk_triples_mfma_levels.  The launches run over segments of the level
  * layout: runs of consecutive parts (P0..P3) holding the same levels of the
- * plan, each with the items of those levels, the segment holding E (every
- * item) first; later segments add to its partials.  bsk_pairs_segments:
+ * plan, the segment holding E (every level) first; each segment has its own
+ * pair tiles and items, planned over the triangles of all its levels at
+ * once (a pair that triangles of several levels need is formed once), and
+ * stores its own partials.  bsk_pairs_segments:
  * *nseg segments in launch order, per segment the parts (bit p), the shell
  * window its launches stage (shells [first, first + count)), its launches,
  * their items per wave and the replicas of their wave lists (a segment with
@@ -265,8 +267,11 @@ int bsk_pairs_segments(const bsk_pairs* pairs, int* nseg, int* parts, int* first
 /* The layout parts (bit p) in which some launch of the plan stages shell
  * b: parts[b] for the nshell shells of the plan. */
 int bsk_pairs_shell_parts(const bsk_pairs* pairs, unsigned char* parts);
-/* Partial-buffer slot of each triangle (item * 256 + row * 16 + column,
- * plus level << 28). */
+/* Partial-buffer slots of each triangle, one per launch segment
+ * (slot_host[4 t + g] for segment g: item * 256 + row * 16 + column, plus
+ * level << 28; -1 where the triangle's level does not include the segment's
+ * parts; the first segment holds every triangle): 4 ntri ints.  The sum of
+ * triangle t is the sum of its slots times 2^level. */
 int bsk_pairs_slots(const bsk_pairs* pairs, int* slot_host);
 /* Host copies of the launch tables, for inspection: items and outpos
  * (per launch 8 waves x its segment's items per wave: descriptor tile | kb << 16 | padding flag

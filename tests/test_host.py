@@ -168,7 +168,8 @@ def test_library_loads_and_host_calls_work():
     from bskit_amd.engine import PairPlan
     plan = PairPlan(np.array([[2, 1, 0], [0, 1, 2], [1, 1, 1]]), 3, L.MODE_AUTO, upload=False)
     sl = plan.slots()
-    assert sl[0] == sl[1]  # symmetric spectra: any side order maps to one slot
+    assert sl.shape == (3, 4) and (sl[:, 1:] == -1).all()  # one segment
+    assert np.array_equal(sl[0], sl[1])  # symmetric spectra: any side order maps to one slot
     L.call('bsk_triples_mfma_workspace', plan.handle, 10, ctypes.byref(n))
     # FP32 partial tiles per workgroup, then the reduction's FP64 sums over 16
     # workgroup ranges per slot
@@ -189,8 +190,9 @@ _LEVEL_PARTS = {0: (0, 1, 2, 3), 1: (0, 1), 2: (1, 2), 3: (1,)}
 def _emulate_pair_plan(plan, sets, tri, mode, nshell, part_sets=None):
     """FP64 numpy emulation of the MFMA kernel's tables: per launch item the
     16x16 tile sum_x (d_a d_b)(x) d_k(x) over the cells of its segment, stored
-    at its partial position by the first segment and added by the others,
-    read back through the triangle slots (scaled by 2^level).  part_sets[p]:
+    at its partial position (each position belongs to one segment), read
+    back through the triangle slots (one per segment, summed, scaled by
+    2^level).  part_sets[p]:
     the role fields on the cells of layout part p; without it the plan must be
     one segment of all parts, summed over `sets`."""
     items, outpos, prs = plan.tables()
@@ -246,18 +248,23 @@ def _emulate_pair_plan(plan, sets, tri, mode, nshell, part_sets=None):
                         k = 16 * kb + col
                         if w0 <= k < w0 + wn:  # other third-side rows are zero
                             tsum[row, col] = np.sum(pa * f2[k])
-                if g == 0:
-                    assert not written[pos]
-                    part[pos], written[pos] = tsum, True
-                else:
-                    assert written[pos]
-                    part[pos] += tsum
+                assert not written[pos]
+                part[pos], written[pos] = tsum, True
             launch += 1
             e0 += 8 * ni
     assert launch == plan.npass
+    assert written.all() or plan.nitem == 0
     flat = part.reshape(-1)
     slots = plan.slots()
-    return flat[slots & 0x0fffffff] * 2.0 ** ((slots >> 28) & 3)
+    assert slots.shape == (len(tri), 4) and (slots[:, 0] >= 0).all()
+    assert (slots[:, len(plan.segments):] == -1).all()
+    level = (slots[:, 0] >> 28) & 3
+    assert all(((s >> 28) & 3) == level[t] for t in range(len(tri)) for s in slots[t] if s >= 0)
+    tot = np.zeros(len(tri))
+    for g in range(4):
+        ok = slots[:, g] >= 0
+        tot[ok] += flat[slots[ok, g] & 0x0fffffff]
+    return tot * 2.0 ** level
 
 
 def _level_reference(part_sets, tri, levels):
