@@ -85,6 +85,23 @@ BSK_DEV void fetch(Pref<PV>& p, const uint16_t* __restrict__ gh, const uint16_t*
   }
 }
 
+// The same loads issued by every thread (threads past the staged rows
+// re-read row 0), so a wave always has the same number of loads in flight
+// and a wait for an older chunk's loads can leave a newer chunk's pending.
+template <int PV>
+BSK_DEV void fetch_all(Pref<PV>& p, const uint16_t* __restrict__ gh,
+                       const uint16_t* __restrict__ gl, long long stride, int n, long long cbase) {
+#pragma unroll
+  for (int k = 0; k < PV; ++k) {
+    int v = threadIdx.x + k * kNT;
+    v = v < n * kSEGS ? v : 0;
+    const int row = v / kSEGS, sg = v - row * kSEGS;
+    const long long g = (long long)row * stride + cbase + sg * 8;
+    p.h[k] = *reinterpret_cast<const uint4*>(gh + g);
+    p.l[k] = *reinterpret_cast<const uint4*>(gl + g);
+  }
+}
+
 // FP32 values d = hi + lo to f (if non-null) and raw planes to h/l (if
 // non-null) for rows < n.
 template <int PV>
@@ -124,15 +141,19 @@ BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l) {
 // without spilling, else 2; 8 for one item and one prefetch segment (at most
 // 64 VGPRs), whose launches stage few shells per chunk and need more
 // workgroups in flight.
-template <int NI, int PV, int SETS>
+template <int NI, int PV, int SETS, bool D2>
 constexpr int pair_occupancy() {
   if (NI == 1 && PV == 1 && SETS == 1) return 8;  // thin launches: 4 workgroups per CU
   constexpr int ni_max = SETS == 1 ? 4 : (SETS == 2 ? 3 : 1);
   return (NI <= ni_max && PV <= 2) ? 4 : 2;
 }
 
-template <int NI, int PV, int SETS>
-__global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_triple_pairs(PairArgs a) {
+// D2 (one field, one prefetch segment per thread): two chunks in flight
+// per workgroup instead of one, for segments whose few items leave the
+// staging loads' latency exposed (replicated wave lists).
+template <int NI, int PV, int SETS, bool D2>
+__global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS, D2>())) void k_triple_pairs(PairArgs a) {
+  static_assert(!D2 || (SETS == 1 && PV == 1), "two-chunk prefetch: one field, one segment");
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const int n0 = a.nsh[0], n1 = a.nsh[1], n2 = a.nsh[2];
   float* fA = reinterpret_cast<float*>(smem);         // (n0 + 1) rows
@@ -183,29 +204,63 @@ __global__ __launch_bounds__(kNT, (pair_occupancy<NI, PV, SETS>())) void k_tripl
   const long long nch = a.ncell / kCK;
   const long long ch0 = nch * lw / a.nwg;
   const long long ch1 = nch * (lw + 1) / a.nwg;
-  Pref<PV> p0, p1, p2;
-  if (ch0 < ch1) {
-    fetch(p0, a.hi[0], a.lo[0], a.stride, n0, ch0 * kCK);
-    if (SETS == 3) fetch(p1, a.hi[1], a.lo[1], a.stride, n1, ch0 * kCK);
-    if (SETS >= 2) fetch(p2, a.hi[2], a.lo[2], a.stride, n2, ch0 * kCK);
-  }
-  for (long long ch = ch0; ch < ch1; ++ch) {
-    __syncthreads();  // every wave is done with the previous chunk
-    put(p0, n0, fA, SETS == 1 ? hKs : nullptr, SETS == 1 ? lKs : nullptr);
-    if (SETS == 3) put(p1, n1, fB, nullptr, nullptr);
-    if (SETS >= 2) put(p2, n2, nullptr, hKs, lKs);
-    __syncthreads();
-    if (ch + 1 < ch1) {  // next chunk's HBM loads overlap this chunk's math
-      const long long cb = (ch + 1) * kCK;
-      fetch(p0, a.hi[0], a.lo[0], a.stride, n0, cb);
-      if (SETS == 3) fetch(p1, a.hi[1], a.lo[1], a.stride, n1, cb);
-      if (SETS >= 2) fetch(p2, a.hi[2], a.lo[2], a.stride, n2, cb);
-    }
+  auto ksteps = [&]() {
 #pragma unroll  // constant LDS offsets per K step
     for (int ks = 0; ks < kKSTEPS; ++ks)
       if (ks % rep == rgrp)  // uniform: this replica's K steps
         kstep<NI, kLDH>(st, desc, oij, fA, fB, hK, lK, ks * 32 + grp * 4, ks * 32 + grp * 8, col,
                         kc);
+  };
+  if constexpr (D2) {
+    // chunks alternate between two register sets, each refilled right after
+    // it is stored to LDS with the chunk two ahead (the last chunk again
+    // past the end: a read that hits L2, keeping every path's loads alike)
+    if (ch0 < ch1) {
+      const long long last = ch1 - 1;
+      Pref<PV> qa, qb;
+      fetch_all(qa, a.hi[0], a.lo[0], a.stride, n0, ch0 * kCK);
+      fetch_all(qb, a.hi[0], a.lo[0], a.stride, n0, (ch0 + 1 < last ? ch0 + 1 : last) * kCK);
+      long long ch = ch0;
+      for (; ch + 1 < ch1; ch += 2) {  // chunk pairs
+        __syncthreads();  // every wave is done with the previous chunk
+        put(qa, n0, fA, hKs, lKs);
+        __syncthreads();
+        fetch_all(qa, a.hi[0], a.lo[0], a.stride, n0, (ch + 2 < last ? ch + 2 : last) * kCK);
+        ksteps();
+        __syncthreads();
+        put(qb, n0, fA, hKs, lKs);
+        __syncthreads();
+        fetch_all(qb, a.hi[0], a.lo[0], a.stride, n0, (ch + 3 < last ? ch + 3 : last) * kCK);
+        ksteps();
+      }
+      if (ch < ch1) {  // odd count: the last chunk
+        __syncthreads();
+        put(qa, n0, fA, hKs, lKs);
+        __syncthreads();
+        ksteps();
+      }
+    }
+  } else {
+    Pref<PV> p0, p1, p2;
+    if (ch0 < ch1) {
+      fetch(p0, a.hi[0], a.lo[0], a.stride, n0, ch0 * kCK);
+      if (SETS == 3) fetch(p1, a.hi[1], a.lo[1], a.stride, n1, ch0 * kCK);
+      if (SETS >= 2) fetch(p2, a.hi[2], a.lo[2], a.stride, n2, ch0 * kCK);
+    }
+    for (long long ch = ch0; ch < ch1; ++ch) {
+      __syncthreads();  // every wave is done with the previous chunk
+      put(p0, n0, fA, SETS == 1 ? hKs : nullptr, SETS == 1 ? lKs : nullptr);
+      if (SETS == 3) put(p1, n1, fB, nullptr, nullptr);
+      if (SETS >= 2) put(p2, n2, nullptr, hKs, lKs);
+      __syncthreads();
+      if (ch + 1 < ch1) {  // next chunk's HBM loads overlap this chunk's math
+        const long long cb = (ch + 1) * kCK;
+        fetch(p0, a.hi[0], a.lo[0], a.stride, n0, cb);
+        if (SETS == 3) fetch(p1, a.hi[1], a.lo[1], a.stride, n1, cb);
+        if (SETS >= 2) fetch(p2, a.hi[2], a.lo[2], a.stride, n2, cb);
+      }
+      ksteps();
+    }
   }
   float* const out = a.partial + (long long)lw * a.item_total * 256;
   if (rep == 1) {
@@ -280,9 +335,9 @@ __global__ __launch_bounds__(256) void k_reduce_tri(const double* __restrict__ s
   out[t] = s * scale * f;
 }
 
-template <int NI, int PV, int SETS>
+template <int NI, int PV, int SETS, bool D2>
 int launch_pairs(const PairArgs& a, size_t lds, hipStream_t s) {
-  auto fn = k_triple_pairs<NI, PV, SETS>;
+  auto fn = k_triple_pairs<NI, PV, SETS, D2>;
   // once per instance (not per launch: keeps launches capturable and cheap)
   static const hipError_t attr = hipFuncSetAttribute(
       (const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -292,15 +347,24 @@ int launch_pairs(const PairArgs& a, size_t lds, hipStream_t s) {
   return BSK_OK;
 }
 
+// Replicated wave lists (few items per workgroup) with one field and one
+// prefetch segment per thread run the two-chunk prefetch.
+template <int NI, int PV, int SETS>
+int launch_d2(const PairArgs& a, size_t lds, hipStream_t s) {
+  if constexpr (SETS == 1 && PV == 1 && NI <= 4)
+    if (a.rep > 1) return launch_pairs<NI, PV, SETS, true>(a, lds, s);
+  return launch_pairs<NI, PV, SETS, false>(a, lds, s);
+}
+
 template <int PV, int SETS>
 int launch_ni(const PairArgs& a, int ni, size_t lds, hipStream_t s) {
   switch (ni) {
-    case 1: return launch_pairs<1, PV, SETS>(a, lds, s);
-    case 2: return launch_pairs<2, PV, SETS>(a, lds, s);
-    case 3: return launch_pairs<3, PV, SETS>(a, lds, s);
-    case 4: return launch_pairs<4, PV, SETS>(a, lds, s);
-    case 6: return launch_pairs<6, PV, SETS>(a, lds, s);
-    case 8: return launch_pairs<8, PV, SETS>(a, lds, s);
+    case 1: return launch_d2<1, PV, SETS>(a, lds, s);
+    case 2: return launch_d2<2, PV, SETS>(a, lds, s);
+    case 3: return launch_d2<3, PV, SETS>(a, lds, s);
+    case 4: return launch_d2<4, PV, SETS>(a, lds, s);
+    case 6: return launch_pairs<6, PV, SETS, false>(a, lds, s);
+    case 8: return launch_pairs<8, PV, SETS, false>(a, lds, s);
     default:
       bsk_set_error("mfma triple: %d items per wave not compiled", ni);
       return BSK_EUNSUPPORTED;
