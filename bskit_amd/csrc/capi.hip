@@ -1183,6 +1183,19 @@ int pairs_create(bsk_pairs** out, int mode, int nshell, int ntri, const int* tri
             break;
           }
         }
+      // a replicated segment with 3 or 4 items per wave: lists of at most 2
+      // items on half the waves when they fit, which lets three workgroups
+      // share a CU instead of two (its staging latency, not its math, binds)
+      if (level && sp[g].rep > 1 && ml > 2) {
+        int np = 0, m2 = 1;
+        auto b2 = bin_units(sp[g].unit_mask, 2, W / 2, &np, &m2);
+        if (np == 1 && m2 <= 2) {
+          sp[g].bins.clear();
+          for (int r = 0; r < 2; ++r) sp[g].bins.insert(sp[g].bins.end(), b2.begin(), b2.end());
+          sp[g].rep = 2;
+          ml = m2;
+        }
+      }
     }
     sp[g].ni = bsk_pair_items_round(ml);
     maxni = std::max(maxni, sp[g].ni);

@@ -144,6 +144,7 @@ BSK_DEV void put(const Pref<PV>& p, int n, float* f, uint16_t* h, uint16_t* l) {
 template <int NI, int PV, int SETS, bool D2>
 constexpr int pair_occupancy() {
   if (NI == 1 && PV == 1 && SETS == 1) return 8;  // thin launches: 4 workgroups per CU
+  if (NI == 2 && PV == 1 && SETS == 1) return 6;  // three workgroups per CU
   constexpr int ni_max = SETS == 1 ? 4 : (SETS == 2 ? 3 : 1);
   return (NI <= ni_max && PV <= 2) ? 4 : 2;
 }
@@ -351,7 +352,7 @@ int launch_pairs(const PairArgs& a, size_t lds, hipStream_t s) {
 // prefetch segment per thread run the two-chunk prefetch.
 template <int NI, int PV, int SETS>
 int launch_d2(const PairArgs& a, size_t lds, hipStream_t s) {
-  if constexpr (SETS == 1 && PV == 1 && NI <= 4)
+  if constexpr (SETS == 1 && PV == 1 && NI <= 4 && NI != 2)  // NI = 2: no registers to spare
     if (a.rep > 1) return launch_pairs<NI, PV, SETS, true>(a, lds, s);
   return launch_pairs<NI, PV, SETS, false>(a, lds, s);
 }
