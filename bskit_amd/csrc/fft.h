@@ -103,6 +103,33 @@ struct Dft<T, 5, INV> {
   }
 };
 
+// 10 = 2 x 5: E = DFT5 of the even inputs, O = DFT5 of the odd ones, then
+// X[k] = E[k] + w^k O[k] and X[k + 5] = E[k] - w^k O[k], w = exp(-+2 pi i / 10).
+template <typename T, bool INV>
+struct Dft<T, 10, INV> {
+  static BSK_DEV void run(cplx<T>* v) {
+    const T c1 = (T)0.80901699437494742410;  // cos(2 pi / 10)
+    const T s1 = (T)0.58778525229247312917;  // sin(2 pi / 10)
+    const T c2 = (T)0.30901699437494742410;  // cos(4 pi / 10)
+    const T s2 = (T)0.95105651629515357212;  // sin(4 pi / 10)
+    cplx<T> e[5] = {v[0], v[2], v[4], v[6], v[8]};
+    cplx<T> o[5] = {v[1], v[3], v[5], v[7], v[9]};
+    Dft<T, 5, INV>::run(e);
+    Dft<T, 5, INV>::run(o);
+    const T sg = INV ? (T)1 : (T)-1;  // sign of the twiddles' imaginary parts
+    const cplx<T> w[5] = {{(T)1, (T)0}, {c1, sg * s1}, {c2, sg * s2}, {-c2, sg * s2},
+                          {-c1, sg * s1}};
+    v[0] = cadd(e[0], o[0]);
+    v[5] = csub(e[0], o[0]);
+#pragma unroll
+    for (int k = 1; k < 5; ++k) {
+      const cplx<T> t = cmul(o[k], w[k]);
+      v[k] = cadd(e[k], t);
+      v[k + 5] = csub(e[k], t);
+    }
+  }
+};
+
 // Pencil stride in LDS: the next odd length >= L (complex elements), so
 // consecutive pencils start two (FP32) or four (FP64) banks apart and the
 // lanes of a wave, which walk consecutive pencils, hit distinct banks.
@@ -280,7 +307,7 @@ BSK_PLAN(64, 0, 8, 8) BSK_PLAN(64, 1, 4, 4, 4)
 BSK_PLAN(72, 0, 8, 3, 3) BSK_PLAN(72, 1, 4, 2, 3, 3)
 BSK_PLAN(80, 0, 8, 2, 5) BSK_PLAN(80, 1, 4, 4, 5)
 BSK_PLAN(96, 0, 8, 4, 3) BSK_PLAN(96, 1, 4, 4, 2, 3)
-BSK_PLAN(100, 0, 4, 5, 5) BSK_PLAN(100, 1, 2, 2, 5, 5)
+BSK_PLAN(100, 0, 4, 5, 5) BSK_PLAN(100, 1, 10, 10)
 BSK_PLAN(108, 0, 4, 3, 3, 3) BSK_PLAN(108, 1, 2, 2, 3, 3, 3)
 BSK_PLAN(120, 0, 8, 3, 5) BSK_PLAN(120, 1, 4, 2, 3, 5)
 BSK_PLAN(128, 0, 8, 4, 4) BSK_PLAN(128, 1, 4, 4, 4, 2)

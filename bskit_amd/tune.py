@@ -107,7 +107,19 @@ def _time(fn, reps=3):
     return a.elapsed_time(b) / reps
 
 
-def tune(eng, tri=None, field=None, reps=3, force=False, log=None, nfield=1):
+def _best_of(cands, reps, rounds=2):
+    """Per candidate (name -> callable) the least mean time over `rounds`
+    interleaved rounds of `reps` calls: a spike or drift during one round
+    does not decide between candidates a few percent apart."""
+    out = {}
+    for _ in range(rounds):
+        for name, fn in cands.items():
+            t = _time(fn, reps)
+            out[name] = min(out.get(name, t), t)
+    return out
+
+
+def tune(eng, tri=None, field=None, reps=4, force=False, log=None, nfield=1):
     """Time the candidates on this engine's shapes and store the best.
 
     field: a float32 device field of the engine's mesh (random if None);
@@ -129,30 +141,35 @@ def tune(eng, tri=None, field=None, reps=3, force=False, log=None, nfield=1):
         field = torch.randn((eng.N,) * 3, generator=g, device=eng.device)
     variants = [0] * 6
     timings = {}
+
+    def with_variants(first, combo, fn):
+        def run():
+            for p, v in enumerate(combo):
+                eng.set_variant(first + p, v)
+            return fn()
+        return run
+
     # forward passes 0..2
-    best = None
-    for combo in itertools.product((0, 1), repeat=3):
-        for p, v in enumerate(combo):
-            eng.set_variant(p, v)
-        t = _time(lambda: eng.forward(field), reps)
-        timings['forward%s' % ''.join(map(str, combo))] = t
-        if best is None or t < best[0]:
-            best = (t, combo)
-    variants[0:3] = best[1]
-    for p, v in enumerate(best[1]):
+    combos = list(itertools.product((0, 1), repeat=3))
+    t = _best_of({c: with_variants(0, c, lambda: eng.forward(field)) for c in combos}, reps)
+    best = min(combos, key=lambda c: t[c])
+    timings.update({'forward%s' % ''.join(map(str, c)): t[c] for c in combos})
+    variants[0:3] = best
+    for p, v in enumerate(best):
         eng.set_variant(p, v)
     dk = eng.forward(field)
     shells = eng.alloc_shells(L.FMT_BF16X2, eng.nb)
-    best = None
-    for combo in itertools.product((0, 1), repeat=3):
-        for p, v in enumerate(combo):
-            eng.set_variant(3 + p, v)
-        t = _time(lambda: eng.shells(dk, fmt=L.FMT_BF16X2, out=shells), reps)
-        timings['shells%s' % ''.join(map(str, combo))] = t
-        if best is None or t < best[0]:
-            best = (t, combo)
-    variants[3:6] = best[1]
-    for p, v in enumerate(best[1]):
+    # the shells as the pipeline writes them: with a level plan only the
+    # layout parts some launch reads
+    parts = (eng.shell_parts(tri, nfield)
+             if tri is not None and ntri and eng.uses_levels(L.FMT_BF16X2) else None)
+    t = _best_of({c: with_variants(3, c, lambda: eng.shells(dk, fmt=L.FMT_BF16X2, out=shells,
+                                                             parts=parts))
+                  for c in combos}, reps)
+    best = min(combos, key=lambda c: t[c])
+    timings.update({'shells%s' % ''.join(map(str, c)): t[c] for c in combos})
+    variants[3:6] = best
+    for p, v in enumerate(best):
         eng.set_variant(3 + p, v)
     cfg = {'variants': [int(v) for v in variants], 'timings_ms': timings}
     if tri is not None and ntri and eng.mfma_supported(eng.nb, nfield, tri):
@@ -161,28 +178,25 @@ def tune(eng, tri=None, field=None, reps=3, force=False, log=None, nfield=1):
         nchunk = eng.ncell // 128
         cands = sorted({n for n in (256, 512, 768, 1024, 1536, 2048) if n <= max(1, nchunk // 4)})
         cands = cands or [eng.mfma_nwg()]
-        best = None
         sets = [shells] * nfield
-        for n in cands:
-            t = _time(lambda: eng.triples(sets, tri, path='mfma', out=out, nwg=n), reps)
-            timings['mfma_nwg%d' % n] = t
-            if best is None or t < best[0]:
-                best = (t, n)
-        cfg['mfma_nwg'] = int(best[1])
+        t = _best_of({n: (lambda n=n: eng.triples(sets, tri, path='mfma', out=out, nwg=n))
+                      for n in cands}, reps)
+        timings.update({'mfma_nwg%d' % n: t[n] for n in cands})
+        cfg['mfma_nwg'] = int(min(cands, key=lambda n: t[n]))
         if nfield == 1 and eng.zfused_supported(tri):
-            best = None
-            for n in (256, 512, 768, 1024):
-                t = _time(lambda: eng.bispectrum_zfused(dk, tri, out=out, nwg=n), reps)
-                timings['zfused_nwg%d' % n] = t
-                if best is None or t < best[0]:
-                    best = (t, n)
-            cfg['zfused_nwg'] = int(best[1])
+            zc = (256, 512, 768, 1024)
+            t = _best_of({n: (lambda n=n: eng.bispectrum_zfused(dk, tri, out=out, nwg=n))
+                          for n in zc}, reps)
+            timings.update({'zfused_nwg%d' % n: t[n] for n in zc})
+            cfg['zfused_nwg'] = int(min(zc, key=lambda n: t[n]))
             # whole shells + sums stage, both ways, each at its best count
-            sep = _time(lambda: (eng.shells(dk, fmt=L.FMT_BF16X2, out=shells),
-                                 eng.triples([shells], tri, path='mfma', out=out,
-                                             nwg=cfg['mfma_nwg'])), reps)
-            fus = _time(lambda: eng.bispectrum_zfused(dk, tri, out=out, nwg=cfg['zfused_nwg']),
-                        reps)
+            t = _best_of({
+                'sep': lambda: (eng.shells(dk, fmt=L.FMT_BF16X2, out=shells, parts=parts),
+                                eng.triples([shells], tri, path='mfma', out=out,
+                                            nwg=cfg['mfma_nwg'])),
+                'fus': lambda: eng.bispectrum_zfused(dk, tri, out=out, nwg=cfg['zfused_nwg'])},
+                reps)
+            sep, fus = t['sep'], t['fus']
             timings['shells_triples_separate'] = sep
             timings['shells_triples_zfused'] = fus
             cfg['zfused'] = bool(fus < sep)

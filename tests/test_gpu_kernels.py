@@ -91,6 +91,30 @@ def test_delta_shells(N, nb, width, S):
     assert ebf < 3e-5
 
 
+@pytest.mark.parametrize('N,nb,width,S', [(256, 4, 16, 200), (96, 5, 2, 40), (128, 12, 2, 80)])
+def test_radix_variants_match_oracle(N, nb, width, S):
+    """Every FFT pass on its second radix plan (the autotuner's other choice,
+    e.g. the two radix-10 stages of the 100-point z transform at S = 200)
+    gives the oracle's forward spectrum and shells."""
+    from bskit_amd import _lib as L
+    box = 700.0
+    lo, hi = _bins(nb, width, box)
+    eng = _engine(N, box, lo, hi)
+    assert eng.S == S
+    for p in range(6):
+        eng.set_variant(p, 1)
+    f64 = _field(N, 5, np.float64)
+    dk_ref = est.forward(f64)
+    ref = np.stack(est.shells_on_mesh(dk_ref, N, S, eng.n2lo, eng.n2hi, 'delta')).reshape(nb, -1)
+    d32 = eng.forward(torch.from_numpy(f64.astype(np.float32)).cuda())
+    cref = _compact(dk_ref, eng)
+    efw = np.abs(d32.cpu().numpy() - cref).max() / np.abs(cref).max()
+    s32 = eng.shells(d32, fmt=L.FMT_F32).data.cpu().numpy()
+    e32 = np.abs(s32 - ref).max() / np.abs(ref).max()
+    print('variant 1 N=%d S=%d forward %.2e shells %.2e' % (N, S, efw, e32))
+    assert efw < 5e-6 and e32 < 5e-6
+
+
 @pytest.mark.parametrize('N,nb,width', [(32, 6, 2), (48, 5, 3), (64, 4, 2), (80, 6, 2)])
 def test_unit_and_kmag_shells(N, nb, width):
     from bskit_amd import _lib as L
