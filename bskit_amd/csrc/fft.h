@@ -285,7 +285,8 @@ BSK_DEV void fft_lds_n(cplx<T>* buf, const cplx<T>* tw, int np, unsigned mdiv, i
 }
 
 // ---------------------------------------------------------------- radix plans
-// Two variants per length: 0 = radix-8 heavy, 1 = radix-4 heavy.
+// Two variants per length: 0 = radix-8 heavy, 1 = radix-4 heavy (100: two
+// radix-10 stages, one LDS round trip fewer than 4 x 5 x 5).
 template <int L, int V>
 struct Plan;
 #define BSK_PLAN(L, V, ...) \
