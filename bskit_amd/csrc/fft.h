@@ -130,6 +130,39 @@ struct Dft<T, 10, INV> {
   }
 };
 
+// 20 = 4 x 5: for s < 4 the DFT5 of v[4m + s] (m < 5), twiddled by
+// w^(s k), w = exp(-+2 pi i / 20), then X[k + 5q] = DFT4 over s (k < 5).
+template <typename T, bool INV>
+struct Dft<T, 20, INV> {
+  static BSK_DEV void run(cplx<T>* v) {
+    const T a = (T)0.95105651629515357212;  // cos(pi / 10) = sin(2 pi / 5)
+    const T b = (T)0.30901699437494742410;  // sin(pi / 10) = cos(2 pi / 5)
+    const T c = (T)0.80901699437494742410;  // cos(pi / 5)
+    const T d = (T)0.58778525229247312917;  // sin(pi / 5)
+    // cos and sin of 2 pi j / 20 for the exponents j = s k used (s, k < 5)
+    const T wc[13] = {(T)1, a, c, d, b, 0, -b, 0, -c, -a, 0, 0, -c};
+    const T ws[13] = {0, b, d, c, a, 0, a, 0, d, b, 0, 0, -d};
+    const T sg = INV ? (T)1 : (T)-1;
+    cplx<T> x[4][5];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+      for (int m = 0; m < 5; ++m) x[s][m] = v[4 * m + s];
+      Dft<T, 5, INV>::run(x[s]);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      cplx<T> y[4] = {x[0][k], x[1][k], x[2][k], x[3][k]};
+#pragma unroll
+      for (int s = 1; s < 4; ++s)
+        if (k > 0) y[s] = cmul(y[s], cplx<T>{wc[s * k], sg * ws[s * k]});
+      Dft<T, 4, INV>::run(y);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[k + 5 * q] = y[q];
+    }
+  }
+};
+
 // Pencil stride in LDS: the next odd length >= L (complex elements), so
 // consecutive pencils start two (FP32) or four (FP64) banks apart and the
 // lanes of a wave, which walk consecutive pencils, hit distinct banks.
@@ -285,8 +318,8 @@ BSK_DEV void fft_lds_n(cplx<T>* buf, const cplx<T>* tw, int np, unsigned mdiv, i
 }
 
 // ---------------------------------------------------------------- radix plans
-// Two variants per length: 0 = radix-8 heavy, 1 = radix-4 heavy (100: two
-// radix-10 stages, one LDS round trip fewer than 4 x 5 x 5).
+// Two variants per length: 0 = radix-8 heavy, 1 = radix-4 heavy (100 and
+// 200: two stages of radix 10 and 20, one LDS round trip fewer).
 template <int L, int V>
 struct Plan;
 #define BSK_PLAN(L, V, ...) \
@@ -315,7 +348,7 @@ BSK_PLAN(128, 0, 8, 4, 4) BSK_PLAN(128, 1, 4, 4, 4, 2)
 BSK_PLAN(144, 0, 8, 2, 3, 3) BSK_PLAN(144, 1, 4, 4, 3, 3)
 BSK_PLAN(160, 0, 8, 4, 5) BSK_PLAN(160, 1, 4, 4, 2, 5)
 BSK_PLAN(192, 0, 8, 8, 3) BSK_PLAN(192, 1, 4, 4, 4, 3)
-BSK_PLAN(200, 0, 8, 5, 5) BSK_PLAN(200, 1, 4, 2, 5, 5)
+BSK_PLAN(200, 0, 8, 5, 5) BSK_PLAN(200, 1, 20, 10)
 BSK_PLAN(216, 0, 8, 3, 3, 3) BSK_PLAN(216, 1, 4, 2, 3, 3, 3)
 BSK_PLAN(240, 0, 8, 2, 3, 5) BSK_PLAN(240, 1, 4, 4, 3, 5)
 BSK_PLAN(256, 0, 8, 8, 4) BSK_PLAN(256, 1, 4, 4, 4, 4)
