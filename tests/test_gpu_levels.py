@@ -375,3 +375,45 @@ def test_grid_pipeline_cross_equals_one_at_a_time():
         pipe.submit(pairs[0][0])
     with pytest.raises(ValueError):
         GridPipeline(eng, tri, nfield=3)
+
+
+def _deep_lists(case):
+    """Triangle lists whose level-0 plan (forced) is one segment of few
+    items with 3-item units (third sides in three 16-shell blocks) over a
+    32-shell window (one prefetch segment per thread): replicated wave lists
+    of 3 or 4 items."""
+    if case == 3:
+        return [(7, 9, k) for k in range(5, 37)]
+    fam = [(i, j, k) for i in range(20, 28) for j in range(30, 34) for k in range(5, 37, 2)]
+    return fam + [(6, 8, k) for k in range(8, 16)]
+
+
+@pytest.mark.parametrize('ni', [3, 4])
+def test_replicated_lists_with_two_staged_chunks(monkeypatch, ni):
+    """Segments with replicated 3- and 4-item wave lists run the pair kernel
+    with two staged chunks in flight per workgroup; over the whole mesh
+    (every triangle at level 0) their sums match the FP64 plain path."""
+    from bskit_amd import _lib as L
+    N, nb, box = 128, 40, 1000.0
+    lo, hi = _bins(nb, 1, box)
+    eng = _engine(N, box, lo, hi)
+    assert eng.layout is not None
+    tri = np.array(_deep_lists(ni), np.int32)
+    monkeypatch.setattr(eng, 'triangle_levels', lambda t: np.zeros(len(t), np.int32))
+    plan = eng.pair_plan(tri, nb, 0, levels=True)
+    parts, first, count, npass, items, rep = plan.segments[0]
+    assert len(plan.segments) == 1 and (parts, npass, items, rep) == (15, 1, ni, 4)
+    assert count <= 32  # one prefetch segment per thread: the two-chunk path
+    sc = 1.0 / eng.ncell
+    f = _field(N, 61, np.float64)
+    d32 = eng.forward(torch.from_numpy(f.astype(np.float32)).cuda())
+    lvl = eng.shells(d32, fmt=L.FMT_BF16X2, out=eng.alloc_shells(L.FMT_BF16X2, nb, True))
+    got = eng.triples([lvl], tri, path='mfma', scale=sc).cpu().numpy()
+    s64 = eng.shells(eng.forward(torch.from_numpy(f).cuda()), fmt=L.FMT_F64, dtype=torch.float64)
+    ref = eng.triples([s64], tri, path='plain', scale=sc).cpu().numpy()
+    a = s64.data
+    absum = torch.stack([(a[i].abs() * a[j].abs() * a[k].abs()).sum()
+                         for i, j, k in tri]).cpu().numpy() * sc
+    err = (np.abs(got - ref) / absum).max()
+    print('replicated %d-item lists: segments %s, vs fp64 %.2e' % (ni, plan.segments, err))
+    assert err < 5e-5
