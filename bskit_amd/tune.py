@@ -107,7 +107,7 @@ def _time(fn, reps=3):
     return a.elapsed_time(b) / reps
 
 
-def _best_of(cands, reps, rounds=2):
+def _best_of(cands, reps, rounds=3):
     """Per candidate (name -> callable) the least mean time over `rounds`
     interleaved rounds of `reps` calls: a spike or drift during one round
     does not decide between candidates a few percent apart."""
