@@ -213,16 +213,21 @@ def test_level_mfma_matches_one_level_and_fp64(N, nb, width, mode):
     assert e2r < 5e-5
 
 
-@pytest.mark.parametrize('nfield', [1, 2])
-def test_part_masked_shells_give_the_same_sums(nfield):
+@pytest.mark.parametrize('nfield,zvar', [(1, 0), (2, 0), (1, 1), (2, 1)])
+def test_part_masked_shells_give_the_same_sums(nfield, zvar):
     """Shells written only in the layout parts the plan reads: the MFMA sums
     are bit-identical to those over fully written shells, the parts left out
-    keep what the buffer held, and a plan that reads more is refused."""
+    keep what the buffer held, and a plan that reads more is refused.  zvar
+    1: the z pass on its 10 x 10 plan (also with the rows gathered for
+    E/O-only shells), whose sums match the 4 x 5 x 5 plan's to FP32
+    rounding."""
     from bskit_amd import _lib as L
     from bskit_amd import bins
     N, nb, box = 512, 32, 1000.0
     lo, hi = _bins(nb, 2, box)
     eng = _engine(N, box, lo, hi)
+    assert eng.S == 200
+    eng.set_variant(5, zvar)
     tri = bins.closed_triangle_indices(np.stack([lo, hi], 1), nfield).astype(np.int32)
     parts = eng.shell_parts(tri, nfield)
     assert parts.host.shape == (nb,) and (parts.host & 2).all()  # E: every shell
@@ -241,6 +246,15 @@ def test_part_masked_shells_give_the_same_sums(nfield):
     a = eng.triples(full, tri, path='mfma', scale=sc).cpu().numpy()
     b = eng.triples(masked, tri, path='mfma', scale=sc).cpu().numpy()
     assert np.array_equal(a, b)
+    if zvar:
+        eng.set_variant(5, 0)
+        ref = [eng.shells(d, fmt=L.FMT_BF16X2, out=eng.alloc_shells(L.FMT_BF16X2, nb, True),
+                          parts=parts) for d in d32]
+        c = eng.triples(ref, tri, path='mfma', scale=sc).cpu().numpy()
+        eng.set_variant(5, zvar)
+        dv = np.abs(b - c).max() / np.abs(c).max()
+        print('z plans 10x10 vs 4x5x5: max |dB| / max |B| = %.2e' % dv)
+        assert dv < 1e-5
     # parts not written keep the fill; written parts equal the full shells
     lay = eng.layout
     pc = lay.part_cells()
