@@ -568,8 +568,8 @@ def main():
     # by HIP events around its launches inside every timed step.  The
     # separate path sums each triangle over the sparsest sublattice of the
     # shell mesh on which its sum is exact (level L: S^3 / 2^L cells), one
-    # launch per pass of each level; figures are per launch (grid totals /
-    # launches).
+    # launch per segment of the level layout (its item-list passes folded
+    # into it); figures are per launch (grid totals / launches).
     N = wl['N']
     ncell = float(eng.ncell)                     # shell-mesh cells S^3
     nsh = eng.nb
